@@ -1,0 +1,341 @@
+// dev_render.h — kernel bodies of the MI355X render path, templated on the world-field
+// implementation (`TableMap` walks the compiled scene tables; generated maps specialise it).
+//
+//   render_tile      one lane per (pixel, AA sample): march + shade + reflection chain
+//                    (lib.rs:1223-1360); the block's samples meet in LDS and the first lane of
+//                    each pixel sums them in the reference's sy-major order.
+//   gather_rays      GI bake rays (lib.rs:686-711): one lane per (probe, face, ray).
+//   gi_reduce        weighted mean per cube face (gi.rs:327-341).
+//   bloom_h / bloom_v_resolve / resolve_only   post.rs:36-115 + framebuffer.rs:102-113.
+//   primary_aov      centre-pixel primary hit (film.rs:114-118, 164-165).
+//   field_probe / march_probe   map / raw-march parity probes (wgsl.rs:641-646 twin).
+#pragma once
+#include "dev_math.h"
+
+namespace mm3e {
+
+MM3E_DEV void camera_ray(const FrameParams& fp, float sx, float sy, V3& ro, V3& rd) {  // camera.rs:59-65
+  const float w = (float)fp.width, h = (float)fp.height;
+  float ndc_x = (2.0f * sx / w - 1.0f) * fp.aspect * fp.fov_scale;
+  float ndc_y = (1.0f - 2.0f * sy / h) * fp.fov_scale;
+  rd = normalize(ld3(fp.fwd) + scale(ld3(fp.right), ndc_x) + scale(ld3(fp.up), ndc_y));
+  ro = ld3(fp.eye);
+}
+
+// One surface interaction: everything of shade_hit (lib.rs:1286-1360) except the reflection
+// recursion, which the caller chains.  Returns the pre-reflection radiance `base`.
+template <class Map>
+MM3E_DEV V3 shade_surface(const DevScene& sc, const Map& field, const Hit& h, V3 n, V3 rd, const DMat& m) {
+  V3 albedo = surface_albedo(m, h.pos);
+  V3 view = scale(rd, -1.0f);
+  float occ = 1.0f;
+  if (sc.flags & SC_AO) occ = ambient_occlusion(sc, field, h.pos, n);
+  V3 gi = V3{0.0f, 0.0f, 0.0f};
+  if (sc.flags & SC_HAS_GI) gi = gi_sample(sc, h.pos, n);
+  V3 amb = ibl_irradiance(sc, n) + gi + ld3(sc.ambient);
+  V3 radiance = scale(cmul(albedo, amb), occ);
+  const uint32_t nl = sc.n_light;
+  for (uint32_t li = 0; li < nl; li++) {
+    const DLight& l = sc.light[li];
+    V3 ld;
+    float dist;
+    light_toward(l, h.pos, ld, dist);
+    if (lambert(n, ld) <= 0.0f) continue;
+    float shadow = 1.0f;
+    if (sc.flags & SC_SHADOWS) {
+      float max_t = rmin(dist, sc.max_dist);
+      float lift = rmax(sc.normal_h * 2.0f, 0.01f);
+      shadow = soft_shadow(sc, field, h.pos + scale(n, lift), ld, max_t, light_shadow_k(l, dist));
+    }
+    if (shadow <= 0.0f) continue;
+    V3 surf = brdf(n, ld, view, albedo, m.metallic, m.roughness, m.specular);
+    radiance = radiance + scale(cmul(surf, ld3(l.color)), shadow * light_atten(l, dist));
+  }
+  return radiance;
+}
+
+// trace (lib.rs:1265-1282).  The reflection recursion is bounded at compile time by MAXB
+// (the scene's `bounces`), so each level is a separate inlined copy and no per-level state
+// goes to scratch: out = fog(base*(1-fr) + trace(reflected)*fr + emissive).
+template <class Map, int D, int MAXB>
+MM3E_DEV V3 trace_level(const DevScene& sc, const Map& field, V3 ro, V3 rd) {
+  Hit h = march(sc, field, ro, rd);
+  if (!h.hit) return sky(rd, ld3(sc.sun));
+  V3 n = hit_normal(sc, field, h.pos);
+  const DMat& m = sc.mat[h.mat % sc.n_mat];
+  V3 radiance = shade_surface(sc, field, h, n, rd, m);
+  if constexpr (D < MAXB) {
+    if ((uint32_t)D < sc.bounces && m.reflectivity > 0.0f) {
+      V3 view = scale(rd, -1.0f);
+      float c = rmax(dot(n, view), 0.0f);
+      float fr = fresnel_schlick(c, m.reflectivity);
+      V3 rdir = normalize(reflect(rd, n));
+      V3 rorg = h.pos + scale(n, rmax(sc.normal_h * 2.0f, 0.02f));
+      V3 refl = trace_level<Map, D + 1, MAXB>(sc, field, rorg, rdir);
+      radiance = scale(radiance, 1.0f - fr) + scale(refl, fr);
+    }
+  }
+  radiance = radiance + ld3(m.emissive);
+  return apply_fog(radiance, ld3(sc.fog), h.t, sc.fog_density);
+}
+
+template <class Map, int MAXB>
+MM3E_DEV V3 trace_path(const DevScene& sc, const Map& field, V3 ro, V3 rd) {
+  return trace_level<Map, 0, MAXB>(sc, field, ro, rd);
+}
+
+// Render one tile: blockDim.x = tw*th*aa*aa lanes; lane -> (pixel in tile, sample).
+template <class Map, int MAXB>
+MM3E_DEV void render_tile(const DevScene* __restrict__ scp, const FrameParams& fp, float* __restrict__ hdr,
+                          uint32_t tw, uint32_t th, uint32_t tile_x, uint32_t tile_y, float* smem) {
+  const DevScene& sc = *scp;
+  Map field{scp};
+  const uint32_t aa = sc.aa;
+  const uint32_t ns = aa * aa;
+  const uint32_t t = threadIdx.x;
+  const uint32_t pl = t / ns, s = t - pl * ns;
+  const uint32_t px = tile_x * tw + pl % tw, py = tile_y * th + pl / tw;
+  const bool valid = pl < tw * th && px < fp.width && py < fp.height;
+  V3 rad = V3{0.0f, 0.0f, 0.0f};
+  if (valid) {
+    const uint32_t sy = s / aa, sx = s - sy * aa;
+    const float ox = ((float)sx + 0.5f) / (float)aa;
+    const float oy = ((float)sy + 0.5f) / (float)aa;
+    V3 ro, rd;
+    camera_ray(fp, (float)px + ox, (float)py + oy, ro, rd);
+    rad = trace_path<Map, MAXB>(sc, field, ro, rd);
+  }
+  if (ns == 1) {
+    if (valid) {
+      float* o = hdr + 3 * ((size_t)py * fp.width + px);
+      // acc = ZERO + s0; acc.scale(1/1)
+      V3 acc = V3{0.0f, 0.0f, 0.0f} + rad;
+      o[0] = acc.x * 1.0f; o[1] = acc.y * 1.0f; o[2] = acc.z * 1.0f;
+    }
+    return;
+  }
+  smem[3 * t + 0] = rad.x;
+  smem[3 * t + 1] = rad.y;
+  smem[3 * t + 2] = rad.z;
+  __syncthreads();
+  if (valid && s == 0) {
+    V3 acc = V3{0.0f, 0.0f, 0.0f};
+    for (uint32_t k = 0; k < ns; k++) {
+      const float* q = smem + 3 * (t + k);
+      acc = acc + V3{q[0], q[1], q[2]};
+    }
+    const float inv = 1.0f / (float)ns;
+    acc = scale(acc, inv);
+    float* o = hdr + 3 * ((size_t)py * fp.width + px);
+    o[0] = acc.x; o[1] = acc.y; o[2] = acc.z;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// GI bake (gi.rs:55-125 with the lib.rs:686-711 gather)
+// ---------------------------------------------------------------------------------------
+struct GiBakeParams {
+  float lo[3], pad0;
+  float cell[3], pad1;
+  uint32_t dims[3];
+  uint32_t rays_per_face;  // 1 or 5
+  float dirs[6][5][4];     // axis_dirs (gi.rs:38-50): xyz + weight, computed on the host
+};
+
+template <class Map>
+MM3E_DEV V3 gather(const DevScene& sc, const Map& field, V3 o, V3 d) {
+  Hit h = march(sc, field, o, d);
+  V3 sun = ld3(sc.sun);
+  if (!h.hit) return sky_diffuse(d, sun);
+  V3 n = hit_normal(sc, field, h.pos);
+  const DMat& m = sc.mat[h.mat % sc.n_mat];
+  V3 albedo = surface_albedo(m, h.pos);
+  V3 c = ld3(m.emissive);
+  const uint32_t nl = sc.n_light;
+  for (uint32_t li = 0; li < nl; li++) {
+    const DLight& l = sc.light[li];
+    V3 ld;
+    float dist;
+    light_toward(l, h.pos, ld, dist);
+    float ndl = lambert(n, ld);
+    if (ndl > 0.0f) {
+      V3 outg = brdf(n, ld, scale(d, -1.0f), albedo, m.metallic, m.roughness, m.specular);
+      c = c + scale(cmul(outg, ld3(l.color)), light_atten(l, dist));
+    }
+  }
+  return c;
+}
+
+// One lane per (probe, face, ray) of probes [first_probe, first_probe + n_probes); radiance is
+// written to out[((probe - first_probe)*6 + face)*R + ray].
+template <class Map>
+MM3E_DEV void gi_gather_rays(const DevScene* __restrict__ scp, const GiBakeParams& bp, float4* __restrict__ out,
+                             uint32_t first_probe, uint32_t n_probes) {
+  const uint32_t R = bp.rays_per_face;
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t per_probe = 6 * R;
+  const uint32_t local = gid / per_probe;
+  if (local >= n_probes) return;
+  const uint32_t probe = first_probe + local;
+  const uint32_t rem = gid - local * per_probe;
+  const uint32_t face = rem / R, ray = rem - face * R;
+  const uint32_t nx = bp.dims[0], ny = bp.dims[1];
+  const uint32_t i = probe % nx, j = (probe / nx) % ny, k = probe / (nx * ny);
+  V3 o = V3{bp.lo[0], bp.lo[1], bp.lo[2]} + V3{(float)i * bp.cell[0], (float)j * bp.cell[1], (float)k * bp.cell[2]};
+  V3 d = V3{bp.dirs[face][ray][0], bp.dirs[face][ray][1], bp.dirs[face][ray][2]};
+  Map field{scp};
+  V3 c = gather(*scp, field, o, d);
+  out[((size_t)local * 6 + face) * R + ray] = float4{c.x, c.y, c.z, 0.0f};
+}
+
+MM3E_DEV void gi_reduce(const GiBakeParams& bp, const float4* __restrict__ rays, float4* __restrict__ cubes,
+                        uint32_t n_probes) {
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= n_probes * 6) return;
+  const uint32_t face = gid % 6;
+  const uint32_t R = bp.rays_per_face;
+  V3 acc = V3{0.0f, 0.0f, 0.0f};
+  float wsum = 0.0f;
+  for (uint32_t r = 0; r < R; r++) {
+    float4 v = rays[(size_t)gid * R + r];
+    float w = bp.dirs[face][r][3];
+    acc = acc + scale(V3{v.x, v.y, v.z}, w);
+    wsum = wsum + w;
+  }
+  V3 c = scale(acc, 1.0f / rmax(wsum, 1e-4f));
+  cubes[gid] = float4{c.x, c.y, c.z, 0.0f};
+}
+
+// ---------------------------------------------------------------------------------------
+// Post (post.rs:36-115): bright-pass fused into the horizontal blur, vertical blur fused
+// with bloom add + exposure + ACES + gamma + 8-bit quantisation.
+// ---------------------------------------------------------------------------------------
+struct PostParams {
+  uint32_t width, height, radius, bloom;
+  float threshold, intensity, exposure, pad;
+};
+
+MM3E_DEV V3 bright_pass(V3 c, float thr) {
+  float l = c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f;
+  if (l > thr) return scale(c, (l - thr) / rmax(l, 1e-4f));
+  return V3{0.0f, 0.0f, 0.0f};
+}
+
+// Block = 256 lanes along x of one row; LDS holds the bright-pass of the row segment + halo.
+MM3E_DEV void bloom_h(const PostParams& pp, const float* __restrict__ kern, const float* __restrict__ hdr,
+                      float* __restrict__ tmp, float* lds) {
+  const int W = (int)pp.width, r = (int)pp.radius;
+  const int y = blockIdx.y;
+  const int x0 = blockIdx.x * blockDim.x;
+  const int span = (int)blockDim.x + 2 * r;
+  for (int i = threadIdx.x; i < span; i += blockDim.x) {
+    int sx = x0 - r + i;
+    sx = sx < 0 ? 0 : (sx > W - 1 ? W - 1 : sx);
+    const float* s = hdr + 3 * ((size_t)y * W + sx);
+    V3 b = bright_pass(V3{s[0], s[1], s[2]}, pp.threshold);
+    lds[3 * i + 0] = b.x; lds[3 * i + 1] = b.y; lds[3 * i + 2] = b.z;
+  }
+  __syncthreads();
+  const int x = x0 + threadIdx.x;
+  if (x >= W) return;
+  V3 acc = V3{0.0f, 0.0f, 0.0f};
+  const int n = 2 * r + 1;
+  for (int k = 0; k < n; k++) {
+    // LDS slot i holds the bright-pass of column clamp(x0 - r + i), so tap k of column x
+    // (source column clamp(x + k - r)) is slot x - x0 + k
+    const float* q = lds + 3 * (x - x0 + k);
+    acc = acc + scale(V3{q[0], q[1], q[2]}, kern[k]);
+  }
+  float* o = tmp + 3 * ((size_t)y * W + x);
+  o[0] = acc.x; o[1] = acc.y; o[2] = acc.z;
+}
+
+MM3E_DEV uint8_t to_u8(float c) {  // framebuffer.rs:102-113: (clamp(c,0,1)*255 + 0.5) as u8
+  float v = rclamp(c, 0.0f, 1.0f) * 255.0f + 0.5f;
+  return (uint8_t)(v >= 255.0f ? 255.0f : v);
+}
+
+MM3E_DEV uchar4 tonemap_px(const PostParams& pp, V3 c) {
+  c = scale(c, pp.exposure);
+  V3 g = V3{gamma1(aces1(c.x)), gamma1(aces1(c.y)), gamma1(aces1(c.z))};
+  g = clamp01(g);
+  return uchar4{to_u8(g.x), to_u8(g.y), to_u8(g.z), (uint8_t)255};
+}
+
+// One lane per pixel; the vertical taps read the horizontally blurred image through L2.
+MM3E_DEV void bloom_v_resolve(const PostParams& pp, const float* __restrict__ kern, const float* __restrict__ hdr,
+                              const float* __restrict__ tmp, uchar4* __restrict__ rgba) {
+  const int W = (int)pp.width, H = (int)pp.height, r = (int)pp.radius;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= W) return;
+  V3 acc = V3{0.0f, 0.0f, 0.0f};
+  const int n = 2 * r + 1;
+  for (int k = 0; k < n; k++) {
+    int sy = y + k - r;
+    sy = sy < 0 ? 0 : (sy > H - 1 ? H - 1 : sy);
+    const float* q = tmp + 3 * ((size_t)sy * W + x);
+    acc = acc + scale(V3{q[0], q[1], q[2]}, kern[k]);
+  }
+  const float* s = hdr + 3 * ((size_t)y * W + x);
+  V3 c = V3{s[0], s[1], s[2]} + scale(acc, pp.intensity);
+  rgba[(size_t)y * W + x] = tonemap_px(pp, c);
+}
+
+MM3E_DEV void resolve_only(const PostParams& pp, const float* __restrict__ hdr, uchar4* __restrict__ rgba) {
+  const int W = (int)pp.width;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= W) return;
+  const float* s = hdr + 3 * ((size_t)y * W + x);
+  rgba[(size_t)y * W + x] = tonemap_px(pp, V3{s[0], s[1], s[2]});
+}
+
+// ---------------------------------------------------------------------------------------
+// parity probes
+// ---------------------------------------------------------------------------------------
+template <class Map>
+MM3E_DEV void primary_aov(const DevScene* __restrict__ scp, const FrameParams& fp, uint8_t* __restrict__ hit,
+                          uint32_t* __restrict__ mat, float* __restrict__ depth, uint32_t* __restrict__ steps) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= fp.width) return;
+  Map field{scp};
+  V3 ro, rd;
+  camera_ray(fp, (float)x + 0.5f, (float)y + 0.5f, ro, rd);
+  Hit h = march(*scp, field, ro, rd);
+  const size_t i = (size_t)y * fp.width + x;
+  hit[i] = h.hit ? 1 : 0;
+  mat[i] = h.hit ? h.mat : 0xffffffffu;
+  depth[i] = h.hit ? dot(h.pos - ld3(fp.eye), ld3(fp.fwd)) : __builtin_inff();
+  if (steps) steps[i] = h.steps;
+}
+
+template <class Map>
+MM3E_DEV void field_probe(const DevScene* __restrict__ scp, const float* __restrict__ pts, uint32_t n,
+                          float* __restrict__ dist, uint32_t* __restrict__ mat) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Map field{scp};
+  Field f = field(V3{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]});
+  dist[i] = f.dist;
+  mat[i] = f.mat;
+}
+
+template <class Map>
+MM3E_DEV void march_probe(const DevScene* __restrict__ scp, const float* __restrict__ rays, uint32_t n,
+                          float* __restrict__ t, uint8_t* __restrict__ hit, uint32_t* __restrict__ mat,
+                          float* __restrict__ nrm, uint32_t* __restrict__ steps) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Map field{scp};
+  const float* r = rays + 6 * i;
+  Hit h = march(*scp, field, V3{r[0], r[1], r[2]}, V3{r[3], r[4], r[5]});
+  V3 nn = h.hit ? hit_normal(*scp, field, h.pos) : V3{0.0f, 0.0f, 0.0f};
+  t[i] = h.t;
+  hit[i] = h.hit ? 1 : 0;
+  mat[i] = h.mat;
+  nrm[3 * i] = nn.x; nrm[3 * i + 1] = nn.y; nrm[3 * i + 2] = nn.z;
+  steps[i] = h.steps;
+}
+
+}  // namespace mm3e

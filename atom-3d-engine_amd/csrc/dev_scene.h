@@ -1,0 +1,145 @@
+// dev_scene.h — the compiled scene as it lives in HBM (one allocation per compiled scene).
+//
+// Plain-old-data shared by the host compiler (scene_compile.cpp) and the device code
+// (dev_math.h / dev_render.h).  One `DevScene` header is followed by its tables; every table
+// pointer in the header is a device address inside the same allocation, so a kernel needs
+// exactly one pointer argument to reach the whole scene.  Scalar fields are read through
+// that uniform pointer and land in SGPRs (s_load); per-object records are 16-byte aligned so
+// a wave walking the object table in lock-step issues dwordx4 scalar/vector loads.
+#pragma once
+#include <stdint.h>
+
+namespace mm3e {
+
+// DObj::flags bit layout
+enum : uint32_t {
+  OBJ_KIND_MASK = 0xFu,      // mm3e_prim_kind
+  OBJ_MIRROR_X = 1u << 4,
+  OBJ_MIRROR_Y = 1u << 5,
+  OBJ_MIRROR_Z = 1u << 6,
+  OBJ_ELONGATE = 1u << 7,
+  OBJ_REPEAT = 1u << 8,
+  OBJ_TWIST = 1u << 9,
+  OBJ_BEND = 1u << 10,
+  OBJ_ROUND = 1u << 11,
+  OBJ_ONION = 1u << 12,
+  OBJ_BOUNDED = 1u << 13,
+  OBJ_COMBINE_SHIFT = 16,    // bits 16-17: mm3e_combine of this object
+};
+
+// One placed object (mm3e-orchestrator/src/lib.rs:330-336) with its world bounding sphere
+// (lib.rs:470-496) precomputed on the host.
+struct alignas(16) DObj {
+  float c0[3];  // Transform::rot columns (to_local takes dot(c_i, p - pos))
+  float inv;    // 1/scale, or 0 for |scale| <= 1e-12 (vec.rs:219)
+  float c1[3];
+  float scale;
+  float c2[3];
+  uint32_t mat;
+  float pos[3];
+  uint32_t flags;
+  float p[8];       // prim parameters (p[0] holds the volume id bits for volumes)
+  float rep[3];
+  float twist;
+  float elo[3];
+  float bend;
+  float round, onion;
+  float bc_r;       // bounding-sphere radius
+  float k;          // smooth-union radius (Combine::Smooth)
+  float bc[3];      // bounding-sphere centre
+  uint32_t pad1;
+};
+
+// accel.rs:13-19 plus a parent link so the device walks the tree without a stack.
+struct alignas(16) DNode {
+  float c[3];
+  float r;
+  int32_t left, right, leaf, parent;
+};
+
+// accel.rs:122-125: Segment::One(index) or Segment::Run{tree, unbounded}.
+struct alignas(16) DSeg {
+  uint32_t kind;        // 0 = one object, 1 = run of plain unions
+  uint32_t index;       // one: object index; run: root node index (global)
+  uint32_t has_tree;
+  uint32_t unb_base;    // run: first entry in the unbounded-object index list
+  uint32_t unb_count;
+  uint32_t combine;     // one: the object's Combine
+  float k;              // one: smooth radius
+  uint32_t pad;
+};
+
+struct alignas(16) DMat {  // mm3e-kit/src/color.rs:37-45
+  float albedo[3];
+  float metallic;
+  float specular;
+  float roughness;
+  float reflectivity;
+  uint32_t checker;
+  float emissive[3];
+  uint32_t pad;
+};
+
+struct alignas(16) DLight {  // lib.rs:503-510
+  float vec[3];
+  float radius;
+  float color[3];
+  uint32_t directional;
+};
+
+struct alignas(16) DVol {  // mm3e-kit/src/volume.rs:14-23
+  uint32_t dims[3];
+  uint32_t offset;   // first sample inside vol_data
+  float min[3];
+  uint32_t valid;    // volume.rs:47-62 has_valid_layout
+  float cell[3];
+  uint32_t pad;
+  float hi[3];       // min + (dims-1)*cell (volume.rs:97-103)
+  uint32_t pad2;
+};
+
+enum : uint32_t {
+  SC_SHADOWS = 1u << 0,
+  SC_AO = 1u << 1,
+  SC_DUAL_SAFE = 1u << 2,
+  SC_SECANT = 1u << 3,
+  SC_HAS_GI = 1u << 4,
+  SC_SS1 = 1u << 5,  // step_scale >= 1: enhanced sphere tracing with omega = 1.4
+};
+
+struct alignas(16) DevScene {
+  uint32_t n_obj, n_seg, n_mat, n_light;
+  uint32_t max_steps, shadow_steps, ao_samples, bounces;
+  uint32_t flags, n_vol, aa, pad0;
+  uint32_t gi_dims[3], pad1;
+  float max_dist, eps, step_scale, lod;
+  float subitize, normal_h, slack, fog_density;
+  float sun[3], exposure;
+  float ambient[3], bloom_threshold;
+  float sky_ambient[3], bloom_intensity;
+  float fog[3], pad2;
+  float gi_min[3], pad3;
+  float gi_size[3], pad4;
+  // tables (device addresses inside this allocation)
+  const DObj* obj;
+  const DSeg* seg;
+  const DNode* nodes;
+  const uint32_t* unb;
+  const DMat* mat;
+  const DLight* light;
+  const float* gi;       // [probe][6 faces][4] (RGB + pad), probe = (k*ny + j)*nx + i
+  const DVol* vol;
+  const float* vol_data;
+};
+
+// Per-frame camera record (mm3e-kit/src/camera.rs:39-47) plus the image size.
+struct FrameParams {
+  float eye[3], fov_scale;
+  float fwd[3], pad0;
+  float right[3], pad1;
+  float up[3], pad2;
+  uint32_t width, height;
+  float aspect, pad3;
+};
+
+}  // namespace mm3e

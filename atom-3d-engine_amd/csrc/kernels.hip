@@ -1,0 +1,125 @@
+// kernels.hip — ahead-of-time gfx950 kernels over the table-driven world field (GI bake,
+// post, parity probes; the render variants live in render_kernel.hip), plus the host-side
+// launch helpers the C++ runtime calls.  Built with
+//   hipcc --offload-arch=gfx950 -O3 -ffp-contract=off
+//         -Xclang -target-feature -Xclang -packed-fp32-ops
+#include <hip/hip_runtime.h>
+
+#include "dev_render.h"
+#include "launch.h"
+
+using namespace mm3e;
+
+extern "C" __global__ void __launch_bounds__(256)
+    mm3e_gi_gather_table(const DevScene* __restrict__ sc, GiBakeParams bp, float4* __restrict__ out,
+                         uint32_t first_probe, uint32_t n_probes) {
+  gi_gather_rays<TableMap>(sc, bp, out, first_probe, n_probes);
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+    mm3e_gi_reduce(GiBakeParams bp, const float4* __restrict__ rays, float4* __restrict__ cubes, uint32_t n_probes) {
+  gi_reduce(bp, rays, cubes, n_probes);
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+    mm3e_bloom_h(PostParams pp, const float* __restrict__ kern, const float* __restrict__ hdr, float* __restrict__ tmp) {
+  extern __shared__ float lds[];
+  bloom_h(pp, kern, hdr, tmp, lds);
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+    mm3e_bloom_v_resolve(PostParams pp, const float* __restrict__ kern, const float* __restrict__ hdr,
+                         const float* __restrict__ tmp, uchar4* __restrict__ rgba) {
+  bloom_v_resolve(pp, kern, hdr, tmp, rgba);
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+    mm3e_resolve_only(PostParams pp, const float* __restrict__ hdr, uchar4* __restrict__ rgba) {
+  resolve_only(pp, hdr, rgba);
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+    mm3e_primary_aov_table(const DevScene* __restrict__ sc, FrameParams fp, uint8_t* __restrict__ hit,
+                           uint32_t* __restrict__ mat, float* __restrict__ depth, uint32_t* __restrict__ steps) {
+  primary_aov<TableMap>(sc, fp, hit, mat, depth, steps);
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+    mm3e_field_probe_table(const DevScene* __restrict__ sc, const float* __restrict__ pts, uint32_t n,
+                           float* __restrict__ dist, uint32_t* __restrict__ mat) {
+  field_probe<TableMap>(sc, pts, n, dist, mat);
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+    mm3e_march_probe_table(const DevScene* __restrict__ sc, const float* __restrict__ rays, uint32_t n,
+                           float* __restrict__ t, uint8_t* __restrict__ hit, uint32_t* __restrict__ mat,
+                           float* __restrict__ nrm, uint32_t* __restrict__ steps) {
+  march_probe<TableMap>(sc, rays, n, t, hit, mat, nrm, steps);
+}
+
+// ------------------------------------------------------------------------------------------
+// launch helpers (host)
+// ------------------------------------------------------------------------------------------
+namespace mm3e {
+
+hipError_t launch_render_table(uint32_t maxb, dim3 grid, dim3 block, hipStream_t st, const DevScene* sc,
+                               const FrameParams& fp, float* hdr, uint32_t tw, uint32_t th) {
+  switch (maxb) {
+    case 0: return launch_render_table_b0(grid, block, st, sc, fp, hdr, tw, th);
+    case 1: return launch_render_table_b1(grid, block, st, sc, fp, hdr, tw, th);
+    case 2: return launch_render_table_b2(grid, block, st, sc, fp, hdr, tw, th);
+    case 3: return launch_render_table_b3(grid, block, st, sc, fp, hdr, tw, th);
+    case 4: return launch_render_table_b4(grid, block, st, sc, fp, hdr, tw, th);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_gi_gather_table(dim3 grid, dim3 block, hipStream_t st, const DevScene* sc, const GiBakeParams& bp,
+                                  float4* out, uint32_t first_probe, uint32_t n_probes) {
+  hipLaunchKernelGGL(mm3e_gi_gather_table, grid, block, 0, st, sc, bp, out, first_probe, n_probes);
+  return hipGetLastError();
+}
+
+hipError_t launch_gi_reduce(dim3 grid, dim3 block, hipStream_t st, const GiBakeParams& bp, const float4* rays,
+                            float4* cubes, uint32_t n_probes) {
+  hipLaunchKernelGGL(mm3e_gi_reduce, grid, block, 0, st, bp, rays, cubes, n_probes);
+  return hipGetLastError();
+}
+
+hipError_t launch_post(hipStream_t st, const PostParams& pp, const float* kern, const float* hdr, float* tmp,
+                       uchar4* rgba) {
+  const uint32_t bx = 256;
+  dim3 grid((pp.width + bx - 1) / bx, pp.height);
+  if (pp.bloom) {
+    size_t lds = (size_t)(bx + 2 * pp.radius) * 3 * sizeof(float);
+    hipLaunchKernelGGL(mm3e_bloom_h, grid, dim3(bx), lds, st, pp, kern, hdr, tmp);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(mm3e_bloom_v_resolve, grid, dim3(bx), 0, st, pp, kern, hdr, tmp, rgba);
+  } else {
+    hipLaunchKernelGGL(mm3e_resolve_only, grid, dim3(bx), 0, st, pp, hdr, rgba);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_primary_aov_table(hipStream_t st, const DevScene* sc, const FrameParams& fp, uint8_t* hit,
+                                    uint32_t* mat, float* depth, uint32_t* steps) {
+  dim3 grid((fp.width + 255) / 256, fp.height);
+  hipLaunchKernelGGL(mm3e_primary_aov_table, grid, dim3(256), 0, st, sc, fp, hit, mat, depth, steps);
+  return hipGetLastError();
+}
+
+hipError_t launch_field_probe_table(hipStream_t st, const DevScene* sc, const float* pts, uint32_t n, float* dist,
+                                    uint32_t* mat) {
+  hipLaunchKernelGGL(mm3e_field_probe_table, dim3((n + 255) / 256), dim3(256), 0, st, sc, pts, n, dist, mat);
+  return hipGetLastError();
+}
+
+hipError_t launch_march_probe_table(hipStream_t st, const DevScene* sc, const float* rays, uint32_t n, float* t,
+                                    uint8_t* hit, uint32_t* mat, float* nrm, uint32_t* steps) {
+  hipLaunchKernelGGL(mm3e_march_probe_table, dim3((n + 255) / 256), dim3(256), 0, st, sc, rays, n, t, hit, mat, nrm,
+                     steps);
+  return hipGetLastError();
+}
+
+}  // namespace mm3e

@@ -1,0 +1,37 @@
+// launch.h — host-side launchers for the ahead-of-time kernels in kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "dev_scene.h"
+
+namespace mm3e {
+
+struct GiBakeParams;
+struct PostParams;
+
+#define MM3E_DECL_RENDER(B)                                                                          \
+  hipError_t launch_render_table_b##B(dim3 grid, dim3 block, hipStream_t st, const DevScene* sc, \
+                                      const FrameParams& fp, float* hdr, uint32_t tw, uint32_t th);
+MM3E_DECL_RENDER(0)
+MM3E_DECL_RENDER(1)
+MM3E_DECL_RENDER(2)
+MM3E_DECL_RENDER(3)
+MM3E_DECL_RENDER(4)
+#undef MM3E_DECL_RENDER
+
+hipError_t launch_render_table(uint32_t maxb, dim3 grid, dim3 block, hipStream_t st, const DevScene* sc,
+                               const FrameParams& fp, float* hdr, uint32_t tw, uint32_t th);
+hipError_t launch_gi_gather_table(dim3 grid, dim3 block, hipStream_t st, const DevScene* sc, const GiBakeParams& bp,
+                                  float4* out, uint32_t first_probe, uint32_t n_probes);
+hipError_t launch_gi_reduce(dim3 grid, dim3 block, hipStream_t st, const GiBakeParams& bp, const float4* rays,
+                            float4* cubes, uint32_t n_probes);
+hipError_t launch_post(hipStream_t st, const PostParams& pp, const float* kern, const float* hdr, float* tmp,
+                       uchar4* rgba);
+hipError_t launch_primary_aov_table(hipStream_t st, const DevScene* sc, const FrameParams& fp, uint8_t* hit,
+                                    uint32_t* mat, float* depth, uint32_t* steps);
+hipError_t launch_field_probe_table(hipStream_t st, const DevScene* sc, const float* pts, uint32_t n, float* dist,
+                                    uint32_t* mat);
+hipError_t launch_march_probe_table(hipStream_t st, const DevScene* sc, const float* rays, uint32_t n, float* t,
+                                    uint8_t* hit, uint32_t* mat, float* nrm, uint32_t* steps);
+
+}  // namespace mm3e

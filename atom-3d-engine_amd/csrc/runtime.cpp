@@ -1,0 +1,702 @@
+// runtime.cpp — the C ABI of libmm3e_hip.so: renderer / compiled-scene objects, device-side
+// GI bake, resident framebuffers and per-frame launches.
+//
+// Mirrors the reference GPU backend surface (mm3e-gpu/src/lib.rs): GpuRenderer::{new,
+// with_adapter, adapter_name, wait_idle, compile_checked} and GpuScene::{render_only,
+// render_rgba(_dyn), render}.  A compiled scene owns one HBM allocation with its tables, its
+// GI probes, and its framebuffers (HDR, bloom scratch, RGBA8) plus a pinned host staging
+// buffer, so a frame is: one camera record by value -> render kernel -> 2 post kernels ->
+// one D2H copy.  All work goes to one default-priority stream per renderer.
+#include "runtime.h"
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "launch.h"
+
+namespace mm3e {
+
+void set_err(char* err, size_t n, const std::string& s) {
+  if (err && n) std::snprintf(err, n, "%s", s.c_str());
+}
+
+bool hip_ok(hipError_t e, const char* what, std::string* msg) {
+  if (e == hipSuccess) return true;
+  if (msg) *msg = std::string(what) + ": " + hipGetErrorString(e);
+  return false;
+}
+
+static std::string describe(int dev) {
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, dev) != hipSuccess) return "unknown";
+  return std::string(p.name) + " (" + p.gcnArchName + ")";
+}
+
+static std::string lower(std::string s) {
+  for (char& c : s) c = (char)std::tolower((unsigned char)c);
+  return s;
+}
+
+FrameParams make_frame(const mm3e_camera& c, uint32_t w, uint32_t h) {
+  FrameParams fp;
+  std::memset(&fp, 0, sizeof fp);
+  std::memcpy(fp.eye, c.eye, 12);
+  fp.fov_scale = c.fov_scale;
+  std::memcpy(fp.fwd, c.forward, 12);
+  std::memcpy(fp.right, c.right, 12);
+  std::memcpy(fp.up, c.up, 12);
+  fp.width = w;
+  fp.height = h;
+  fp.aspect = (float)w / (float)h;  // camera.rs:60
+  return fp;
+}
+
+// Tile shape for one block of <= 256 lanes: tw*th pixels x aa^2 samples.
+void tile_shape(uint32_t aa, uint32_t* tw, uint32_t* th) {
+  uint32_t ns = aa * aa;
+  uint32_t P = std::max(1u, 256u / ns);
+  uint32_t t = (uint32_t)std::floor(std::sqrt((double)P));
+  if (t == 0) t = 1;
+  *th = t;
+  *tw = std::max(1u, P / t);
+}
+
+// ---- device GI bake (gi.rs:55-125, lib.rs:665-711) ---------------------------------------
+bool bake_gi_device(mm3e_scene* s, std::string& msg) {
+  const mm3e_ir_header& h = s->host.hdr;
+  float lo[3], hi[3];
+  if (!gi_bounds(s->host, lo, hi)) return true;  // nothing bounded: no GI (gi.rs:192-196)
+  const uint32_t nx = std::max(h.gi_dims[0], 1u), ny = std::max(h.gi_dims[1], 1u), nz = std::max(h.gi_dims[2], 1u);
+  const uint64_t n_probes = (uint64_t)nx * ny * nz;
+  GiBakeParams bp;
+  std::memset(&bp, 0, sizeof bp);
+  float size[3];
+  for (int a = 0; a < 3; a++) {
+    size[a] = hi[a] - lo[a];
+    bp.lo[a] = lo[a];
+  }
+  const uint32_t dd[3] = {nx, ny, nz};
+  for (int a = 0; a < 3; a++) {
+    bp.cell[a] = size[a] / (float)(std::max(dd[a], 2u) - 1);
+    bp.dims[a] = dd[a];
+  }
+  gi_axis_dirs(h.gi_samples, bp.dirs, &bp.rays_per_face);
+  const uint32_t R = bp.rays_per_face;
+  float* d_gi = nullptr;
+  float4* d_rays = nullptr;
+  if (!hip_ok(hipMalloc(&d_gi, n_probes * 24 * sizeof(float)), "hipMalloc(gi)", &msg)) return false;
+  s->d_gi = d_gi;
+  // bounded scratch: at most 2^20 rays in flight per pass
+  const uint64_t probes_per_pass = std::max<uint64_t>(1, (1u << 20) / (6 * R));
+  const uint64_t pass_probes = std::min<uint64_t>(probes_per_pass, n_probes);
+  if (!hip_ok(hipMalloc(&d_rays, pass_probes * 6 * R * sizeof(float4)), "hipMalloc(gi rays)", &msg)) return false;
+  hipStream_t st = s->r->stream;
+  bool ok = true;
+  for (uint64_t p0 = 0; p0 < n_probes && ok; p0 += pass_probes) {
+    const uint32_t np = (uint32_t)std::min<uint64_t>(pass_probes, n_probes - p0);
+    const uint64_t lanes = (uint64_t)np * 6 * R;
+    ok = hip_ok(launch_gi_gather_table(dim3((uint32_t)((lanes + 255) / 256)), dim3(256), st, s->d_scene, bp, d_rays,
+                                       (uint32_t)p0, np),
+                "gi gather launch", &msg);
+    if (!ok) break;
+    ok = hip_ok(launch_gi_reduce(dim3((np * 6 + 255) / 256), dim3(256), st, bp, d_rays,
+                                 reinterpret_cast<float4*>(d_gi) + (size_t)p0 * 6, np),
+                "gi reduce launch", &msg);
+  }
+  if (ok) ok = hip_ok(hipStreamSynchronize(st), "gi bake", &msg);
+  hipFree(d_rays);
+  if (!ok) return false;
+  s->gi_dims[0] = nx; s->gi_dims[1] = ny; s->gi_dims[2] = nz;
+  for (int a = 0; a < 3; a++) {
+    s->gi_min[a] = lo[a];
+    s->gi_size[a] = size[a];
+  }
+  s->has_gi = true;
+  return true;
+}
+
+static void patch_gi(mm3e_scene* s, DevScene& d) {
+  if (!s->has_gi) return;
+  d.flags |= SC_HAS_GI;
+  d.gi = s->d_gi;
+  for (int a = 0; a < 3; a++) {
+    d.gi_dims[a] = s->gi_dims[a];
+    d.gi_min[a] = s->gi_min[a];
+    d.gi_size[a] = s->gi_size[a];
+  }
+}
+
+bool upload_tables(mm3e_scene* s, const std::vector<mm3e_ir_object>& xo, const std::vector<mm3e_ir_material>& xm,
+                   uint8_t** d_buf, size_t* cap, DevScene* hdr_out, std::string& msg) {
+  const uint64_t need = blob_size(s->host, xo.size(), xm.size());
+  if (*d_buf == nullptr || *cap < need) {
+    if (*d_buf) hipFree(*d_buf);
+    *d_buf = nullptr;
+    if (!hip_ok(hipMalloc((void**)d_buf, need), "hipMalloc(scene tables)", &msg)) return false;
+    *cap = need;
+  }
+  Blob b;
+  build_blob(s->host, xo, xm, s->aa, s->bounces, (uint64_t)(uintptr_t)*d_buf, s->d_gi, false, b);
+  patch_gi(s, b.header);
+  std::memcpy(b.bytes.data(), &b.header, sizeof b.header);
+  if (!hip_ok(hipMemcpyAsync(*d_buf, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice, s->r->stream),
+              "upload scene tables", &msg))
+    return false;
+  if (!hip_ok(hipStreamSynchronize(s->r->stream), "upload scene tables", &msg)) return false;
+  if (hdr_out) *hdr_out = b.header;
+  return true;
+}
+
+bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std::string& msg) {
+  hipStream_t st = s->r->stream;
+  dim3 grid((fp.width + s->tw - 1) / s->tw, (fp.height + s->th - 1) / s->th);
+  dim3 block(s->tw * s->th * s->aa * s->aa);
+  hipEvent_t* ev = s->ev[s->frames % mm3e_scene::kRing];
+  hipEventRecord(ev[0], st);
+  if (!hip_ok(launch_render_table(s->bounces, grid, block, st, dsc, fp, s->d_hdr, s->tw, s->th), "render launch", &msg))
+    return false;
+  hipEventRecord(ev[1], st);
+  if (!hip_ok(launch_post(st, s->pp, s->d_kern, s->d_hdr, s->d_tmp, s->d_rgba), "post launch", &msg)) return false;
+  hipEventRecord(ev[2], st);
+  s->frames++;
+  s->timing_valid = true;
+  return true;
+}
+
+}  // namespace mm3e
+
+using namespace mm3e;
+
+extern "C" {
+
+int mm3e_list_adapters(char* buf, size_t buflen) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  std::string all;
+  for (int i = 0; i < n; i++) all += (i ? " | " : "") + describe(i);
+  set_err(buf, buflen, all);
+  return n;
+}
+
+int mm3e_renderer_create(const char* adapter_filter, mm3e_renderer** out, char* err, size_t errlen) {
+  if (!out) return MM3E_ERR_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    set_err(err, errlen, "no GPU adapter: no HIP device is visible");
+    return MM3E_ERR_DEVICE;
+  }
+  std::string filter;
+  if (adapter_filter) filter = adapter_filter;
+  else if (const char* env = std::getenv("MM3E_GPU_ADAPTER")) filter = env;  // lib.rs:81-84
+  while (!filter.empty() && std::isspace((unsigned char)filter.back())) filter.pop_back();
+  while (!filter.empty() && std::isspace((unsigned char)filter.front())) filter.erase(filter.begin());
+  int dev = -1;
+  if (!filter.empty()) {
+    std::string want = lower(filter), names;
+    for (int i = 0; i < n; i++) {
+      std::string d = describe(i);
+      names += (i ? " | " : "") + d;
+      if (dev < 0 && lower(d).find(want) != std::string::npos) dev = i;
+    }
+    if (dev < 0) {  // lib.rs:100
+      set_err(err, errlen, "no GPU adapter matching '" + filter + "'; available: " + names);
+      return MM3E_ERR_INVALID;
+    }
+  } else {
+    dev = 0;
+  }
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, dev) != hipSuccess) {
+    set_err(err, errlen, "request_device failed: hipGetDeviceProperties");
+    return MM3E_ERR_DEVICE;
+  }
+  if (std::strncmp(p.gcnArchName, "gfx950", 6) != 0) {
+    set_err(err, errlen, std::string("request_device failed: libmm3e_hip is built for gfx950, device is ") + p.gcnArchName);
+    return MM3E_ERR_DEVICE;
+  }
+  auto* r = new mm3e_renderer();
+  r->device = dev;
+  r->name = describe(dev);
+  std::string msg;
+  if (!hip_ok(hipSetDevice(dev), "hipSetDevice", &msg) || !hip_ok(hipStreamCreate(&r->stream), "hipStreamCreate", &msg)) {
+    set_err(err, errlen, "request_device failed: " + msg);
+    delete r;
+    return MM3E_ERR_DEVICE;
+  }
+  *out = r;
+  return MM3E_OK;
+}
+
+const char* mm3e_renderer_adapter_name(const mm3e_renderer* r) { return r ? r->name.c_str() : ""; }
+
+int mm3e_renderer_wait_idle(mm3e_renderer* r) {
+  if (!r) return MM3E_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(r->mu);
+  hipSetDevice(r->device);
+  return hipStreamSynchronize(r->stream) == hipSuccess ? MM3E_OK : MM3E_ERR_DEVICE;
+}
+
+void mm3e_renderer_destroy(mm3e_renderer* r) {
+  if (!r) return;
+  hipSetDevice(r->device);
+  hipStreamSynchronize(r->stream);
+  hipStreamDestroy(r->stream);
+  delete r;
+}
+
+int mm3e_scene_compile(mm3e_renderer* r, const void* ir, size_t ir_len, uint32_t w, uint32_t h, mm3e_scene** out,
+                       char* err, size_t errlen) {
+  if (!r || !out) return MM3E_ERR_INVALID;
+  *out = nullptr;
+  if (w == 0 || h == 0 || w > 16384 || h > 16384) {  // lib.rs:139-141
+    set_err(err, errlen, "GpuRenderer::compile: " + std::to_string(w) + "x" + std::to_string(h) + " out of range 1..=16384");
+    return MM3E_ERR_INVALID;
+  }
+  auto* s = new mm3e_scene();
+  std::string msg;
+  if (!parse_scene_ir(ir, ir_len, s->host, msg)) {
+    set_err(err, errlen, msg);
+    delete s;
+    return MM3E_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lk(r->mu);
+  hipSetDevice(r->device);
+  s->r = r;
+  s->w = w;
+  s->h = h;
+  s->aa = std::max(s->host.hdr.aa, 1u);  // lib.rs:282
+  s->bounces = s->host.hdr.bounces;
+  tile_shape(s->aa, &s->tw, &s->th);
+  bool ok = true;
+  if (s->host.hdr.gi_mode == MM3E_GI_BAKED) {
+    // the baked cubes travel inside the table blob; the header points at them
+    s->has_gi = true;
+    for (int a = 0; a < 3; a++) {
+      s->gi_dims[a] = s->host.hdr.gi_dims[a];
+      s->gi_min[a] = s->host.hdr.gi_min[a];
+      s->gi_size[a] = s->host.hdr.gi_size[a];
+    }
+  }
+  // tables first (the GI bake marches this scene, without GI)
+  DevScene hdr;
+  if (s->host.hdr.gi_mode == MM3E_GI_BAKED) {
+    // point d_gi at the in-blob array after upload
+    const uint64_t need = blob_size(s->host, 0, 0);
+    ok = hip_ok(hipMalloc((void**)&s->d_tables, need), "hipMalloc(scene tables)", &msg);
+    if (ok) {
+      s->tables_cap = need;
+      Blob b;
+      build_blob(s->host, {}, {}, s->aa, s->bounces, (uint64_t)(uintptr_t)s->d_tables, nullptr, false, b);
+      s->d_gi_in_blob = (float*)(s->d_tables + b.gi_offset);
+      s->d_gi = s->d_gi_in_blob;
+      patch_gi(s, b.header);
+      std::memcpy(b.bytes.data(), &b.header, sizeof b.header);
+      ok = hip_ok(hipMemcpy(s->d_tables, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice), "upload tables", &msg);
+      hdr = b.header;
+    }
+  } else {
+    ok = upload_tables(s, {}, {}, &s->d_tables, &s->tables_cap, &hdr, msg);
+    if (ok && s->host.hdr.gi_mode == MM3E_GI_BAKE) {
+      s->d_scene = reinterpret_cast<const DevScene*>(s->d_tables);
+      ok = bake_gi_device(s, msg);
+      if (ok && s->has_gi) ok = upload_tables(s, {}, {}, &s->d_tables, &s->tables_cap, &hdr, msg);
+    }
+  }
+  s->d_scene = reinterpret_cast<const DevScene*>(s->d_tables);
+  s->hdr = hdr;
+  // post: bloom kernel weights + params (post.rs:36-58)
+  const mm3e_ir_header& ih = s->host.hdr;
+  std::vector<float> kern = gaussian_kernel(ih.bloom_radius);
+  s->pp.width = w;
+  s->pp.height = h;
+  s->pp.radius = (uint32_t)(kern.size() / 2);
+  s->pp.bloom = (ih.bloom && ih.bloom_intensity > 0.0f) ? 1u : 0u;
+  s->pp.threshold = ih.bloom_threshold;
+  s->pp.intensity = ih.bloom_intensity;
+  s->pp.exposure = ih.exposure;
+  const size_t npx = (size_t)w * h;
+  if (ok) ok = hip_ok(hipMalloc(&s->d_kern, kern.size() * 4), "hipMalloc(kernel)", &msg);
+  if (ok) ok = hip_ok(hipMemcpy(s->d_kern, kern.data(), kern.size() * 4, hipMemcpyHostToDevice), "upload kernel", &msg);
+  if (ok) ok = hip_ok(hipMalloc(&s->d_hdr, npx * 12), "hipMalloc(hdr)", &msg);
+  if (ok) ok = hip_ok(hipMalloc(&s->d_tmp, npx * 12), "hipMalloc(bloom)", &msg);
+  if (ok) ok = hip_ok(hipMalloc(&s->d_rgba, npx * 4), "hipMalloc(rgba)", &msg);
+  if (ok) ok = hip_ok(hipHostMalloc((void**)&s->h_rgba, npx * 4, hipHostMallocDefault), "hipHostMalloc(rgba)", &msg);
+  for (int f = 0; ok && f < mm3e_scene::kRing; f++)
+    for (int i = 0; ok && i < 3; i++) ok = hip_ok(hipEventCreate(&s->ev[f][i]), "hipEventCreate", &msg);
+  if (!ok) {
+    set_err(err, errlen, msg);
+    mm3e_scene_destroy(s);
+    return MM3E_ERR_DEVICE;
+  }
+  *out = s;
+  return MM3E_OK;
+}
+
+int mm3e_scene_check(const void* ir, size_t ir_len, uint32_t w, uint32_t h, char* err, size_t errlen) {
+  if (w == 0 || h == 0 || w > 16384 || h > 16384) {  // lib.rs:139-141
+    set_err(err, errlen, "GpuRenderer::compile: " + std::to_string(w) + "x" + std::to_string(h) + " out of range 1..=16384");
+    return MM3E_ERR_INVALID;
+  }
+  HostScene hs;
+  std::string msg;
+  if (!parse_scene_ir(ir, ir_len, hs, msg)) {
+    set_err(err, errlen, msg);
+    return MM3E_ERR_INVALID;
+  }
+  return MM3E_OK;
+}
+
+void mm3e_scene_destroy(mm3e_scene* s) {
+  if (!s) return;
+  if (s->r) {
+    hipSetDevice(s->r->device);
+    hipStreamSynchronize(s->r->stream);
+  }
+  if (s->d_tables) hipFree(s->d_tables);
+  if (s->d_dyn) hipFree(s->d_dyn);
+  if (s->d_gi && s->d_gi != s->d_gi_in_blob) hipFree(s->d_gi);
+  if (s->d_kern) hipFree(s->d_kern);
+  if (s->d_hdr) hipFree(s->d_hdr);
+  if (s->d_tmp) hipFree(s->d_tmp);
+  if (s->d_rgba) hipFree(s->d_rgba);
+  if (s->h_rgba) hipHostFree(s->h_rgba);
+  for (int f = 0; f < mm3e_scene::kRing; f++)
+    for (int i = 0; i < 3; i++)
+      if (s->ev[f][i]) hipEventDestroy(s->ev[f][i]);
+  delete s;
+}
+
+static int validate_cam(const mm3e_camera* c) { return c ? MM3E_OK : MM3E_ERR_INVALID; }
+
+// Dynamic spheres as appended objects with Material{albedo, metallic, ..default}
+// (mm3e-cpu-parallel/src/lib.rs:50-84); at most MM3E_MAX_DYN (mm3e-gpu/src/lib.rs:306-307).
+static bool dyn_scene(mm3e_scene* s, const mm3e_dyn_sphere* dyn, uint32_t n_dyn, const DevScene** out,
+                      std::string& msg) {
+  n_dyn = std::min<uint32_t>(n_dyn, MM3E_MAX_DYN);
+  std::vector<mm3e_ir_object> xo;
+  std::vector<mm3e_ir_material> xm;
+  const uint32_t base_mat = (uint32_t)s->host.materials.size();
+  for (uint32_t i = 0; i < n_dyn; i++) {
+    const mm3e_dyn_sphere& d = dyn[i];
+    char buf[160];
+    if (!(std::isfinite(d.pos[0]) && std::isfinite(d.pos[1]) && std::isfinite(d.pos[2]))) {
+      std::snprintf(buf, sizeof buf, "dynamic sphere %u: position must be finite", i);
+      msg = buf;
+      return false;
+    }
+    if (!std::isfinite(d.radius) || d.radius <= 0.0f) {
+      std::snprintf(buf, sizeof buf, "dynamic sphere %u: radius must be finite and positive", i);
+      msg = buf;
+      return false;
+    }
+    const float v[4] = {d.albedo[0], d.albedo[1], d.albedo[2], d.metallic};
+    for (float x : v)
+      if (!std::isfinite(x) || x < 0.0f || x > 1.0f) {
+        std::snprintf(buf, sizeof buf, "dynamic sphere %u: linear albedo and metallic must be finite and in [0,1]", i);
+        msg = buf;
+        return false;
+      }
+    mm3e_ir_object o;
+    std::memset(&o, 0, sizeof o);
+    o.prim = MM3E_PRIM_SPHERE;
+    o.mat = base_mat + i;
+    o.combine = MM3E_COMBINE_UNION;
+    o.p[0] = d.radius;
+    o.rot[0] = o.rot[4] = o.rot[8] = 1.0f;
+    std::memcpy(o.pos, d.pos, 12);
+    o.scale = 1.0f;
+    xo.push_back(o);
+    mm3e_ir_material m;
+    std::memset(&m, 0, sizeof m);
+    std::memcpy(m.albedo, d.albedo, 12);
+    m.metallic = d.metallic;
+    m.specular = 0.5f;   // color.rs:64-75 Material::default
+    m.roughness = 0.35f;
+    xm.push_back(m);
+  }
+  if (!upload_tables(s, xo, xm, &s->d_dyn, &s->dyn_cap, nullptr, msg)) return false;
+  *out = reinterpret_cast<const DevScene*>(s->d_dyn);
+  return true;
+}
+
+int mm3e_scene_render_only(mm3e_scene* s, const mm3e_camera* cam) {
+  if (!s || validate_cam(cam)) return MM3E_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->r->mu);
+  hipSetDevice(s->r->device);
+  std::string msg;
+  return launch_frame(s, s->d_scene, make_frame(*cam, s->w, s->h), msg) ? MM3E_OK : MM3E_ERR_DEVICE;
+}
+
+int mm3e_scene_render_rgba(mm3e_scene* s, const mm3e_camera* cam, const mm3e_dyn_sphere* dyn, uint32_t n_dyn,
+                           uint8_t* out, size_t out_len) {
+  if (!s || validate_cam(cam) || (n_dyn && !dyn)) return MM3E_ERR_INVALID;
+  const size_t need = (size_t)s->w * s->h * 4;
+  if (out && out_len < need) return MM3E_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->r->mu);
+  hipSetDevice(s->r->device);
+  std::string msg;
+  const DevScene* dsc = s->d_scene;
+  if (n_dyn && !dyn_scene(s, dyn, n_dyn, &dsc, msg)) return MM3E_ERR_INVALID;
+  if (!launch_frame(s, dsc, make_frame(*cam, s->w, s->h), msg)) return MM3E_ERR_DEVICE;
+  if (!hip_ok(hipMemcpyAsync(s->h_rgba, s->d_rgba, need, hipMemcpyDeviceToHost, s->r->stream), "readback", &msg))
+    return MM3E_ERR_DEVICE;
+  if (!hip_ok(hipStreamSynchronize(s->r->stream), "frame", &msg)) return MM3E_ERR_DEVICE;
+  if (out) std::memcpy(out, s->h_rgba, need);
+  return MM3E_OK;
+}
+
+int mm3e_scene_render_linear(mm3e_scene* s, const mm3e_camera* cam, float* out, size_t out_floats) {
+  if (!s || validate_cam(cam) || !out) return MM3E_ERR_INVALID;
+  const size_t need = (size_t)s->w * s->h * 3;
+  if (out_floats < need) return MM3E_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->r->mu);
+  hipSetDevice(s->r->device);
+  std::string msg;
+  if (!launch_frame(s, s->d_scene, make_frame(*cam, s->w, s->h), msg)) return MM3E_ERR_DEVICE;
+  if (!hip_ok(hipMemcpyAsync(out, s->d_hdr, need * 4, hipMemcpyDeviceToHost, s->r->stream), "readback", &msg))
+    return MM3E_ERR_DEVICE;
+  return hip_ok(hipStreamSynchronize(s->r->stream), "frame", &msg) ? MM3E_OK : MM3E_ERR_DEVICE;
+}
+
+int mm3e_scene_last_timing(mm3e_scene* s, float* render_ms, float* post_ms) {
+  if (!s || !s->timing_valid || s->frames == 0) return MM3E_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->r->mu);
+  hipSetDevice(s->r->device);
+  hipEvent_t* ev = s->ev[(s->frames - 1) % mm3e_scene::kRing];
+  if (hipEventSynchronize(ev[2]) != hipSuccess) return MM3E_ERR_DEVICE;
+  float a = 0, b = 0;
+  hipEventElapsedTime(&a, ev[0], ev[1]);
+  hipEventElapsedTime(&b, ev[1], ev[2]);
+  if (render_ms) *render_ms = a;
+  if (post_ms) *post_ms = b;
+  return MM3E_OK;
+}
+
+int mm3e_scene_timing(mm3e_scene* s, int reset, uint64_t* frames, double* render_ms_sum, double* post_ms_sum) {
+  if (!s) return MM3E_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->r->mu);
+  hipSetDevice(s->r->device);
+  const uint64_t n = std::min<uint64_t>(s->frames, mm3e_scene::kRing);
+  double ra = 0.0, pa = 0.0;
+  if (n) {
+    if (hipStreamSynchronize(s->r->stream) != hipSuccess) return MM3E_ERR_DEVICE;
+    for (uint64_t k = s->frames - n; k < s->frames; k++) {
+      hipEvent_t* ev = s->ev[k % mm3e_scene::kRing];
+      float a = 0, b = 0;
+      if (hipEventElapsedTime(&a, ev[0], ev[1]) != hipSuccess) return MM3E_ERR_DEVICE;
+      if (hipEventElapsedTime(&b, ev[1], ev[2]) != hipSuccess) return MM3E_ERR_DEVICE;
+      ra += a;
+      pa += b;
+    }
+  }
+  if (frames) *frames = n;
+  if (render_ms_sum) *render_ms_sum = ra;
+  if (post_ms_sum) *post_ms_sum = pa;
+  if (reset) s->frames = 0;
+  return MM3E_OK;
+}
+
+int mm3e_scene_render_aov(mm3e_scene* s, const mm3e_camera* cam, uint8_t* hit, uint32_t* mat, float* depth,
+                          uint32_t* steps) {
+  if (!s || validate_cam(cam)) return MM3E_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->r->mu);
+  hipSetDevice(s->r->device);
+  const size_t n = (size_t)s->w * s->h;
+  uint8_t* d_hit = nullptr;
+  uint32_t *d_mat = nullptr, *d_steps = nullptr;
+  float* d_depth = nullptr;
+  std::string msg;
+  bool ok = hip_ok(hipMalloc(&d_hit, n), "hipMalloc", &msg) && hip_ok(hipMalloc(&d_mat, n * 4), "hipMalloc", &msg) &&
+            hip_ok(hipMalloc(&d_depth, n * 4), "hipMalloc", &msg) && hip_ok(hipMalloc(&d_steps, n * 4), "hipMalloc", &msg);
+  FrameParams fp = make_frame(*cam, s->w, s->h);
+  if (ok) ok = hip_ok(launch_primary_aov_table(s->r->stream, s->d_scene, fp, d_hit, d_mat, d_depth, d_steps), "aov", &msg);
+  if (ok) ok = hip_ok(hipStreamSynchronize(s->r->stream), "aov", &msg);
+  if (ok && hit) ok = hip_ok(hipMemcpy(hit, d_hit, n, hipMemcpyDeviceToHost), "readback", &msg);
+  if (ok && mat) ok = hip_ok(hipMemcpy(mat, d_mat, n * 4, hipMemcpyDeviceToHost), "readback", &msg);
+  if (ok && depth) ok = hip_ok(hipMemcpy(depth, d_depth, n * 4, hipMemcpyDeviceToHost), "readback", &msg);
+  if (ok && steps) ok = hip_ok(hipMemcpy(steps, d_steps, n * 4, hipMemcpyDeviceToHost), "readback", &msg);
+  hipFree(d_hit);
+  hipFree(d_mat);
+  hipFree(d_depth);
+  hipFree(d_steps);
+  return ok ? MM3E_OK : MM3E_ERR_DEVICE;
+}
+
+int mm3e_scene_gi_raw(mm3e_scene* s, float* min3, float* size3, uint32_t* dims3, float* cubes, uint64_t* n_probes) {
+  if (!s) return MM3E_ERR_INVALID;
+  if (!s->has_gi) {
+    if (n_probes) *n_probes = 0;
+    return MM3E_OK;
+  }
+  const uint64_t n = (uint64_t)s->gi_dims[0] * s->gi_dims[1] * s->gi_dims[2];
+  if (n_probes) *n_probes = n;
+  if (min3) std::memcpy(min3, s->gi_min, 12);
+  if (size3) std::memcpy(size3, s->gi_size, 12);
+  if (dims3) std::memcpy(dims3, s->gi_dims, 12);
+  if (cubes) {
+    std::lock_guard<std::mutex> lk(s->r->mu);
+    hipSetDevice(s->r->device);
+    std::vector<float> g4(n * 24);
+    if (hipMemcpy(g4.data(), s->d_gi, n * 24 * 4, hipMemcpyDeviceToHost) != hipSuccess) return MM3E_ERR_DEVICE;
+    for (uint64_t i = 0; i < n * 6; i++) {
+      cubes[3 * i] = g4[4 * i];
+      cubes[3 * i + 1] = g4[4 * i + 1];
+      cubes[3 * i + 2] = g4[4 * i + 2];
+    }
+  }
+  return MM3E_OK;
+}
+
+int mm3e_scene_field(mm3e_scene* s, const float* pts, uint32_t n, float* dist, uint32_t* mat) {
+  if (!s || (n && (!pts || !dist || !mat))) return MM3E_ERR_INVALID;
+  if (n == 0) return MM3E_OK;
+  std::lock_guard<std::mutex> lk(s->r->mu);
+  hipSetDevice(s->r->device);
+  float *d_p = nullptr, *d_d = nullptr;
+  uint32_t* d_m = nullptr;
+  std::string msg;
+  bool ok = hip_ok(hipMalloc(&d_p, (size_t)n * 12), "hipMalloc", &msg) && hip_ok(hipMalloc(&d_d, (size_t)n * 4), "hipMalloc", &msg) &&
+            hip_ok(hipMalloc(&d_m, (size_t)n * 4), "hipMalloc", &msg);
+  if (ok) ok = hip_ok(hipMemcpy(d_p, pts, (size_t)n * 12, hipMemcpyHostToDevice), "upload", &msg);
+  if (ok) ok = hip_ok(launch_field_probe_table(s->r->stream, s->d_scene, d_p, n, d_d, d_m), "field probe", &msg);
+  if (ok) ok = hip_ok(hipStreamSynchronize(s->r->stream), "field probe", &msg);
+  if (ok) ok = hip_ok(hipMemcpy(dist, d_d, (size_t)n * 4, hipMemcpyDeviceToHost), "readback", &msg);
+  if (ok) ok = hip_ok(hipMemcpy(mat, d_m, (size_t)n * 4, hipMemcpyDeviceToHost), "readback", &msg);
+  hipFree(d_p);
+  hipFree(d_d);
+  hipFree(d_m);
+  return ok ? MM3E_OK : MM3E_ERR_DEVICE;
+}
+
+int mm3e_scene_march(mm3e_scene* s, const float* rays, uint32_t n, float* t, uint8_t* hit, uint32_t* mat, float* normal,
+                     uint32_t* steps) {
+  if (!s || (n && (!rays || !t || !hit || !mat || !normal || !steps))) return MM3E_ERR_INVALID;
+  if (n == 0) return MM3E_OK;
+  std::lock_guard<std::mutex> lk(s->r->mu);
+  hipSetDevice(s->r->device);
+  float *d_r = nullptr, *d_t = nullptr, *d_n = nullptr;
+  uint8_t* d_h = nullptr;
+  uint32_t *d_m = nullptr, *d_s = nullptr;
+  std::string msg;
+  bool ok = hip_ok(hipMalloc(&d_r, (size_t)n * 24), "hipMalloc", &msg) && hip_ok(hipMalloc(&d_t, (size_t)n * 4), "hipMalloc", &msg) &&
+            hip_ok(hipMalloc(&d_n, (size_t)n * 12), "hipMalloc", &msg) && hip_ok(hipMalloc(&d_h, n), "hipMalloc", &msg) &&
+            hip_ok(hipMalloc(&d_m, (size_t)n * 4), "hipMalloc", &msg) && hip_ok(hipMalloc(&d_s, (size_t)n * 4), "hipMalloc", &msg);
+  if (ok) ok = hip_ok(hipMemcpy(d_r, rays, (size_t)n * 24, hipMemcpyHostToDevice), "upload", &msg);
+  if (ok) ok = hip_ok(launch_march_probe_table(s->r->stream, s->d_scene, d_r, n, d_t, d_h, d_m, d_n, d_s), "march probe", &msg);
+  if (ok) ok = hip_ok(hipStreamSynchronize(s->r->stream), "march probe", &msg);
+  if (ok) ok = hip_ok(hipMemcpy(t, d_t, (size_t)n * 4, hipMemcpyDeviceToHost), "readback", &msg);
+  if (ok) ok = hip_ok(hipMemcpy(hit, d_h, n, hipMemcpyDeviceToHost), "readback", &msg);
+  if (ok) ok = hip_ok(hipMemcpy(mat, d_m, (size_t)n * 4, hipMemcpyDeviceToHost), "readback", &msg);
+  if (ok) ok = hip_ok(hipMemcpy(normal, d_n, (size_t)n * 12, hipMemcpyDeviceToHost), "readback", &msg);
+  if (ok) ok = hip_ok(hipMemcpy(steps, d_s, (size_t)n * 4, hipMemcpyDeviceToHost), "readback", &msg);
+  hipFree(d_r);
+  hipFree(d_t);
+  hipFree(d_n);
+  hipFree(d_h);
+  hipFree(d_m);
+  hipFree(d_s);
+  return ok ? MM3E_OK : MM3E_ERR_DEVICE;
+}
+
+// ---- host helpers ------------------------------------------------------------------------
+void mm3e_camera_look_at(const float eye[3], const float target[3], const float up[3], float fov_y, mm3e_camera* out) {
+  // camera.rs:50-55
+  auto nrm = [](float v[3]) {
+    float l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (l > 1e-12f) {
+      float inv = 1.0f / l;
+      v[0] *= inv; v[1] *= inv; v[2] *= inv;
+    } else {
+      v[0] = v[1] = v[2] = 0.0f;
+    }
+  };
+  auto cross = [](const float* a, const float* o, float* r) {
+    r[0] = a[1] * o[2] - a[2] * o[1];
+    r[1] = a[2] * o[0] - a[0] * o[2];
+    r[2] = a[0] * o[1] - a[1] * o[0];
+  };
+  float f[3] = {target[0] - eye[0], target[1] - eye[1], target[2] - eye[2]};
+  nrm(f);
+  float r[3];
+  cross(f, up, r);
+  nrm(r);
+  float u[3];
+  cross(r, f, u);
+  std::memcpy(out->eye, eye, 12);
+  std::memcpy(out->forward, f, 12);
+  std::memcpy(out->right, r, 12);
+  std::memcpy(out->up, u, 12);
+  out->fov_scale = tanf(fov_y * 0.5f);
+}
+
+void mm3e_orbit_camera(const float target[3], float radius, float yaw, float pitch, float fov_y, mm3e_camera* out) {
+  // lib.rs:1414-1435
+  const float PI = 3.14159265358979323846f, TAU = 6.28318530717958647692f;
+  if (std::isfinite(yaw)) {
+    float v = std::fmod(yaw + PI, TAU);
+    if (v < 0.0f) v += std::fabs(TAU);
+    yaw = v - PI;
+  } else {
+    yaw = 0.0f;
+  }
+  const float lim = PI / 2.0f - 1.0e-4f;
+  pitch = std::isfinite(pitch) ? (pitch < -lim ? -lim : (pitch > lim ? lim : pitch)) : 0.0f;
+  radius = std::isfinite(radius) ? std::fmax(std::fabs(radius), 1.0e-4f) : 1.0f;
+  if (std::isfinite(fov_y)) {
+    const float lo = 1.0e-4f, hi = PI - 1.0e-4f;
+    fov_y = fov_y < lo ? lo : (fov_y > hi ? hi : fov_y);
+  } else {
+    fov_y = 45.0f * (PI / 180.0f);
+  }
+  float eye[3] = {target[0] + radius * cosf(yaw) * cosf(pitch), target[1] + radius * sinf(pitch),
+                  target[2] + radius * sinf(yaw) * cosf(pitch)};
+  const float up[3] = {0.0f, 1.0f, 0.0f};
+  mm3e_camera_look_at(eye, target, up, fov_y, out);
+}
+
+int mm3e_write_bmp(const uint8_t* rgba, uint32_t w, uint32_t h, uint8_t* out, size_t* out_len) {
+  // framebuffer.rs:55-98 (an opaque pixel round-trips through the f32 framebuffer exactly)
+  if (!out_len || (!rgba && out)) return MM3E_ERR_INVALID;
+  const size_t pad = (4 - ((size_t)w * 3) % 4) % 4;
+  const size_t row = (size_t)w * 3 + pad, pix = row * h, total = 54 + pix;
+  if (total > 0xffffffffull) return MM3E_ERR_INVALID;  // framebuffer.rs:68
+  if (!out) {
+    *out_len = total;
+    return MM3E_OK;
+  }
+  if (*out_len < total) return MM3E_ERR_INVALID;
+  *out_len = total;
+  auto u32le = [](uint8_t* p, uint32_t v) { std::memcpy(p, &v, 4); };
+  auto u16le = [](uint8_t* p, uint16_t v) { std::memcpy(p, &v, 2); };
+  std::memset(out, 0, 54);
+  out[0] = 'B';
+  out[1] = 'M';
+  u32le(out + 2, (uint32_t)total);
+  u32le(out + 10, 54);
+  u32le(out + 14, 40);
+  u32le(out + 18, w);
+  u32le(out + 22, h);
+  u16le(out + 26, 1);
+  u16le(out + 28, 24);
+  u32le(out + 34, (uint32_t)pix);
+  u32le(out + 38, 2835);
+  u32le(out + 42, 2835);
+  uint8_t* q = out + 54;
+  for (int64_t y = (int64_t)h - 1; y >= 0; y--) {
+    for (uint32_t x = 0; x < w; x++) {
+      const uint8_t* px = rgba + 4 * ((size_t)y * w + x);
+      *q++ = px[2];
+      *q++ = px[1];
+      *q++ = px[0];
+    }
+    for (size_t i = 0; i < pad; i++) *q++ = 0;
+  }
+  return MM3E_OK;
+}
+
+}  // extern "C"

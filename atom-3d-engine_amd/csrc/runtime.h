@@ -1,0 +1,52 @@
+// runtime.h — internal state behind the C ABI handles.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mm3e_hip.h"
+#include "dev_render.h"
+#include "scene_compile.h"
+
+struct mm3e_renderer {
+  int device = 0;
+  std::string name;
+  hipStream_t stream = nullptr;  // default priority
+  std::mutex mu;
+};
+
+struct mm3e_scene {
+  mm3e_renderer* r = nullptr;
+  mm3e::HostScene host;
+  uint32_t w = 0, h = 0, aa = 1, bounces = 0, tw = 16, th = 16;
+  uint8_t* d_tables = nullptr;   // DevScene + tables
+  size_t tables_cap = 0;
+  uint8_t* d_dyn = nullptr;      // tables with dynamic spheres appended
+  size_t dyn_cap = 0;
+  const mm3e::DevScene* d_scene = nullptr;
+  mm3e::DevScene hdr{};
+  float* d_gi = nullptr;         // [probe][6][4]
+  float* d_gi_in_blob = nullptr;
+  bool has_gi = false;
+  uint32_t gi_dims[3] = {0, 0, 0};
+  float gi_min[3] = {0, 0, 0}, gi_size[3] = {0, 0, 0};
+  mm3e::PostParams pp{};
+  float* d_kern = nullptr;
+  float* d_hdr = nullptr;
+  float* d_tmp = nullptr;
+  uchar4* d_rgba = nullptr;
+  uint8_t* h_rgba = nullptr;     // pinned staging
+  static constexpr int kRing = 128;   // frames whose kernel times are kept
+  hipEvent_t ev[kRing][3] = {};
+  uint64_t frames = 0;                 // frames launched since the last timing reset
+  bool timing_valid = false;
+};
+
+namespace mm3e {
+void set_err(char* err, size_t n, const std::string& s);
+bool hip_ok(hipError_t e, const char* what, std::string* msg);
+FrameParams make_frame(const mm3e_camera& c, uint32_t w, uint32_t h);
+void tile_shape(uint32_t aa, uint32_t* tw, uint32_t* th);
+}  // namespace mm3e

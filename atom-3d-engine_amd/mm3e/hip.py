@@ -1,0 +1,267 @@
+"""ctypes bindings over the C ABI of libmm3e_hip.so (include/mm3e_hip.h).
+
+This is the Python-side equivalent of the FFI stub a reference-side caller would write: it
+mirrors `mm3e_gpu::GpuRenderer` / `GpuScene` (mm3e-gpu/src/lib.rs:57-416) method for method.
+The library is the HIP/gfx950 render path; there is no CPU fallback, and constructing a
+`Renderer` without a usable MI355X raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "csrc", "libmm3e_hip.so")
+MAX_DYN = 24
+
+EXPORTS = (
+    "mm3e_renderer_create", "mm3e_renderer_adapter_name", "mm3e_renderer_wait_idle", "mm3e_renderer_destroy",
+    "mm3e_list_adapters", "mm3e_scene_compile", "mm3e_scene_destroy", "mm3e_scene_render_rgba",
+    "mm3e_scene_render_only", "mm3e_scene_render_linear", "mm3e_scene_render_aov", "mm3e_scene_gi_raw",
+    "mm3e_scene_last_timing", "mm3e_scene_timing", "mm3e_scene_field", "mm3e_scene_march", "mm3e_camera_look_at",
+    "mm3e_orbit_camera", "mm3e_write_bmp", "mm3e_scene_check",
+)
+
+_lib = None
+
+
+class Mm3eError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libmm3e_hip.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.mm3e_renderer_create.argtypes = [C.c_char_p, C.POINTER(P), C.c_char_p, C.c_size_t]
+        L.mm3e_renderer_adapter_name.argtypes = [P]
+        L.mm3e_renderer_adapter_name.restype = C.c_char_p
+        L.mm3e_renderer_wait_idle.argtypes = [P]
+        L.mm3e_renderer_destroy.argtypes = [P]
+        L.mm3e_list_adapters.argtypes = [C.c_char_p, C.c_size_t]
+        L.mm3e_scene_compile.argtypes = [P, P, C.c_size_t, C.c_uint32, C.c_uint32, C.POINTER(P), C.c_char_p, C.c_size_t]
+        L.mm3e_scene_destroy.argtypes = [P]
+        L.mm3e_scene_render_rgba.argtypes = [P, P, P, C.c_uint32, P, C.c_size_t]
+        L.mm3e_scene_render_only.argtypes = [P, P]
+        L.mm3e_scene_render_linear.argtypes = [P, P, P, C.c_size_t]
+        L.mm3e_scene_render_aov.argtypes = [P, P, P, P, P, P]
+        L.mm3e_scene_gi_raw.argtypes = [P, P, P, P, P, P]
+        L.mm3e_scene_last_timing.argtypes = [P, P, P]
+        L.mm3e_scene_timing.argtypes = [P, C.c_int, P, P, P]
+        L.mm3e_scene_field.argtypes = [P, P, C.c_uint32, P, P]
+        L.mm3e_scene_march.argtypes = [P, P, C.c_uint32, P, P, P, P, P]
+        L.mm3e_camera_look_at.argtypes = [P, P, P, C.c_float, P]
+        L.mm3e_camera_look_at.restype = None
+        L.mm3e_orbit_camera.argtypes = [P, C.c_float, C.c_float, C.c_float, C.c_float, P]
+        L.mm3e_orbit_camera.restype = None
+        L.mm3e_write_bmp.argtypes = [P, C.c_uint32, C.c_uint32, P, C.POINTER(C.c_size_t)]
+        L.mm3e_scene_check.argtypes = [P, C.c_size_t, C.c_uint32, C.c_uint32, C.c_char_p, C.c_size_t]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _cam(cam) -> np.ndarray:
+    c = np.ascontiguousarray(cam, dtype=np.float32).reshape(-1)
+    if c.size != 13:
+        raise ValueError("camera record must hold 13 floats (eye, forward, right, up, fov_scale)")
+    return c
+
+
+def camera_look_at(eye, target, up, fov_y) -> np.ndarray:
+    out = np.zeros(13, np.float32)
+    lib().mm3e_camera_look_at(_p(np.asarray(eye, np.float32)), _p(np.asarray(target, np.float32)),
+                              _p(np.asarray(up, np.float32)), float(fov_y), _p(out))
+    return out
+
+
+def orbit_camera(target, radius, yaw, pitch, fov_y) -> np.ndarray:
+    out = np.zeros(13, np.float32)
+    lib().mm3e_orbit_camera(_p(np.asarray(target, np.float32)), radius, yaw, pitch, fov_y, _p(out))
+    return out
+
+
+def write_bmp(rgba: np.ndarray) -> bytes:
+    rgba = np.ascontiguousarray(rgba, np.uint8)
+    h, w = rgba.shape[:2]
+    n = C.c_size_t(0)
+    lib().mm3e_write_bmp(_p(rgba), w, h, None, C.byref(n))
+    out = np.zeros(n.value, np.uint8)
+    rc = lib().mm3e_write_bmp(_p(rgba), w, h, _p(out), C.byref(n))
+    if rc != 0:
+        raise Mm3eError(rc, "mm3e_write_bmp failed")
+    return out.tobytes()
+
+
+def scene_check(ir: bytes, width: int, height: int):
+    """Device-free half of compile_checked: None if the scene compiles, else the message."""
+    err = C.create_string_buffer(1024)
+    rc = lib().mm3e_scene_check(ir, len(ir), width, height, err, 1024)
+    return None if rc == 0 else err.value.decode()
+
+
+def list_adapters() -> str:
+    buf = C.create_string_buffer(4096)
+    lib().mm3e_list_adapters(buf, 4096)
+    return buf.value.decode()
+
+
+class Renderer:
+    """GpuRenderer (mm3e-gpu/src/lib.rs:57-128)."""
+
+    def __init__(self, adapter: Optional[str] = None):
+        h = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        rc = lib().mm3e_renderer_create(adapter.encode() if adapter else None, C.byref(h), err, 1024)
+        if rc != 0:
+            raise Mm3eError(rc, err.value.decode())
+        self.h = h
+
+    @property
+    def adapter_name(self) -> str:
+        return lib().mm3e_renderer_adapter_name(self.h).decode()
+
+    def wait_idle(self):
+        rc = lib().mm3e_renderer_wait_idle(self.h)
+        if rc != 0:
+            raise Mm3eError(rc, "wait_idle failed")
+
+    def compile(self, ir: bytes, width: int, height: int) -> "GpuScene":
+        return GpuScene(self, ir, width, height)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mm3e_renderer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GpuScene:
+    """GpuScene (mm3e-gpu/src/lib.rs:289-416)."""
+
+    def __init__(self, renderer: Renderer, ir: bytes, width: int, height: int):
+        self.renderer = renderer
+        self.width, self.height = int(width), int(height)
+        self._ir = C.create_string_buffer(ir, len(ir))
+        h = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        rc = lib().mm3e_scene_compile(renderer.h, C.cast(self._ir, C.c_void_p), len(ir), self.width, self.height,
+                                      C.byref(h), err, 1024)
+        if rc != 0:
+            raise Mm3eError(rc, err.value.decode())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mm3e_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render_rgba(self, cam, dyn: Optional[Sequence] = None) -> np.ndarray:
+        out = np.zeros((self.height, self.width, 4), np.uint8)
+        d = None
+        n = 0
+        if dyn is not None and len(dyn):
+            d = np.ascontiguousarray(dyn, np.float32).reshape(-1, 8)
+            n = len(d)
+        rc = lib().mm3e_scene_render_rgba(self.h, _p(_cam(cam)), _p(d), n, _p(out), out.nbytes)
+        if rc != 0:
+            raise Mm3eError(rc, "render_rgba failed")
+        return out
+
+    def render_only(self, cam):
+        rc = lib().mm3e_scene_render_only(self.h, _p(_cam(cam)))
+        if rc != 0:
+            raise Mm3eError(rc, "render_only failed")
+
+    def render_linear(self, cam) -> np.ndarray:
+        out = np.zeros((self.height, self.width, 3), np.float32)
+        rc = lib().mm3e_scene_render_linear(self.h, _p(_cam(cam)), _p(out), out.size)
+        if rc != 0:
+            raise Mm3eError(rc, "render_linear failed")
+        return out
+
+    def render_aov(self, cam):
+        sh = (self.height, self.width)
+        hit = np.zeros(sh, np.uint8)
+        mat = np.zeros(sh, np.uint32)
+        depth = np.zeros(sh, np.float32)
+        steps = np.zeros(sh, np.uint32)
+        rc = lib().mm3e_scene_render_aov(self.h, _p(_cam(cam)), _p(hit), _p(mat), _p(depth), _p(steps))
+        if rc != 0:
+            raise Mm3eError(rc, "render_aov failed")
+        return hit, mat, depth, steps
+
+    def last_timing(self):
+        a, b = C.c_float(0), C.c_float(0)
+        rc = lib().mm3e_scene_last_timing(self.h, C.byref(a), C.byref(b))
+        if rc != 0:
+            raise Mm3eError(rc, "no frame timed yet")
+        return a.value, b.value
+
+    def timing(self, reset=True):
+        """(frames, mean render-kernel ms, mean post ms) over the frames since the last reset."""
+        n = C.c_uint64(0)
+        a, b = C.c_double(0), C.c_double(0)
+        rc = lib().mm3e_scene_timing(self.h, 1 if reset else 0, C.byref(n), C.byref(a), C.byref(b))
+        if rc != 0:
+            raise Mm3eError(rc, "timing failed")
+        k = max(n.value, 1)
+        return n.value, a.value / k, b.value / k
+
+    def gi_raw(self):
+        mn = np.zeros(3, np.float32)
+        sz = np.zeros(3, np.float32)
+        dims = np.zeros(3, np.uint32)
+        n = C.c_uint64(0)
+        lib().mm3e_scene_gi_raw(self.h, _p(mn), _p(sz), _p(dims), None, C.byref(n))
+        if n.value == 0:
+            return None
+        cubes = np.zeros((n.value, 6, 3), np.float32)
+        rc = lib().mm3e_scene_gi_raw(self.h, None, None, None, _p(cubes), C.byref(n))
+        if rc != 0:
+            raise Mm3eError(rc, "gi_raw failed")
+        return mn, sz, dims, cubes
+
+    def field(self, pts):
+        pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+        d = np.zeros(len(pts), np.float32)
+        m = np.zeros(len(pts), np.uint32)
+        rc = lib().mm3e_scene_field(self.h, _p(pts), len(pts), _p(d), _p(m))
+        if rc != 0:
+            raise Mm3eError(rc, "field probe failed")
+        return d, m
+
+    def march(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        n = len(rays)
+        t = np.zeros(n, np.float32)
+        hit = np.zeros(n, np.uint8)
+        mat = np.zeros(n, np.uint32)
+        nor = np.zeros((n, 3), np.float32)
+        st = np.zeros(n, np.uint32)
+        rc = lib().mm3e_scene_march(self.h, _p(rays), n, _p(t), _p(hit), _p(mat), _p(nor), _p(st))
+        if rc != 0:
+            raise Mm3eError(rc, "march probe failed")
+        return t, hit.astype(bool), mat, nor, st

@@ -1,0 +1,131 @@
+/*
+ * mm3e_hip.h — C ABI of libmm3e_hip.so, the MI355X-native per-pixel SDF render path of MM3E.
+ *
+ * Drop-in boundary for the reference GPU backend `mm3e-gpu` (mm3e-gpu/src/lib.rs): compile a
+ * scene once, re-upload only the camera per frame, render to RGBA8 / a framebuffer / BMP.
+ * Each entry point names the reference item it replaces.  Conventions:
+ *   - every function returns an int status, MM3E_OK (0) on success;
+ *   - functions that can fail with a message copy it into (err, errlen) when err != NULL,
+ *     NUL-terminated and truncated to errlen; the reference returns these as
+ *     `Result<_, String>` (lib.rs:80-142) or panics via `expect` (lib.rs:389-392);
+ *   - the caller owns every output buffer;
+ *   - a renderer may be used from several host threads (its calls are serialised by an
+ *     internal mutex); one compiled scene must be driven by one host thread at a time;
+ *   - scenes travel as the flat IR of mm3e_scene_ir.h.
+ * The library has no CPU fallback: without a usable gfx950 device every constructor fails.
+ */
+#ifndef MM3E_HIP_H
+#define MM3E_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "mm3e_scene_ir.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MM3E_OK 0
+#define MM3E_ERR_INVALID 1   /* bad argument / IR / unsupported scene                   */
+#define MM3E_ERR_DEVICE 2    /* HIP runtime failure (device lost, OOM, launch failure)  */
+#define MM3E_ERR_BUSY 3      /* admission refused (serving: HBM budget / queue full)    */
+
+/* mm3e-gpu/src/lib.rs:22 */
+#define MM3E_MAX_DYN 24
+
+typedef struct mm3e_renderer mm3e_renderer; /* GpuRenderer (lib.rs:57-61) */
+typedef struct mm3e_scene mm3e_scene;       /* GpuScene    (lib.rs:289-300) */
+
+/* Camera (mm3e-kit/src/camera.rs:39-47), host-built as Camera::look_at does. */
+typedef struct mm3e_camera {
+  float eye[3];
+  float forward[3];
+  float right[3];
+  float up[3];
+  float fov_scale; /* tan(fov_y / 2) */
+} mm3e_camera;
+
+/* DynSphere (mm3e-gpu/src/lib.rs:27-32). */
+typedef struct mm3e_dyn_sphere {
+  float pos[3];
+  float radius;
+  float albedo[3];
+  float metallic;
+} mm3e_dyn_sphere;
+
+/* ---- renderer ------------------------------------------------------------------------ */
+
+/* GpuRenderer::new / with_adapter (lib.rs:80-118).  adapter_filter == NULL honours the
+ * MM3E_GPU_ADAPTER environment variable; otherwise the first HIP device whose descriptor
+ * ("<name> (<gcnArchName>)") contains the filter, case-insensitively, is opened.
+ * Error: "no GPU adapter matching '<f>'; available: <a> | <b>". */
+int mm3e_renderer_create(const char* adapter_filter, mm3e_renderer** out, char* err, size_t errlen);
+/* GpuRenderer::adapter_name (lib.rs:120-122); valid until the renderer is destroyed. */
+const char* mm3e_renderer_adapter_name(const mm3e_renderer* r);
+/* GpuRenderer::wait_idle (lib.rs:126-128): block until all submitted work finished. */
+int mm3e_renderer_wait_idle(mm3e_renderer* r);
+void mm3e_renderer_destroy(mm3e_renderer* r);
+/* list_adapters (lib.rs:73-76): writes "a | b | ..." into buf; returns the device count. */
+int mm3e_list_adapters(char* buf, size_t buflen);
+
+/* ---- scene compile ------------------------------------------------------------------- */
+
+/* GpuRenderer::compile_checked (lib.rs:138-285).  Validates w,h in 1..=16384
+ * ("GpuRenderer::compile: {w}x{h} out of range 1..=16384"), rejects native triangle
+ * surfaces and texture appearance with the reference's messages (wgsl.rs:42-47), uploads
+ * the scene tables to HBM, bakes a requested GI volume on the device, and allocates the
+ * resident framebuffers.  Captures aa and bounces at compile time, as GpuScene does. */
+int mm3e_scene_compile(mm3e_renderer* r, const void* scene_ir, size_t ir_len, uint32_t width, uint32_t height,
+                       mm3e_scene** out, char* err, size_t errlen);
+void mm3e_scene_destroy(mm3e_scene* s);
+/* The validation half of compile_checked without a device: the same checks and messages as
+ * mm3e_scene_compile (lib.rs:139-141, wgsl.rs:42-47), nothing allocated. */
+int mm3e_scene_check(const void* scene_ir, size_t ir_len, uint32_t width, uint32_t height, char* err, size_t errlen);
+
+/* ---- per-frame ----------------------------------------------------------------------- */
+
+/* GpuScene::render_rgba / render_rgba_dyn (lib.rs:377-402): unpadded RGBA8, top-down, alpha
+ * 255; out_len must be >= w*h*4.  Dynamic spheres are unioned as appended objects with the
+ * default material (the CPU backend's semantics, mm3e-cpu-parallel/src/lib.rs:50-65); at most
+ * MM3E_MAX_DYN are used, extra spheres are ignored as in lib.rs:306-307. */
+int mm3e_scene_render_rgba(mm3e_scene* s, const mm3e_camera* cam, const mm3e_dyn_sphere* dyn, uint32_t n_dyn,
+                           uint8_t* out, size_t out_len);
+/* GpuScene::render_only (lib.rs:363-374): enqueue a frame, no readback, no wait. */
+int mm3e_scene_render_only(mm3e_scene* s, const mm3e_camera* cam);
+/* Scene-linear HDR radiance before post (render_linear, lib.rs:888-947): w*h*3 floats. */
+int mm3e_scene_render_linear(mm3e_scene* s, const mm3e_camera* cam, float* out, size_t out_floats);
+/* Centre-pixel primary-hit AOVs (film.rs:114-118): hit mask, material id (u32::MAX on a
+ * miss), camera-forward depth (+inf on a miss), march steps.  Any pointer may be NULL. */
+int mm3e_scene_render_aov(mm3e_scene* s, const mm3e_camera* cam, uint8_t* hit, uint32_t* mat, float* depth,
+                          uint32_t* steps);
+/* Raw GI probe volume of the compiled scene (GiVolume::raw, gi.rs:133-135): min, size, dims
+ * and, when cubes != NULL, the [probe][6][3] irradiance cubes.  *n_probes = 0 if no GI. */
+int mm3e_scene_gi_raw(mm3e_scene* s, float* min3, float* size3, uint32_t* dims3, float* cubes, uint64_t* n_probes);
+/* Last frame's device time of the render kernel and of the post kernels, in milliseconds. */
+int mm3e_scene_last_timing(mm3e_scene* s, float* render_ms, float* post_ms);
+/* HIP-event kernel times summed over the frames launched since the last reset (at most the
+ * last 128 frames are kept): *frames, sum of render-kernel ms, sum of post-kernel ms.
+ * Synchronises the renderer's stream.  reset != 0 starts a new window afterwards. */
+int mm3e_scene_timing(mm3e_scene* s, int reset, uint64_t* frames, double* render_ms_sum, double* post_ms_sum);
+
+/* ---- parity probes (the reference's test-only `probe` / `film_probe` kernels,
+ *      wgsl.rs:641-646, film_depth_tests.rs:31-42) ---------------------------------------- */
+int mm3e_scene_field(mm3e_scene* s, const float* pts, uint32_t n, float* dist, uint32_t* mat);
+int mm3e_scene_march(mm3e_scene* s, const float* rays, uint32_t n, float* t, uint8_t* hit, uint32_t* mat,
+                     float* normal, uint32_t* steps);
+
+/* ---- host helpers -------------------------------------------------------------------- */
+
+/* Camera::look_at (camera.rs:50-55) and orbit_camera (lib.rs:1426-1435). */
+void mm3e_camera_look_at(const float eye[3], const float target[3], const float up[3], float fov_y, mm3e_camera* out);
+void mm3e_orbit_camera(const float target[3], float radius, float yaw, float pitch, float fov_y, mm3e_camera* out);
+/* Framebuffer::to_bmp (framebuffer.rs:55-98) of an opaque RGBA8 image.  With out == NULL
+ * only *out_len is set to the required size. */
+int mm3e_write_bmp(const uint8_t* rgba, uint32_t w, uint32_t h, uint8_t* out, size_t* out_len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MM3E_HIP_H */

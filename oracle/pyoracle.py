@@ -1,0 +1,214 @@
+"""ctypes bindings for the CPU oracle (liboracle_mm3e.so) — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this module, as
+the checker of the HIP render path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle_mm3e.so")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle library not built: run `make -C oracle` (or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        fp = C.POINTER(C.c_float)
+        L.orc_scene_load.argtypes = [P, C.c_size_t, C.c_uint, C.POINTER(P), C.c_char_p, C.c_size_t]
+        L.orc_scene_free.argtypes = [P]
+        L.orc_gi_raw.argtypes = [P, P, P, P, P]
+        L.orc_gi_raw.restype = C.c_uint64
+        L.orc_render_linear.argtypes = [P, P, C.c_uint, P]
+        L.orc_render_linear_rows.argtypes = [P, P, C.c_uint, C.c_uint32, C.c_uint32, P]
+        L.orc_resolve.argtypes = [P, P, P]
+        L.orc_render_rgba.argtypes = [P, P, C.c_uint, P]
+        L.orc_render_aov.argtypes = [P, P, C.c_uint, P, P, P, P]
+        L.orc_field.argtypes = [P, P, C.c_size_t, C.c_int, P, P]
+        L.orc_march.argtypes = [P, P, C.c_size_t, P, P, P, P, P]
+        L.orc_prim_distance.argtypes = [C.c_uint32, P, P]
+        L.orc_prim_distance.restype = C.c_float
+        L.orc_prim_dual.argtypes = [C.c_uint32, P, P, P]
+        L.orc_normals.argtypes = [P, P, P, P]
+        L.orc_dual_safe.argtypes = [P]
+        L.orc_brdf.argtypes = [P, P, P, P, C.c_float, C.c_float, C.c_float, P]
+        L.orc_fresnel.argtypes = [C.c_float, C.c_float]
+        L.orc_fresnel.restype = C.c_float
+        L.orc_aces_gamma.argtypes = [P, P, P]
+        L.orc_sky.argtypes = [P, P, C.c_int, P]
+        L.orc_hash_cell.argtypes = [C.c_int32, C.c_int32]
+        L.orc_hash_cell.restype = C.c_float
+        L.orc_fnv1a.argtypes = [P, C.c_size_t]
+        L.orc_fnv1a.restype = C.c_uint64
+        L.orc_powi.argtypes = [C.c_float, C.c_int]
+        L.orc_powi.restype = C.c_float
+        L.orc_soft_shadow.argtypes = [P, P, P, C.c_float, C.c_float]
+        L.orc_soft_shadow.restype = C.c_float
+        L.orc_ao.argtypes = [P, P, P]
+        L.orc_ao.restype = C.c_float
+        L.orc_gi_sample.argtypes = [P, P, P, P]
+        L.orc_gi_bake_linear_x.argtypes = [P, P, P, C.c_uint32, P, P, P]
+        L.orc_volume_sample.argtypes = [P, P, P, P, C.c_size_t, P]
+        L.orc_volume_sample.restype = C.c_float
+        L.orc_camera_look_at.argtypes = [P, P, P, C.c_float, P]
+        L.orc_gaussian_kernel.argtypes = [C.c_uint32, P, P]
+        L.orc_count_rows.argtypes = [P, P, C.c_uint, C.c_uint32, C.c_uint32, P]
+        L.orc_phase_count.restype = C.c_int
+        L.orc_to_bmp.argtypes = [P, C.c_uint32, C.c_uint32, P]
+        L.orc_to_bmp.restype = C.c_size_t
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+PHASES = ("march", "refl_march", "normal", "shadow", "ao", "gi", "other")
+
+
+class OracleScene:
+    """A scene loaded into the oracle (bake requests are executed at load, on `threads` cores)."""
+
+    def __init__(self, ir: bytes, threads: int = 0):
+        L = lib()
+        self._buf = C.create_string_buffer(ir, len(ir))
+        h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        threads = threads or os.cpu_count() or 1
+        rc = L.orc_scene_load(C.cast(self._buf, C.c_void_p), len(ir), threads, C.byref(h), err, 512)
+        if rc != 0:
+            raise ValueError(err.value.decode())
+        self.h = h
+        import struct
+        self.width, self.height = struct.unpack_from("<II", ir, 16)
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().orc_scene_free(self.h)
+        except Exception:
+            pass
+
+    def render_linear(self, cam, threads=0):
+        out = np.zeros((self.height, self.width, 3), np.float32)
+        lib().orc_render_linear(self.h, _p(f32(cam)), threads or os.cpu_count(), _p(out))
+        return out
+
+    def render_linear_rows(self, cam, y0, y1, threads=0):
+        out = np.zeros((y1 - y0, self.width, 3), np.float32)
+        lib().orc_render_linear_rows(self.h, _p(f32(cam)), threads or os.cpu_count(), y0, y1, _p(out))
+        return out
+
+    def resolve(self, hdr):
+        out = np.zeros((self.height, self.width, 4), np.uint8)
+        lib().orc_resolve(self.h, _p(f32(hdr)), _p(out))
+        return out
+
+    def render_rgba(self, cam, threads=0):
+        return self.resolve(self.render_linear(cam, threads))
+
+    def render_aov(self, cam, threads=0):
+        n = self.width * self.height
+        hit = np.zeros(n, np.uint8)
+        mat = np.zeros(n, np.uint32)
+        depth = np.zeros(n, np.float32)
+        steps = np.zeros(n, np.uint32)
+        lib().orc_render_aov(self.h, _p(f32(cam)), threads or os.cpu_count(), _p(hit), _p(mat), _p(depth), _p(steps))
+        sh = (self.height, self.width)
+        return hit.reshape(sh), mat.reshape(sh), depth.reshape(sh), steps.reshape(sh)
+
+    def field(self, pts, linear=False):
+        pts = f32(pts).reshape(-1, 3)
+        d = np.zeros(len(pts), np.float32)
+        m = np.zeros(len(pts), np.uint32)
+        lib().orc_field(self.h, _p(pts), len(pts), 1 if linear else 0, _p(d), _p(m))
+        return d, m
+
+    def march(self, rays):
+        rays = f32(rays).reshape(-1, 6)
+        n = len(rays)
+        t = np.zeros(n, np.float32)
+        hit = np.zeros(n, np.uint8)
+        mat = np.zeros(n, np.uint32)
+        nor = np.zeros((n, 3), np.float32)
+        st = np.zeros(n, np.uint32)
+        lib().orc_march(self.h, _p(rays), n, _p(t), _p(hit), _p(mat), _p(nor), _p(st))
+        return t, hit.astype(bool), mat, nor, st
+
+    def gi_raw(self):
+        L = lib()
+        mn = np.zeros(3, np.float32)
+        sz = np.zeros(3, np.float32)
+        dims = np.zeros(3, np.uint32)
+        n = L.orc_gi_raw(self.h, _p(mn), _p(sz), _p(dims), None)
+        if n == 0:
+            return None
+        cubes = np.zeros((int(n), 6, 3), np.float32)
+        L.orc_gi_raw(self.h, None, None, None, _p(cubes))
+        return mn, sz, dims, cubes
+
+    def normals(self, p):
+        a = np.zeros(3, np.float32)
+        b = np.zeros(3, np.float32)
+        lib().orc_normals(self.h, _p(f32(p)), _p(a), _p(b))
+        return a, b
+
+    def dual_safe(self):
+        return bool(lib().orc_dual_safe(self.h))
+
+    def soft_shadow(self, o, d, max_t, k):
+        return lib().orc_soft_shadow(self.h, _p(f32(o)), _p(f32(d)), max_t, k)
+
+    def ao(self, p, n):
+        return lib().orc_ao(self.h, _p(f32(p)), _p(f32(n)))
+
+    def gi_sample(self, p, n):
+        out = np.zeros(3, np.float32)
+        lib().orc_gi_sample(self.h, _p(f32(p)), _p(f32(n)), _p(out))
+        return out
+
+    def count_rows(self, cam, y0, y1, threads=0):
+        nph = lib().orc_phase_count()
+        c = np.zeros(6 + nph, np.uint64)
+        lib().orc_count_rows(self.h, _p(f32(cam)), threads or os.cpu_count(), y0, y1, _p(c))
+        out = {"flops": int(c[0]), "trans": int(c[1]), "sqrt": int(c[2]), "div": int(c[3]),
+               "obj_exact": int(c[4 + nph]), "obj_bound": int(c[5 + nph])}
+        out["evals"] = {PHASES[i]: int(c[4 + i]) for i in range(nph)}
+        return out
+
+
+def prim_distance(kind, params, p):
+    pr = np.zeros(8, np.float32)
+    pr[: len(params)] = params
+    return lib().orc_prim_distance(kind, _p(pr), _p(f32(p)))
+
+
+def prim_dual(kind, params, p):
+    pr = np.zeros(8, np.float32)
+    pr[: len(params)] = params
+    out = np.zeros(4, np.float32)
+    lib().orc_prim_dual(kind, _p(pr), _p(f32(p)), _p(out))
+    return out
+
+
+def to_bmp(rgba):
+    h, w = rgba.shape[:2]
+    rgba = np.ascontiguousarray(rgba, np.uint8)
+    n = lib().orc_to_bmp(_p(rgba), w, h, None)
+    out = np.zeros(n, np.uint8)
+    lib().orc_to_bmp(_p(rgba), w, h, _p(out))
+    return out.tobytes()
