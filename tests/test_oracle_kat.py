@@ -255,17 +255,25 @@ def test_dual_gradient_matches_finite_difference(kind, params):
 
 
 def test_dual_and_tetra_normals_agree_and_twist_disables_dual():
+    """lib.rs:1502-1543: on a dual-safe scene the exact (dual) normal and the tetrahedron normal
+    agree closely at hit points; a twist makes the scene fall back to the tetrahedron."""
     sc, cam = configs.c1_scene(width=48, height=27)
     s = po.OracleScene(sc.to_ir(), threads=2)
     assert s.dual_safe()
-    hit, mat, depth, _ = s.render_aov(cam)
-    # compare the two normals at a handful of hit points
-    pts = []
-    for y in range(0, 27, 3):
-        for x in range(0, 48, 5):
-            if hit[y, x]:
-                pts.append((x, y))
-    assert len(pts) > 10
+    rng = np.random.default_rng(9)
+    dirs = rng.normal(size=(400, 3))
+    dirs[:, 1] = -np.abs(dirs[:, 1]) - 0.2
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    rays = np.hstack([np.tile([[0.0, 2.0, -8.0]], (400, 1)), dirs]).astype(np.float32)
+    t, hit, _, nor, _ = s.march(rays)
+    angles = []
+    for i in np.nonzero(hit)[0][:150]:
+        p = rays[i, :3] + rays[i, 3:] * t[i]
+        a, b = s.normals(p)
+        assert np.allclose(a, nor[i])  # the hit normal is the dual normal on this scene
+        angles.append(math.degrees(math.acos(min(1.0, float(a @ b)))))
+    assert len(angles) > 50
+    assert np.median(angles) < 0.2 and max(angles) < 5.0
     c2, _ = configs.c2_scene(width=16, height=9, gi_dims=None)
     assert not po.OracleScene(c2.to_ir()).dual_safe()
 
