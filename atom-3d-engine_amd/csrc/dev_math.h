@@ -243,6 +243,21 @@ MM3E_DEV float object_distance(const DevScene& sc, const DObj& o, V3 p) {
   return d;
 }
 
+// Provable no-op test for a far object.  When |p - c|^2 exceeds t^2 with
+// t = (thr + r) * (1 + 2e-5) + 1e-6 > 0, the object's bounding-sphere lower bound
+// lower = |p - c| - r (what `contrib` returns once lower > slack) satisfies lower >= thr with
+// a relative margin far above the rounding of the squared test, so folding it in leaves the
+// accumulator bit-identical.  The caller picks thr per combine mode:
+//   union:        thr = max(acc, slack)          (lower >= acc keeps acc; ties keep acc)
+//   smooth, k>0:  thr = max(acc + k, slack)      ((lower - acc)/k >= 1 -> h = 1 -> acc)
+//   subtract:     thr = max(-acc, slack)         (max(acc, -lower) = acc)
+// No sqrt and no smooth-union division are spent on objects that cannot matter.
+MM3E_DEV bool far_noop(const DObj& o, V3 p, float thr) {
+  const float t = (thr + o.bc_r) * 1.00002f + 1e-6f;
+  V3 d = p - ld3(o.bc);
+  return t > 0.0f && dot(d, d) > t * t;
+}
+
 // accel.rs:187-207 WorldPlan::contrib.
 MM3E_DEV Field contrib(const DevScene& sc, const DObj& o, V3 p) {
   if (o.flags & OBJ_BOUNDED) {
@@ -252,19 +267,29 @@ MM3E_DEV Field contrib(const DevScene& sc, const DObj& o, V3 p) {
   return Field{object_distance(sc, o, p), o.mat};
 }
 
+#ifdef MM3E_MAP_NOINLINE
+#define MM3E_MAP_ATTR __device__ __noinline__
+#else
+#define MM3E_MAP_ATTR MM3E_DEV
+#endif
+
 // accel.rs:209-252 WorldPlan::eval — the reference `world()` field.  Runs of plain unions walk
 // the same bounding-sphere tree in the same near-first order with the same `lower > best`
 // cut and earliest-index tie break as Bvh::query (accel.rs:79-119); instead of a 64-entry
 // stack each lane keeps one bit per tree level saying which child it visited first, and
 // climbs back through parent links, so traversal state is four registers.
-MM3E_DEV Field world_eval(const DevScene& sc, V3 p) {
+MM3E_MAP_ATTR Field world_eval(const DevScene& sc, const Tables& tb, V3 p) {
   Field acc{__builtin_inff(), 0u};
   bool seeded = false;
   const uint32_t nseg = sc.n_seg;
   for (uint32_t si = 0; si < nseg; si++) {
-    const DSeg& seg = sc.seg[si];
+    const DSeg& seg = tb.seg[si];
     if (seg.kind == 0) {
-      const DObj& o = sc.obj[seg.index];
+      const DObj& o = tb.obj[seg.index];
+      if (seeded && (o.flags & OBJ_BOUNDED)) {
+        float thr = seg.combine == 2 ? -acc.dist : (seg.combine == 1 && seg.k > 0.0f ? acc.dist + seg.k : acc.dist);
+        if (far_noop(o, p, rmax(thr, sc.slack))) continue;
+      }
       Field f = contrib(sc, o, p);
       if (!seeded) {
         acc = f;
@@ -276,6 +301,24 @@ MM3E_DEV Field world_eval(const DevScene& sc, V3 p) {
       else acc = f_subtract(acc, f);
       continue;
     }
+    if (seg.kind == 2) {
+      // Every member has an exact SDF inside a true bounding sphere (SDF >= |p - c| - r), so
+      // the tree walk can only skip objects that would not have won: the run's result is the
+      // plain index-order minimum (strict < keeps the earlier object, and the seeded
+      // accumulator wins ties, as in accel.rs:236-247).
+      float bd = seeded ? acc.dist : __builtin_inff();
+      uint32_t bm = seeded ? acc.mat : 0u;
+      bool any = seeded;
+      for (uint32_t u = 0; u < seg.unb_count; u++) {
+        const DObj& o = tb.obj[tb.unb[seg.unb_base + u]];
+        if (any && (o.flags & OBJ_BOUNDED) && far_noop(o, p, rmax(bd, sc.slack))) continue;
+        Field f = contrib(sc, o, p);
+        if (!any || f.dist < bd) { bd = f.dist; bm = f.mat; any = true; }
+      }
+      acc = Field{bd, bm};
+      seeded = true;
+      continue;
+    }
     float bd;
     int64_t bi;
     uint32_t bm;
@@ -285,25 +328,30 @@ MM3E_DEV Field world_eval(const DevScene& sc, V3 p) {
       bd = __builtin_inff(); bi = INT64_MAX; bm = 0u;
     }
     for (uint32_t u = 0; u < seg.unb_count; u++) {
-      uint32_t i = sc.unb[seg.unb_base + u];
-      const DObj& o = sc.obj[i];
+      uint32_t i = tb.unb[seg.unb_base + u];
+      const DObj& o = tb.obj[i];
       float d = object_distance(sc, o, p);
       if (d < bd || (d == bd && (int64_t)i < bi)) { bd = d; bi = i; bm = o.mat; }
     }
     if (seg.has_tree) {
-      const DNode* nodes = sc.nodes;
+      const DNode* nodes = tb.nodes;
       const int32_t root = (int32_t)seg.index;
       int32_t node = root;
       uint32_t depth = 0, nearbits = 0;
       for (;;) {
-        // descend into `node`
+        // descend into `node`; the squared pre-test proves `lower > bd` without a sqrt
         const DNode& n = nodes[node];
-        float lower = length(p - ld3(n.c)) - n.r;
         bool climb = true;
-        if (!(lower > bd)) {
+        const float tn = (bd + n.r) * 1.00002f + 1e-6f;
+        V3 dn = p - ld3(n.c);
+        const float dn2 = dot(dn, dn);
+        float lower = 0.0f;
+        const bool pruned = tn > 0.0f && dn2 > tn * tn;
+        if (!pruned) lower = __builtin_sqrtf(dn2) - n.r;
+        if (!pruned && !(lower > bd)) {
           if (n.leaf >= 0) {
             const uint32_t i = (uint32_t)n.leaf;
-            const DObj& o = sc.obj[i];
+            const DObj& o = tb.obj[i];
             float d;
             if (lower > sc.slack) d = lower;
             else d = object_distance(sc, o, p);
@@ -486,12 +534,12 @@ MM3E_DEV Dual object_dual(const DObj& o, V3 w) {
   return d;
 }
 
-MM3E_DEV V3 normal_dual(const DevScene& sc, V3 p) {
+MM3E_DEV V3 normal_dual(const DevScene& sc, const Tables& tb, V3 p) {
   const uint32_t n = sc.n_obj;
   if (n == 0) return V3{0.0f, 0.0f, 0.0f};
-  Dual acc = object_dual(sc.obj[0], p);
+  Dual acc = object_dual(tb.obj[0], p);
   for (uint32_t i = 1; i < n; i++) {
-    const DObj& o = sc.obj[i];
+    const DObj& o = tb.obj[i];
     Dual f = object_dual(o, p);
     const uint32_t comb = (o.flags >> OBJ_COMBINE_SHIFT) & 3u;
     if (comb == 0) acc = dmin(acc, f);
@@ -598,7 +646,7 @@ template <class Map> MM3E_DEV Hit march(const DevScene& sc, const Map& field, V3
 
 // Hit normal: dual-number gradient on dual-safe scenes, tetrahedron otherwise (lib.rs:1273-1277).
 template <class Map> MM3E_DEV V3 hit_normal(const DevScene& sc, const Map& field, V3 p) {
-  if (sc.flags & SC_DUAL_SAFE) return normal_dual(sc, p);
+  if (sc.flags & SC_DUAL_SAFE) return normal_dual(sc, field.tb, p);
   return tetra_normal(field, sc.normal_h, p);
 }
 
@@ -804,10 +852,26 @@ MM3E_DEV V3 ibl_irradiance(const DevScene& sc, V3 n) {
   return cmul(scale(acc, 1.0f / rmax(w, 1e-4f)), ld3(sc.sky_ambient));
 }
 
-// Table-driven world field: the compiled scene in HBM.
+// Table-driven world field over the compiled scene (tables in HBM or staged in LDS).
 struct TableMap {
   const DevScene* sc;
-  MM3E_DEV Field operator()(V3 p) const { return world_eval(*sc, p); }
+  Tables tb;
+  MM3E_DEV Field operator()(V3 p) const { return world_eval(*sc, tb, p); }
 };
+
+MM3E_DEV Tables global_tables(const DevScene* sc) { return Tables{sc->obj, sc->seg, sc->nodes, sc->unb}; }
+
+// Copy the object/segment/node/run-list tables (one contiguous range of `bytes`, which the
+// launch sizes the dynamic LDS for) into LDS with the whole block, 16 bytes per lane per
+// step, and return a view onto the copy.
+MM3E_DEV Tables stage_tables(const DevScene* sc, uint4* lds, uint32_t bytes) {
+  const uint32_t n16 = bytes / 16;
+  const uint4* src = reinterpret_cast<const uint4*>(sc->obj);
+  for (uint32_t i = threadIdx.x; i < n16; i += blockDim.x) lds[i] = src[i];
+  __syncthreads();
+  const char* b = reinterpret_cast<const char*>(lds);
+  return Tables{reinterpret_cast<const DObj*>(b), reinterpret_cast<const DSeg*>(b + sc->st_seg),
+                reinterpret_cast<const DNode*>(b + sc->st_node), reinterpret_cast<const uint32_t*>(b + sc->st_unb)};
+}
 
 }  // namespace mm3e

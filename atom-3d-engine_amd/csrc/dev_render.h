@@ -86,10 +86,11 @@ MM3E_DEV V3 trace_path(const DevScene& sc, const Map& field, V3 ro, V3 rd) {
 
 // Render one tile: blockDim.x = tw*th*aa*aa lanes; lane -> (pixel in tile, sample).
 template <class Map, int MAXB>
-MM3E_DEV void render_tile(const DevScene* __restrict__ scp, const FrameParams& fp, float* __restrict__ hdr,
-                          uint32_t tw, uint32_t th, uint32_t tile_x, uint32_t tile_y, float* smem) {
+MM3E_DEV void render_tile(const DevScene* __restrict__ scp, const Tables& tb, const FrameParams& fp,
+                          float* __restrict__ hdr, uint32_t tw, uint32_t th, uint32_t tile_x, uint32_t tile_y,
+                          float* smem) {
   const DevScene& sc = *scp;
-  Map field{scp};
+  Map field{scp, tb};
   const uint32_t aa = sc.aa;
   const uint32_t ns = aa * aa;
   const uint32_t t = threadIdx.x;
@@ -169,8 +170,8 @@ MM3E_DEV V3 gather(const DevScene& sc, const Map& field, V3 o, V3 d) {
 // One lane per (probe, face, ray) of probes [first_probe, first_probe + n_probes); radiance is
 // written to out[((probe - first_probe)*6 + face)*R + ray].
 template <class Map>
-MM3E_DEV void gi_gather_rays(const DevScene* __restrict__ scp, const GiBakeParams& bp, float4* __restrict__ out,
-                             uint32_t first_probe, uint32_t n_probes) {
+MM3E_DEV void gi_gather_rays(const DevScene* __restrict__ scp, const Tables& tb, const GiBakeParams& bp,
+                             float4* __restrict__ out, uint32_t first_probe, uint32_t n_probes) {
   const uint32_t R = bp.rays_per_face;
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t per_probe = 6 * R;
@@ -183,7 +184,7 @@ MM3E_DEV void gi_gather_rays(const DevScene* __restrict__ scp, const GiBakeParam
   const uint32_t i = probe % nx, j = (probe / nx) % ny, k = probe / (nx * ny);
   V3 o = V3{bp.lo[0], bp.lo[1], bp.lo[2]} + V3{(float)i * bp.cell[0], (float)j * bp.cell[1], (float)k * bp.cell[2]};
   V3 d = V3{bp.dirs[face][ray][0], bp.dirs[face][ray][1], bp.dirs[face][ray][2]};
-  Map field{scp};
+  Map field{scp, tb};
   V3 c = gather(*scp, field, o, d);
   out[((size_t)local * 6 + face) * R + ray] = float4{c.x, c.y, c.z, 0.0f};
 }
@@ -299,7 +300,7 @@ MM3E_DEV void primary_aov(const DevScene* __restrict__ scp, const FrameParams& f
                           uint32_t* __restrict__ mat, float* __restrict__ depth, uint32_t* __restrict__ steps) {
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
   if (x >= fp.width) return;
-  Map field{scp};
+  Map field{scp, global_tables(scp)};
   V3 ro, rd;
   camera_ray(fp, (float)x + 0.5f, (float)y + 0.5f, ro, rd);
   Hit h = march(*scp, field, ro, rd);
@@ -315,7 +316,7 @@ MM3E_DEV void field_probe(const DevScene* __restrict__ scp, const float* __restr
                           float* __restrict__ dist, uint32_t* __restrict__ mat) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  Map field{scp};
+  Map field{scp, global_tables(scp)};
   Field f = field(V3{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]});
   dist[i] = f.dist;
   mat[i] = f.mat;
@@ -327,7 +328,7 @@ MM3E_DEV void march_probe(const DevScene* __restrict__ scp, const float* __restr
                           float* __restrict__ nrm, uint32_t* __restrict__ steps) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  Map field{scp};
+  Map field{scp, global_tables(scp)};
   const float* r = rays + 6 * i;
   Hit h = march(*scp, field, V3{r[0], r[1], r[2]}, V3{r[3], r[4], r[5]});
   V3 nn = h.hit ? hit_normal(*scp, field, h.pos) : V3{0.0f, 0.0f, 0.0f};

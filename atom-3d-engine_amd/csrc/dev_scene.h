@@ -59,7 +59,8 @@ struct alignas(16) DNode {
 
 // accel.rs:122-125: Segment::One(index) or Segment::Run{tree, unbounded}.
 struct alignas(16) DSeg {
-  uint32_t kind;        // 0 = one object, 1 = run of plain unions
+  uint32_t kind;        // 0 = one object, 1 = run of plain unions walked through its tree,
+                        // 2 = run of plain unions folded linearly (members in unb[])
   uint32_t index;       // one: object index; run: root node index (global)
   uint32_t has_tree;
   uint32_t unb_base;    // run: first entry in the unbounded-object index list
@@ -111,6 +112,10 @@ struct alignas(16) DevScene {
   uint32_t n_obj, n_seg, n_mat, n_light;
   uint32_t max_steps, shadow_steps, ao_samples, bounces;
   uint32_t flags, n_vol, aa, pad0;
+  // The object/segment/node/run-list tables are one contiguous byte range of the blob, so a
+  // block can copy them to LDS in one sweep: stage_bytes long, starting at `obj`; the other
+  // tables start st_seg / st_node / st_unb bytes into it.
+  uint32_t stage_bytes, st_seg, st_node, st_unb;
   uint32_t gi_dims[3], pad1;
   float max_dist, eps, step_scale, lod;
   float subitize, normal_h, slack, fog_density;
@@ -133,6 +138,14 @@ struct alignas(16) DevScene {
 };
 
 // Per-frame camera record (mm3e-kit/src/camera.rs:39-47) plus the image size.
+// Table view used by the field walk: device addresses in HBM, or the same tables staged in LDS.
+struct Tables {
+  const DObj* obj;
+  const DSeg* seg;
+  const DNode* nodes;
+  const uint32_t* unb;
+};
+
 struct FrameParams {
   float eye[3], fov_scale;
   float fwd[3], pad0;

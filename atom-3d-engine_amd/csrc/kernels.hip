@@ -13,7 +13,15 @@ using namespace mm3e;
 extern "C" __global__ void __launch_bounds__(256)
     mm3e_gi_gather_table(const DevScene* __restrict__ sc, GiBakeParams bp, float4* __restrict__ out,
                          uint32_t first_probe, uint32_t n_probes) {
-  gi_gather_rays<TableMap>(sc, bp, out, first_probe, n_probes);
+  gi_gather_rays<TableMap>(sc, global_tables(sc), bp, out, first_probe, n_probes);
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+    mm3e_gi_gather_lds(const DevScene* __restrict__ sc, GiBakeParams bp, float4* __restrict__ out,
+                       uint32_t first_probe, uint32_t n_probes, uint32_t stage_bytes) {
+  extern __shared__ uint4 tables_lds[];
+  const Tables tb = stage_tables(sc, tables_lds, stage_bytes);
+  gi_gather_rays<TableMap>(sc, tb, bp, out, first_probe, n_probes);
 }
 
 extern "C" __global__ void __launch_bounds__(256)
@@ -63,20 +71,22 @@ extern "C" __global__ void __launch_bounds__(256)
 namespace mm3e {
 
 hipError_t launch_render_table(uint32_t maxb, dim3 grid, dim3 block, hipStream_t st, const DevScene* sc,
-                               const FrameParams& fp, float* hdr, uint32_t tw, uint32_t th) {
+                               uint32_t stage_bytes, const FrameParams& fp, float* hdr, uint32_t tw, uint32_t th) {
   switch (maxb) {
-    case 0: return launch_render_table_b0(grid, block, st, sc, fp, hdr, tw, th);
-    case 1: return launch_render_table_b1(grid, block, st, sc, fp, hdr, tw, th);
-    case 2: return launch_render_table_b2(grid, block, st, sc, fp, hdr, tw, th);
-    case 3: return launch_render_table_b3(grid, block, st, sc, fp, hdr, tw, th);
-    case 4: return launch_render_table_b4(grid, block, st, sc, fp, hdr, tw, th);
+    case 0: return launch_render_table_b0(grid, block, st, sc, stage_bytes, fp, hdr, tw, th);
+    case 1: return launch_render_table_b1(grid, block, st, sc, stage_bytes, fp, hdr, tw, th);
+    case 2: return launch_render_table_b2(grid, block, st, sc, stage_bytes, fp, hdr, tw, th);
+    case 3: return launch_render_table_b3(grid, block, st, sc, stage_bytes, fp, hdr, tw, th);
+    case 4: return launch_render_table_b4(grid, block, st, sc, stage_bytes, fp, hdr, tw, th);
     default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t launch_gi_gather_table(dim3 grid, dim3 block, hipStream_t st, const DevScene* sc, const GiBakeParams& bp,
-                                  float4* out, uint32_t first_probe, uint32_t n_probes) {
-  hipLaunchKernelGGL(mm3e_gi_gather_table, grid, block, 0, st, sc, bp, out, first_probe, n_probes);
+hipError_t launch_gi_gather_table(dim3 grid, dim3 block, hipStream_t st, const DevScene* sc, uint32_t stage_bytes,
+                                  const GiBakeParams& bp, float4* out, uint32_t first_probe, uint32_t n_probes) {
+  if (stage_bytes)
+    hipLaunchKernelGGL(mm3e_gi_gather_lds, grid, block, stage_bytes, st, sc, bp, out, first_probe, n_probes, stage_bytes);
+  else hipLaunchKernelGGL(mm3e_gi_gather_table, grid, block, 0, st, sc, bp, out, first_probe, n_probes);
   return hipGetLastError();
 }
 

@@ -11,7 +11,8 @@ struct PostParams;
 
 #define MM3E_DECL_RENDER(B)                                                                          \
   hipError_t launch_render_table_b##B(dim3 grid, dim3 block, hipStream_t st, const DevScene* sc, \
-                                      const FrameParams& fp, float* hdr, uint32_t tw, uint32_t th);
+                                      uint32_t stage_bytes, const FrameParams& fp, float* hdr, uint32_t tw,  \
+                                      uint32_t th);
 MM3E_DECL_RENDER(0)
 MM3E_DECL_RENDER(1)
 MM3E_DECL_RENDER(2)
@@ -19,10 +20,11 @@ MM3E_DECL_RENDER(3)
 MM3E_DECL_RENDER(4)
 #undef MM3E_DECL_RENDER
 
+// stage_bytes > 0 selects the variant that copies the scene tables into LDS first.
 hipError_t launch_render_table(uint32_t maxb, dim3 grid, dim3 block, hipStream_t st, const DevScene* sc,
-                               const FrameParams& fp, float* hdr, uint32_t tw, uint32_t th);
-hipError_t launch_gi_gather_table(dim3 grid, dim3 block, hipStream_t st, const DevScene* sc, const GiBakeParams& bp,
-                                  float4* out, uint32_t first_probe, uint32_t n_probes);
+                               uint32_t stage_bytes, const FrameParams& fp, float* hdr, uint32_t tw, uint32_t th);
+hipError_t launch_gi_gather_table(dim3 grid, dim3 block, hipStream_t st, const DevScene* sc, uint32_t stage_bytes,
+                                  const GiBakeParams& bp, float4* out, uint32_t first_probe, uint32_t n_probes);
 hipError_t launch_gi_reduce(dim3 grid, dim3 block, hipStream_t st, const GiBakeParams& bp, const float4* rays,
                             float4* cubes, uint32_t n_probes);
 hipError_t launch_post(hipStream_t st, const PostParams& pp, const float* kern, const float* hdr, float* tmp,

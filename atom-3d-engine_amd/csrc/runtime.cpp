@@ -99,7 +99,8 @@ bool bake_gi_device(mm3e_scene* s, std::string& msg) {
   for (uint64_t p0 = 0; p0 < n_probes && ok; p0 += pass_probes) {
     const uint32_t np = (uint32_t)std::min<uint64_t>(pass_probes, n_probes - p0);
     const uint64_t lanes = (uint64_t)np * 6 * R;
-    ok = hip_ok(launch_gi_gather_table(dim3((uint32_t)((lanes + 255) / 256)), dim3(256), st, s->d_scene, bp, d_rays,
+    ok = hip_ok(launch_gi_gather_table(dim3((uint32_t)((lanes + 255) / 256)), dim3(256), st, s->d_scene,
+                                       s->stage_bytes, bp, d_rays,
                                        (uint32_t)p0, np),
                 "gi gather launch", &msg);
     if (!ok) break;
@@ -148,16 +149,21 @@ bool upload_tables(mm3e_scene* s, const std::vector<mm3e_ir_object>& xo, const s
     return false;
   if (!hip_ok(hipStreamSynchronize(s->r->stream), "upload scene tables", &msg)) return false;
   if (hdr_out) *hdr_out = b.header;
+  const uint32_t stage = b.header.stage_bytes <= s->r->lds_budget ? b.header.stage_bytes : 0;
+  if (*d_buf == s->d_tables) s->stage_bytes = stage;
+  else s->dyn_stage_bytes = stage;
   return true;
 }
 
 bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std::string& msg) {
   hipStream_t st = s->r->stream;
+  const uint32_t stage = dsc == s->d_scene ? s->stage_bytes : s->dyn_stage_bytes;
   dim3 grid((fp.width + s->tw - 1) / s->tw, (fp.height + s->th - 1) / s->th);
   dim3 block(s->tw * s->th * s->aa * s->aa);
   hipEvent_t* ev = s->ev[s->frames % mm3e_scene::kRing];
   hipEventRecord(ev[0], st);
-  if (!hip_ok(launch_render_table(s->bounces, grid, block, st, dsc, fp, s->d_hdr, s->tw, s->th), "render launch", &msg))
+  if (!hip_ok(launch_render_table(s->bounces, grid, block, st, dsc, stage, fp, s->d_hdr, s->tw, s->th),
+              "render launch", &msg))
     return false;
   hipEventRecord(ev[1], st);
   if (!hip_ok(launch_post(st, s->pp, s->d_kern, s->d_hdr, s->d_tmp, s->d_rgba), "post launch", &msg)) return false;
@@ -222,6 +228,9 @@ int mm3e_renderer_create(const char* adapter_filter, mm3e_renderer** out, char* 
   auto* r = new mm3e_renderer();
   r->device = dev;
   r->name = describe(dev);
+  // Tables staged in LDS share the block's LDS with the 3 KiB sample-reduction buffer; keep
+  // the copy to 56 KiB so three 256-lane blocks still fit one CU.
+  r->lds_budget = (uint32_t)std::min<size_t>(56 * 1024, p.sharedMemPerBlock > 4096 ? p.sharedMemPerBlock - 4096 : 0);
   std::string msg;
   if (!hip_ok(hipSetDevice(dev), "hipSetDevice", &msg) || !hip_ok(hipStreamCreate(&r->stream), "hipStreamCreate", &msg)) {
     set_err(err, errlen, "request_device failed: " + msg);
@@ -298,6 +307,7 @@ int mm3e_scene_compile(mm3e_renderer* r, const void* ir, size_t ir_len, uint32_t
       std::memcpy(b.bytes.data(), &b.header, sizeof b.header);
       ok = hip_ok(hipMemcpy(s->d_tables, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice), "upload tables", &msg);
       hdr = b.header;
+      s->stage_bytes = b.header.stage_bytes <= r->lds_budget ? b.header.stage_bytes : 0;
     }
   } else {
     ok = upload_tables(s, {}, {}, &s->d_tables, &s->tables_cap, &hdr, msg);

@@ -12,6 +12,7 @@
 
 struct mm3e_renderer {
   int device = 0;
+  uint32_t lds_budget = 0;       // largest table copy staged in LDS per block
   std::string name;
   hipStream_t stream = nullptr;  // default priority
   std::mutex mu;
@@ -42,6 +43,8 @@ struct mm3e_scene {
   hipEvent_t ev[kRing][3] = {};
   uint64_t frames = 0;                 // frames launched since the last timing reset
   bool timing_valid = false;
+  uint32_t stage_bytes = 0;            // LDS bytes of the staged tables (0: read from HBM)
+  uint32_t dyn_stage_bytes = 0;        // the same for the tables with dynamic spheres appended
 };
 
 namespace mm3e {

@@ -14,6 +14,12 @@
 
 namespace mm3e {
 
+#ifndef MM3E_LINEAR_RUN_MAX
+#define MM3E_LINEAR_RUN_MAX 16
+#endif
+// Longest run folded linearly; longer runs walk their tree (O(log n) per evaluation).
+static const size_t kLinearRunMax = MM3E_LINEAR_RUN_MAX;
+
 namespace {
 
 struct F3 {
@@ -394,8 +400,38 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
   std::vector<DNode> nodes;
   std::vector<uint32_t> unb;
   std::vector<uint32_t> run;
+  // An object is "exact-bounded" when its field value never drops below its bounding-sphere
+  // lower bound |p - c| - r: an exact Euclidean SDF (sphere, box, rounded box, torus, capped
+  // cylinder, capsule, round cone with |r1 - r2| < h) placed by a rigid transform with positive
+  // scale, optionally mirrored / elongated / rounded / hollowed (all accounted for in the
+  // bound).  Unbounded members are evaluated exactly anyway.  For a run of such objects the
+  // reference's tree query returns exactly the index-order minimum.
+  auto exact_bounded = [&](uint32_t i) {
+    const mm3e_ir_object& o = objs[i];
+    if (!bounds[i].some) return true;
+    if (!(o.scale > 0.0f) || o.twist != 0.0f) return false;
+    switch (o.prim) {
+      case MM3E_PRIM_SPHERE: case MM3E_PRIM_BOX: case MM3E_PRIM_ROUND_BOX: case MM3E_PRIM_TORUS:
+      case MM3E_PRIM_CYLINDER: case MM3E_PRIM_CAPSULE:
+        return true;
+      case MM3E_PRIM_CONE: return std::fabs(o.p[0] - o.p[1]) < o.p[2];
+      default: return false;
+    }
+  };
   auto close_run = [&]() {
     if (run.empty()) return;
+    bool linear = run.size() <= kLinearRunMax;
+    for (uint32_t i : run) linear = linear && exact_bounded(i);
+    if (linear) {
+      DSeg s{};
+      s.kind = 2;
+      s.unb_base = (uint32_t)unb.size();
+      for (uint32_t i : run) unb.push_back(i);
+      s.unb_count = (uint32_t)run.size();
+      segs.push_back(s);
+      run.clear();
+      return;
+    }
     DSeg s{};
     s.kind = 1;
     s.unb_base = (uint32_t)unb.size();
@@ -554,6 +590,10 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
   d.vol = (const DVol*)(dev_base + o_vol);
   d.vol_data = (const float*)(dev_base + o_vdata);
   d.gi = gi_dev ? gi_dev : (o_gi ? (const float*)(dev_base + o_gi) : nullptr);
+  d.stage_bytes = (uint32_t)(o_mat - o_obj);
+  d.st_seg = (uint32_t)(o_seg - o_obj);
+  d.st_node = (uint32_t)(o_node - o_obj);
+  d.st_unb = (uint32_t)(o_unb - o_obj);
   std::memcpy(out.bytes.data(), &d, sizeof d);
 }
 
