@@ -1,0 +1,370 @@
+// batch.cpp — batched rendering of many (scene, camera, size) jobs in one persistent launch.
+//
+// A batch owns a job table in HBM, per-job framebuffers (HDR, bloom scratch, RGBA8) carved
+// from one arena, and a queue counter.  A frame of the batch is three kernels — queue reset,
+// the persistent tile-queue render (batch_kernel.hip), the batched bloom/tonemap over all
+// jobs' rows — captured once into a hipGraph and replayed; cameras change between replays by
+// rewriting the job table (outside the graph).  The autotuner times the persistent launch
+// over blocks-per-CU and, for single-sample jobs, the tile shape, and keeps the fastest.
+// Tiling never changes a pixel: each pixel's samples are summed in the same order whatever
+// tile or block renders it.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "batch.h"
+#include "launch.h"
+#include "runtime.h"
+
+struct mm3e_batch {
+  mm3e_renderer* r = nullptr;
+  std::vector<mm3e_batch_job> in;     // caller's jobs (original order)
+  std::vector<uint32_t> order;        // table position -> caller job index
+  std::vector<uint32_t> pos_of;       // caller job index -> table position
+  std::vector<mm3e::BatchJob> jobs;   // host copy of the device job table
+  std::vector<mm3e::BatchPost> posts;
+  mm3e::BatchJob* d_jobs = nullptr;
+  mm3e::BatchPost* d_posts = nullptr;
+  uint32_t* d_counter = nullptr;
+  uint8_t* d_arena = nullptr;
+  size_t arena_bytes = 0;
+  uint32_t total_tiles = 0, total_rows = 0, max_w = 0, max_radius = 0, maxb = 0;
+  uint32_t lds_bytes = 0;             // > 0: scene tables staged in LDS
+  uint32_t blocks_per_cu = 4, cus = 256;
+  uint32_t tile_aa1[2] = {16, 16};    // tile shape for single-sample jobs
+  bool jobs_dirty = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool launched = false;
+  uint64_t total_pixels = 0;
+};
+
+namespace mm3e {
+namespace {
+
+void job_tiles(const mm3e_batch* b, uint32_t aa, uint32_t* tw, uint32_t* th) {
+  if (aa <= 1) {
+    *tw = b->tile_aa1[0];
+    *th = b->tile_aa1[1];
+  } else {
+    tile_shape(aa, tw, th);
+  }
+}
+
+// (Re)build the job and post tables for the current tiling parameters.
+void build_tables(mm3e_batch* b) {
+  b->jobs.clear();
+  b->posts.clear();
+  uint32_t tile = 0, row = 0;
+  b->max_w = 0;
+  b->max_radius = 1;
+  b->maxb = 0;
+  uint32_t stage_max = 0;
+  bool all_fit = true;
+  size_t off = 0;
+  for (uint32_t k = 0; k < b->order.size(); k++) {
+    const mm3e_batch_job& in = b->in[b->order[k]];
+    mm3e_scene* s = in.scene;
+    const uint32_t w = in.width, h = in.height;
+    BatchJob J{};
+    J.sc = s->d_scene;
+    J.fp = make_frame(in.cam, w, h);
+    job_tiles(b, s->aa, &J.tw, &J.th);
+    J.tiles_x = (w + J.tw - 1) / J.tw;
+    const uint32_t tiles_y = (h + J.th - 1) / J.th;
+    J.tile_base = tile;
+    tile += J.tiles_x * tiles_y;
+    J.stage_bytes = s->hdr.stage_bytes;
+    stage_max = std::max(stage_max, J.stage_bytes);
+    all_fit = all_fit && J.stage_bytes <= b->r->lds_budget;
+    const size_t px = (size_t)w * h;
+    J.hdr = (float*)(b->d_arena + off);
+    BatchPost P{};
+    P.pp = s->pp;
+    P.pp.width = w;
+    P.pp.height = h;
+    P.kern = s->d_kern;
+    P.hdr = J.hdr;
+    P.tmp = (float*)(b->d_arena + off + px * 12);
+    P.rgba = (uchar4*)(b->d_arena + off + px * 24);
+    P.row_base = row;
+    row += h;
+    off += px * 28;
+    off = (off + 255) & ~size_t(255);
+    b->max_w = std::max(b->max_w, w);
+    b->max_radius = std::max(b->max_radius, P.pp.radius);
+    b->maxb = std::max(b->maxb, s->bounces);
+    b->jobs.push_back(J);
+    b->posts.push_back(P);
+  }
+  b->total_tiles = tile;
+  b->total_rows = row;
+  b->lds_bytes = all_fit ? stage_max : 0;
+}
+
+size_t arena_size(const std::vector<mm3e_batch_job>& in) {
+  size_t off = 0;
+  for (const auto& j : in) {
+    off += (size_t)j.width * j.height * 28;
+    off = (off + 255) & ~size_t(255);
+  }
+  return off;
+}
+
+bool enqueue(mm3e_batch* b, std::string& msg) {
+  hipStream_t st = b->r->stream;
+  if (!hip_ok(launch_batch_reset(st, b->d_counter), "batch reset", &msg)) return false;
+  if (!hip_ok(launch_batch_render(b->maxb, b->cus * b->blocks_per_cu, st, b->d_jobs, (uint32_t)b->jobs.size(),
+                                  b->total_tiles, b->d_counter, b->lds_bytes),
+              "batch render", &msg))
+    return false;
+  return hip_ok(launch_batch_post(st, b->d_posts, (uint32_t)b->posts.size(), b->total_rows, b->max_w, b->max_radius),
+                "batch post", &msg);
+}
+
+bool upload(mm3e_batch* b, std::string& msg) {
+  hipStream_t st = b->r->stream;
+  if (!hip_ok(hipMemcpyAsync(b->d_jobs, b->jobs.data(), b->jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice, st),
+              "upload jobs", &msg))
+    return false;
+  if (!hip_ok(hipMemcpyAsync(b->d_posts, b->posts.data(), b->posts.size() * sizeof(BatchPost), hipMemcpyHostToDevice,
+                             st),
+              "upload posts", &msg))
+    return false;
+  b->jobs_dirty = false;
+  return hip_ok(hipStreamSynchronize(st), "upload", &msg);
+}
+
+bool capture(mm3e_batch* b, std::string& msg) {
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  hipStream_t st = b->r->stream;
+  if (!hip_ok(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture", &msg)) return false;
+  std::string m2;
+  bool ok = enqueue(b, m2);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &g);
+  if (!ok) {
+    msg = m2;
+    if (g) hipGraphDestroy(g);
+    return false;
+  }
+  if (!hip_ok(e, "end capture", &msg)) return false;
+  b->graph = g;
+  return hip_ok(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "graph instantiate", &msg);
+}
+
+}  // namespace
+}  // namespace mm3e
+
+using namespace mm3e;
+
+extern "C" {
+
+void mm3e_batch_destroy(mm3e_batch* b) {
+  if (!b) return;
+  if (b->r) {
+    hipSetDevice(b->r->device);
+    hipStreamSynchronize(b->r->stream);
+  }
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  if (b->d_jobs) hipFree(b->d_jobs);
+  if (b->d_posts) hipFree(b->d_posts);
+  if (b->d_counter) hipFree(b->d_counter);
+  if (b->d_arena) hipFree(b->d_arena);
+  if (b->ev0) hipEventDestroy(b->ev0);
+  if (b->ev1) hipEventDestroy(b->ev1);
+  delete b;
+}
+
+int mm3e_batch_create(mm3e_renderer* r, const mm3e_batch_job* jobs, uint32_t n_jobs, mm3e_batch** out, char* err,
+                      size_t errlen) {
+  if (!r || !out || (n_jobs && !jobs)) return MM3E_ERR_INVALID;
+  *out = nullptr;
+  if (n_jobs == 0) {
+    set_err(err, errlen, "a batch needs at least one job");
+    return MM3E_ERR_INVALID;
+  }
+  for (uint32_t i = 0; i < n_jobs; i++) {
+    const mm3e_batch_job& j = jobs[i];
+    if (!j.scene || j.scene->r != r) {
+      set_err(err, errlen, "batch job " + std::to_string(i) + ": scene is missing or compiled on another renderer");
+      return MM3E_ERR_INVALID;
+    }
+    if (j.width == 0 || j.height == 0 || j.width > 16384 || j.height > 16384) {
+      set_err(err, errlen, "batch job " + std::to_string(i) + ": " + std::to_string(j.width) + "x" +
+                               std::to_string(j.height) + " out of range 1..=16384");
+      return MM3E_ERR_INVALID;
+    }
+  }
+  std::lock_guard<std::mutex> lk(r->mu);
+  hipSetDevice(r->device);
+  auto* b = new mm3e_batch();
+  b->r = r;
+  b->in.assign(jobs, jobs + n_jobs);
+  b->order.resize(n_jobs);
+  for (uint32_t i = 0; i < n_jobs; i++) b->order[i] = i;
+  // group jobs of the same scene so a block rarely re-stages scene tables
+  std::stable_sort(b->order.begin(), b->order.end(),
+                   [&](uint32_t a, uint32_t c) { return (uintptr_t)b->in[a].scene < (uintptr_t)b->in[c].scene; });
+  b->pos_of.resize(n_jobs);
+  for (uint32_t k = 0; k < n_jobs; k++) b->pos_of[b->order[k]] = k;
+  for (const auto& j : b->in) b->total_pixels += (uint64_t)j.width * j.height;
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, r->device) == hipSuccess && p.multiProcessorCount > 0) b->cus = p.multiProcessorCount;
+  std::string msg;
+  b->arena_bytes = arena_size(b->in);
+  bool ok = hip_ok(hipMalloc(&b->d_arena, b->arena_bytes), "hipMalloc(batch framebuffers)", &msg) &&
+            hip_ok(hipMalloc(&b->d_jobs, n_jobs * sizeof(BatchJob)), "hipMalloc(jobs)", &msg) &&
+            hip_ok(hipMalloc(&b->d_posts, n_jobs * sizeof(BatchPost)), "hipMalloc(posts)", &msg) &&
+            hip_ok(hipMalloc(&b->d_counter, 256), "hipMalloc(counter)", &msg) &&
+            hip_ok(hipEventCreate(&b->ev0), "hipEventCreate", &msg) && hip_ok(hipEventCreate(&b->ev1), "hipEventCreate", &msg);
+  if (ok) {
+    build_tables(b);
+    ok = upload(b, msg) && capture(b, msg);
+  }
+  if (!ok) {
+    set_err(err, errlen, msg);
+    mm3e_batch_destroy(b);
+    return MM3E_ERR_DEVICE;
+  }
+  *out = b;
+  return MM3E_OK;
+}
+
+int mm3e_batch_set_camera(mm3e_batch* b, uint32_t job, const mm3e_camera* cam) {
+  if (!b || !cam || job >= b->in.size()) return MM3E_ERR_INVALID;
+  b->in[job].cam = *cam;
+  const uint32_t k = b->pos_of[job];
+  b->jobs[k].fp = make_frame(*cam, b->in[job].width, b->in[job].height);
+  b->jobs_dirty = true;
+  return MM3E_OK;
+}
+
+int mm3e_batch_launch(mm3e_batch* b) {
+  if (!b) return MM3E_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(b->r->mu);
+  hipSetDevice(b->r->device);
+  std::string msg;
+  if (b->jobs_dirty && !upload(b, msg)) return MM3E_ERR_DEVICE;
+  hipStream_t st = b->r->stream;
+  hipEventRecord(b->ev0, st);
+  if (!hip_ok(hipGraphLaunch(b->exec, st), "graph launch", &msg)) return MM3E_ERR_DEVICE;
+  hipEventRecord(b->ev1, st);
+  b->launched = true;
+  return MM3E_OK;
+}
+
+int mm3e_batch_wait(mm3e_batch* b, float* gpu_ms) {
+  if (!b || !b->launched) return MM3E_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(b->r->mu);
+  hipSetDevice(b->r->device);
+  if (hipEventSynchronize(b->ev1) != hipSuccess) return MM3E_ERR_DEVICE;
+  if (gpu_ms) {
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, b->ev0, b->ev1);
+    *gpu_ms = ms;
+  }
+  return MM3E_OK;
+}
+
+int mm3e_batch_read(mm3e_batch* b, uint32_t job, uint8_t* out, size_t out_len) {
+  if (!b || !out || job >= b->in.size()) return MM3E_ERR_INVALID;
+  const size_t need = (size_t)b->in[job].width * b->in[job].height * 4;
+  if (out_len < need) return MM3E_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(b->r->mu);
+  hipSetDevice(b->r->device);
+  const BatchPost& P = b->posts[b->pos_of[job]];
+  std::string msg;
+  if (!hip_ok(hipMemcpyAsync(out, P.rgba, need, hipMemcpyDeviceToHost, b->r->stream), "batch readback", &msg))
+    return MM3E_ERR_DEVICE;
+  return hipStreamSynchronize(b->r->stream) == hipSuccess ? MM3E_OK : MM3E_ERR_DEVICE;
+}
+
+int mm3e_batch_info(mm3e_batch* b, uint64_t* total_pixels, uint32_t* total_tiles, uint32_t* blocks_per_cu,
+                    uint32_t* tile_w, uint32_t* tile_h, uint32_t* lds_bytes) {
+  if (!b) return MM3E_ERR_INVALID;
+  if (total_pixels) *total_pixels = b->total_pixels;
+  if (total_tiles) *total_tiles = b->total_tiles;
+  if (blocks_per_cu) *blocks_per_cu = b->blocks_per_cu;
+  if (tile_w) *tile_w = b->tile_aa1[0];
+  if (tile_h) *tile_h = b->tile_aa1[1];
+  if (lds_bytes) *lds_bytes = b->lds_bytes;
+  return MM3E_OK;
+}
+
+int mm3e_batch_autotune(mm3e_batch* b, uint32_t reps, char* report, size_t report_len) {
+  if (!b) return MM3E_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(b->r->mu);
+  hipSetDevice(b->r->device);
+  reps = std::max(1u, reps);
+  static const uint32_t bpcs[] = {2, 3, 4, 6, 8};
+  static const uint32_t tiles[][2] = {{16, 16}, {32, 8}, {64, 4}, {8, 32}};
+  bool any_aa1 = false;
+  for (const auto& j : b->in) any_aa1 = any_aa1 || j.scene->aa <= 1;
+  const size_t n_tiles = any_aa1 ? sizeof(tiles) / sizeof(tiles[0]) : 1;
+  std::string msg, rep;
+  float best = 1e30f;
+  uint32_t best_bpc = b->blocks_per_cu, best_t[2] = {b->tile_aa1[0], b->tile_aa1[1]};
+  hipStream_t st = b->r->stream;
+  for (size_t ti = 0; ti < n_tiles; ti++) {
+    if (any_aa1) {
+      b->tile_aa1[0] = tiles[ti][0];
+      b->tile_aa1[1] = tiles[ti][1];
+    }
+    build_tables(b);
+    if (!upload(b, msg)) return MM3E_ERR_DEVICE;
+    for (uint32_t bpc : bpcs) {
+      b->blocks_per_cu = bpc;
+      if (!enqueue(b, msg)) return MM3E_ERR_DEVICE;  // warm-up
+      hipEventRecord(b->ev0, st);
+      for (uint32_t k = 0; k < reps; k++)
+        if (!enqueue(b, msg)) return MM3E_ERR_DEVICE;
+      hipEventRecord(b->ev1, st);
+      if (hipEventSynchronize(b->ev1) != hipSuccess) return MM3E_ERR_DEVICE;
+      float ms = 0.0f;
+      hipEventElapsedTime(&ms, b->ev0, b->ev1);
+      ms /= (float)reps;
+      char line[96];
+      std::snprintf(line, sizeof line, "tile %ux%u blocks/CU %u: %.3f ms\n", b->tile_aa1[0], b->tile_aa1[1], bpc, ms);
+      rep += line;
+      if (ms < best) {
+        best = ms;
+        best_bpc = bpc;
+        best_t[0] = b->tile_aa1[0];
+        best_t[1] = b->tile_aa1[1];
+      }
+    }
+  }
+  b->blocks_per_cu = best_bpc;
+  b->tile_aa1[0] = best_t[0];
+  b->tile_aa1[1] = best_t[1];
+  build_tables(b);
+  if (!upload(b, msg) || !capture(b, msg)) return MM3E_ERR_DEVICE;
+  char line[96];
+  std::snprintf(line, sizeof line, "chosen: tile %ux%u blocks/CU %u (%.3f ms)\n", best_t[0], best_t[1], best_bpc, best);
+  rep += line;
+  set_err(report, report_len, rep);
+  return MM3E_OK;
+}
+
+int mm3e_render_batch(mm3e_renderer* r, const mm3e_batch_job* jobs, uint32_t n_jobs, char* err, size_t errlen) {
+  mm3e_batch* b = nullptr;
+  int rc = mm3e_batch_create(r, jobs, n_jobs, &b, err, errlen);
+  if (rc != MM3E_OK) return rc;
+  rc = mm3e_batch_launch(b);
+  if (rc == MM3E_OK) rc = mm3e_batch_wait(b, nullptr);
+  for (uint32_t i = 0; rc == MM3E_OK && i < n_jobs; i++)
+    if (jobs[i].out) rc = mm3e_batch_read(b, i, jobs[i].out, (size_t)jobs[i].width * jobs[i].height * 4);
+  if (rc != MM3E_OK) set_err(err, errlen, "batched render failed on the device");
+  mm3e_batch_destroy(b);
+  return rc;
+}
+
+}  // extern "C"

@@ -224,10 +224,8 @@ MM3E_DEV V3 bright_pass(V3 c, float thr) {
 
 // Block = 256 lanes along x of one row; LDS holds the bright-pass of the row segment + halo.
 MM3E_DEV void bloom_h(const PostParams& pp, const float* __restrict__ kern, const float* __restrict__ hdr,
-                      float* __restrict__ tmp, float* lds) {
+                      float* __restrict__ tmp, float* lds, int x0, int y) {
   const int W = (int)pp.width, r = (int)pp.radius;
-  const int y = blockIdx.y;
-  const int x0 = blockIdx.x * blockDim.x;
   const int span = (int)blockDim.x + 2 * r;
   for (int i = threadIdx.x; i < span; i += blockDim.x) {
     int sx = x0 - r + i;
@@ -265,10 +263,9 @@ MM3E_DEV uchar4 tonemap_px(const PostParams& pp, V3 c) {
 
 // One lane per pixel; the vertical taps read the horizontally blurred image through L2.
 MM3E_DEV void bloom_v_resolve(const PostParams& pp, const float* __restrict__ kern, const float* __restrict__ hdr,
-                              const float* __restrict__ tmp, uchar4* __restrict__ rgba) {
+                              const float* __restrict__ tmp, uchar4* __restrict__ rgba, int x0, int y) {
   const int W = (int)pp.width, H = (int)pp.height, r = (int)pp.radius;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y;
+  const int x = x0 + (int)threadIdx.x;
   if (x >= W) return;
   V3 acc = V3{0.0f, 0.0f, 0.0f};
   const int n = 2 * r + 1;
@@ -283,10 +280,10 @@ MM3E_DEV void bloom_v_resolve(const PostParams& pp, const float* __restrict__ ke
   rgba[(size_t)y * W + x] = tonemap_px(pp, c);
 }
 
-MM3E_DEV void resolve_only(const PostParams& pp, const float* __restrict__ hdr, uchar4* __restrict__ rgba) {
+MM3E_DEV void resolve_only(const PostParams& pp, const float* __restrict__ hdr, uchar4* __restrict__ rgba, int x0,
+                           int y) {
   const int W = (int)pp.width;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y;
+  const int x = x0 + (int)threadIdx.x;
   if (x >= W) return;
   const float* s = hdr + 3 * ((size_t)y * W + x);
   rgba[(size_t)y * W + x] = tonemap_px(pp, V3{s[0], s[1], s[2]});

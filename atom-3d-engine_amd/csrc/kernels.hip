@@ -5,6 +5,7 @@
 //         -Xclang -target-feature -Xclang -packed-fp32-ops
 #include <hip/hip_runtime.h>
 
+#include "batch.h"
 #include "dev_render.h"
 #include "launch.h"
 
@@ -32,18 +33,46 @@ extern "C" __global__ void __launch_bounds__(256)
 extern "C" __global__ void __launch_bounds__(256)
     mm3e_bloom_h(PostParams pp, const float* __restrict__ kern, const float* __restrict__ hdr, float* __restrict__ tmp) {
   extern __shared__ float lds[];
-  bloom_h(pp, kern, hdr, tmp, lds);
+  bloom_h(pp, kern, hdr, tmp, lds, blockIdx.x * blockDim.x, blockIdx.y);
 }
 
 extern "C" __global__ void __launch_bounds__(256)
     mm3e_bloom_v_resolve(PostParams pp, const float* __restrict__ kern, const float* __restrict__ hdr,
                          const float* __restrict__ tmp, uchar4* __restrict__ rgba) {
-  bloom_v_resolve(pp, kern, hdr, tmp, rgba);
+  bloom_v_resolve(pp, kern, hdr, tmp, rgba, blockIdx.x * blockDim.x, blockIdx.y);
 }
 
 extern "C" __global__ void __launch_bounds__(256)
     mm3e_resolve_only(PostParams pp, const float* __restrict__ hdr, uchar4* __restrict__ rgba) {
-  resolve_only(pp, hdr, rgba);
+  resolve_only(pp, hdr, rgba, blockIdx.x * blockDim.x, blockIdx.y);
+}
+
+// Batched post: blockIdx.y enumerates the rows of all jobs (BatchPost::row_base), blockIdx.x
+// the 256-pixel segments of the widest job; blocks past a job's width leave at once.
+extern "C" __global__ void __launch_bounds__(256)
+    mm3e_batch_bloom_h(const BatchPost* __restrict__ posts, uint32_t n) {
+  extern __shared__ float lds[];
+  const uint32_t row = blockIdx.y;
+  const BatchPost& P = posts[find_job(posts, n, row, [](const BatchPost& b) { return b.row_base; })];
+  const int x0 = blockIdx.x * blockDim.x;
+  if (!P.pp.bloom || x0 >= (int)P.pp.width) return;  // uniform per block
+  bloom_h(P.pp, P.kern, P.hdr, P.tmp, lds, x0, (int)(row - P.row_base));
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+    mm3e_batch_resolve(const BatchPost* __restrict__ posts, uint32_t n) {
+  const uint32_t row = blockIdx.y;
+  const BatchPost& P = posts[find_job(posts, n, row, [](const BatchPost& b) { return b.row_base; })];
+  const int x0 = blockIdx.x * blockDim.x;
+  if (x0 >= (int)P.pp.width) return;
+  const int y = (int)(row - P.row_base);
+  if (P.pp.bloom) bloom_v_resolve(P.pp, P.kern, P.hdr, P.tmp, P.rgba, x0, y);
+  else resolve_only(P.pp, P.hdr, P.rgba, x0, y);
+}
+
+// Queue reset as a kernel (no memset node inside captured graphs).
+extern "C" __global__ void mm3e_batch_reset(uint32_t* counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0u;
 }
 
 extern "C" __global__ void __launch_bounds__(256)
@@ -110,6 +139,35 @@ hipError_t launch_post(hipStream_t st, const PostParams& pp, const float* kern, 
     hipLaunchKernelGGL(mm3e_resolve_only, grid, dim3(bx), 0, st, pp, hdr, rgba);
   }
   return hipGetLastError();
+}
+
+hipError_t launch_batch_post(hipStream_t st, const BatchPost* posts, uint32_t n, uint32_t total_rows, uint32_t max_w,
+                             uint32_t max_radius) {
+  const uint32_t bx = 256;
+  dim3 grid((max_w + bx - 1) / bx, total_rows);
+  size_t lds = (size_t)(bx + 2 * max_radius) * 3 * sizeof(float);
+  hipLaunchKernelGGL(mm3e_batch_bloom_h, grid, dim3(bx), lds, st, posts, n);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(mm3e_batch_resolve, grid, dim3(bx), 0, st, posts, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_reset(hipStream_t st, uint32_t* counter) {
+  hipLaunchKernelGGL(mm3e_batch_reset, dim3(1), dim3(64), 0, st, counter);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_render(uint32_t maxb, uint32_t blocks, hipStream_t st, const BatchJob* jobs, uint32_t n_jobs,
+                               uint32_t total_tiles, uint32_t* counter, uint32_t lds_bytes) {
+  switch (maxb) {
+    case 0:
+    case 1: return launch_batch_render_b1(blocks, st, jobs, n_jobs, total_tiles, counter, lds_bytes);
+    case 2: return launch_batch_render_b2(blocks, st, jobs, n_jobs, total_tiles, counter, lds_bytes);
+    case 3:
+    case 4: return launch_batch_render_b4(blocks, st, jobs, n_jobs, total_tiles, counter, lds_bytes);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_primary_aov_table(hipStream_t st, const DevScene* sc, const FrameParams& fp, uint8_t* hit,

@@ -22,7 +22,9 @@ EXPORTS = (
     "mm3e_list_adapters", "mm3e_scene_compile", "mm3e_scene_destroy", "mm3e_scene_render_rgba",
     "mm3e_scene_render_only", "mm3e_scene_render_linear", "mm3e_scene_render_aov", "mm3e_scene_gi_raw",
     "mm3e_scene_last_timing", "mm3e_scene_timing", "mm3e_scene_field", "mm3e_scene_march", "mm3e_camera_look_at",
-    "mm3e_orbit_camera", "mm3e_write_bmp", "mm3e_scene_check",
+    "mm3e_orbit_camera", "mm3e_write_bmp", "mm3e_scene_check", "mm3e_batch_create", "mm3e_batch_autotune",
+    "mm3e_batch_set_camera", "mm3e_batch_launch", "mm3e_batch_wait", "mm3e_batch_read", "mm3e_batch_info",
+    "mm3e_batch_destroy", "mm3e_render_batch",
 )
 
 _lib = None
@@ -64,6 +66,15 @@ def lib():
         L.mm3e_orbit_camera.restype = None
         L.mm3e_write_bmp.argtypes = [P, C.c_uint32, C.c_uint32, P, C.POINTER(C.c_size_t)]
         L.mm3e_scene_check.argtypes = [P, C.c_size_t, C.c_uint32, C.c_uint32, C.c_char_p, C.c_size_t]
+        L.mm3e_batch_create.argtypes = [P, P, C.c_uint32, C.POINTER(P), C.c_char_p, C.c_size_t]
+        L.mm3e_batch_autotune.argtypes = [P, C.c_uint32, C.c_char_p, C.c_size_t]
+        L.mm3e_batch_set_camera.argtypes = [P, C.c_uint32, P]
+        L.mm3e_batch_launch.argtypes = [P]
+        L.mm3e_batch_wait.argtypes = [P, P]
+        L.mm3e_batch_read.argtypes = [P, C.c_uint32, P, C.c_size_t]
+        L.mm3e_batch_info.argtypes = [P, P, P, P, P, P, P]
+        L.mm3e_batch_destroy.argtypes = [P]
+        L.mm3e_render_batch.argtypes = [P, P, C.c_uint32, C.c_char_p, C.c_size_t]
         _lib = L
     return _lib
 
@@ -265,3 +276,80 @@ class GpuScene:
         if rc != 0:
             raise Mm3eError(rc, "march probe failed")
         return t, hit.astype(bool), mat, nor, st
+
+
+class _BatchJob(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("cam", C.c_float * 13), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("out", C.c_void_p)]
+
+
+class Batch:
+    """Many (compiled scene, camera, size) jobs in one persistent tile-queue launch, replayed
+    from a captured hipGraph (mm3e_batch_* in include/mm3e_hip.h)."""
+
+    def __init__(self, renderer: Renderer, jobs):
+        """jobs: sequence of (GpuScene, camera[13], (width, height))."""
+        self.renderer = renderer
+        self.sizes = [(int(w), int(h)) for _, _, (w, h) in jobs]
+        self._scenes = [g for g, _, _ in jobs]  # keep compiled scenes alive
+        arr = (_BatchJob * len(jobs))()
+        for i, (g, cam, (w, h)) in enumerate(jobs):
+            arr[i].scene = g.h
+            arr[i].cam[:] = [float(x) for x in _cam(cam)]
+            arr[i].width, arr[i].height = int(w), int(h)
+            arr[i].out = None
+        h = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        rc = lib().mm3e_batch_create(renderer.h, C.cast(arr, C.c_void_p), len(jobs), C.byref(h), err, 1024)
+        if rc != 0:
+            raise Mm3eError(rc, err.value.decode())
+        self.h = h
+
+    def autotune(self, reps=2) -> str:
+        rep = C.create_string_buffer(4096)
+        rc = lib().mm3e_batch_autotune(self.h, reps, rep, 4096)
+        if rc != 0:
+            raise Mm3eError(rc, "autotune failed")
+        return rep.value.decode()
+
+    def set_camera(self, job, cam):
+        rc = lib().mm3e_batch_set_camera(self.h, job, _p(_cam(cam)))
+        if rc != 0:
+            raise Mm3eError(rc, "set_camera failed")
+
+    def launch(self):
+        rc = lib().mm3e_batch_launch(self.h)
+        if rc != 0:
+            raise Mm3eError(rc, "batch launch failed")
+
+    def wait(self) -> float:
+        ms = C.c_float(0)
+        rc = lib().mm3e_batch_wait(self.h, C.byref(ms))
+        if rc != 0:
+            raise Mm3eError(rc, "batch wait failed")
+        return ms.value
+
+    def read(self, job) -> np.ndarray:
+        w, h = self.sizes[job]
+        out = np.zeros((h, w, 4), np.uint8)
+        rc = lib().mm3e_batch_read(self.h, job, _p(out), out.nbytes)
+        if rc != 0:
+            raise Mm3eError(rc, "batch read failed")
+        return out
+
+    def info(self):
+        px, tiles, bpc, tw, th, lds = C.c_uint64(), C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        lib().mm3e_batch_info(self.h, C.byref(px), C.byref(tiles), C.byref(bpc), C.byref(tw), C.byref(th), C.byref(lds))
+        return {"pixels": px.value, "tiles": tiles.value, "blocks_per_cu": bpc.value, "tile": (tw.value, th.value),
+                "lds_bytes": lds.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mm3e_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

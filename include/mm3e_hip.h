@@ -124,6 +124,41 @@ void mm3e_orbit_camera(const float target[3], float radius, float yaw, float pit
  * only *out_len is set to the required size. */
 int mm3e_write_bmp(const uint8_t* rgba, uint32_t w, uint32_t h, uint8_t* out, size_t* out_len);
 
+/* ---- batched rendering ----------------------------------------------------------------
+ * Many (compiled scene, camera, size) jobs rendered by one persistent launch that pulls
+ * tiles from a global atomic queue, followed by one batched bloom/tonemap over all jobs;
+ * the three kernels are captured once into a hipGraph and replayed.  The reference renders
+ * one frame per dispatch (mm3e-gpu/src/lib.rs:328-356); every job's image equals
+ * mm3e_scene_render_rgba of the same scene at the job's size, bit for bit (a job may use a
+ * size other than the one its scene was compiled for).  Jobs keep their own framebuffers. */
+typedef struct mm3e_batch mm3e_batch;
+typedef struct mm3e_batch_job {
+  mm3e_scene* scene;  /* compiled on the same renderer */
+  mm3e_camera cam;
+  uint32_t width, height;
+  uint8_t* out;       /* RGBA8 destination for mm3e_render_batch (may be NULL) */
+} mm3e_batch_job;
+
+int mm3e_batch_create(mm3e_renderer* r, const mm3e_batch_job* jobs, uint32_t n_jobs, mm3e_batch** out, char* err,
+                      size_t errlen);
+/* Time the persistent launch over blocks-per-CU {2,3,4,6,8} and, when single-sample jobs are
+ * present, tile shapes {16x16, 32x8, 64x4, 8x32}; keep the fastest and re-capture the graph.
+ * A human-readable table of the trials is written to report. */
+int mm3e_batch_autotune(mm3e_batch* b, uint32_t reps, char* report, size_t report_len);
+/* Change one job's camera for the next launch (the job table is re-uploaded, the graph kept). */
+int mm3e_batch_set_camera(mm3e_batch* b, uint32_t job, const mm3e_camera* cam);
+/* Enqueue one replay of the batch graph; returns without waiting. */
+int mm3e_batch_launch(mm3e_batch* b);
+/* Wait for the last launch; *gpu_ms = its device time (queue reset + render + post). */
+int mm3e_batch_wait(mm3e_batch* b, float* gpu_ms);
+/* Copy job `job`'s RGBA8 image (width*height*4 bytes) to host memory. */
+int mm3e_batch_read(mm3e_batch* b, uint32_t job, uint8_t* out, size_t out_len);
+int mm3e_batch_info(mm3e_batch* b, uint64_t* total_pixels, uint32_t* total_tiles, uint32_t* blocks_per_cu,
+                    uint32_t* tile_w, uint32_t* tile_h, uint32_t* lds_bytes);
+void mm3e_batch_destroy(mm3e_batch* b);
+/* One-shot: create, launch, wait, read every job with out != NULL, destroy. */
+int mm3e_render_batch(mm3e_renderer* r, const mm3e_batch_job* jobs, uint32_t n_jobs, char* err, size_t errlen);
+
 #ifdef __cplusplus
 }
 #endif

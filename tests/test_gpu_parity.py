@@ -178,3 +178,48 @@ def test_compile_errors_on_device(renderer):
     with pytest.raises(hip.Mm3eError) as e:
         hip.Renderer("no-such-adapter-xyz")
     assert "no GPU adapter matching 'no-such-adapter-xyz'; available:" in str(e.value)
+
+
+def test_batch_equals_per_frame_renders_across_replays(renderer):
+    """Every job of a batched launch equals the plain per-frame render of the same scene at the
+    job's size, bit for bit, on every graph replay (cameras change between replays)."""
+    specs = []
+    scenes = []
+    for seed, n in ((51, 40), (52, 90)):
+        sc = configs.procedural_scene(seed, n, 320, 180)
+        scenes.append(sc)
+    c2, _ = configs.c2_scene(width=160, height=90, gi_dims=(4, 4, 4), gi_rays=5)  # aa = 2 job
+    scenes.append(c2)
+    compiled = [renderer.compile(sc.to_ir(), sc.width, sc.height) for sc in scenes]
+    rng = configs.SplitMix64(77)
+    sizes = [(320, 180), (200, 120), (96, 54), (320, 180), (160, 90), (130, 77)]
+    jobs = []
+    for i, (w, h) in enumerate(sizes):
+        k = i % len(scenes)
+        jobs.append((compiled[k], configs.orbit_pose(rng), (w, h)))
+    b = hip.Batch(renderer, jobs)
+    for rep in range(3):
+        if rep:
+            for i in range(len(jobs)):
+                cam = configs.orbit_pose(rng)
+                jobs[i] = (jobs[i][0], cam, jobs[i][2])
+                b.set_camera(i, cam)
+        b.launch()
+        b.wait()
+        for i, (g, cam, (w, h)) in enumerate(jobs):
+            k = i % len(scenes)
+            plain = renderer.compile(scenes[k].to_ir(), w, h)
+            ref = plain.render_rgba(cam)
+            got = b.read(i)
+            assert (got == ref).all(), f"replay {rep} job {i}"
+            plain.close()
+    report = b.autotune(reps=1)
+    print("\n" + report, flush=True)
+    b.launch()
+    b.wait()
+    for i, (g, cam, (w, h)) in enumerate(jobs):
+        k = i % len(scenes)
+        plain = renderer.compile(scenes[k].to_ir(), w, h)
+        assert (b.read(i) == plain.render_rgba(cam)).all(), f"after autotune job {i}"
+        plain.close()
+    b.close()
