@@ -194,11 +194,15 @@ MM3E_DEV float vol_sample(const DVol& v, const float* __restrict__ data, V3 p) {
 // lib.rs:400-419 object-local point.
 MM3E_DEV V3 local_point(const DObj& o, V3 p) {
   V3 v = p - ld3(o.pos);
-  // Rᵀ·v: component i is dot(column_i, v) (vec.rs:125-133, 216-221)
-  V3 l = V3{(o.c0[0] * v.x + o.c0[1] * v.y) + o.c0[2] * v.z, (o.c1[0] * v.x + o.c1[1] * v.y) + o.c1[2] * v.z,
-            (o.c2[0] * v.x + o.c2[1] * v.y) + o.c2[2] * v.z};
-  l = scale(l, o.inv);
   const uint32_t f = o.flags;
+  // Rᵀ·v: component i is dot(column_i, v) (vec.rs:125-133, 216-221).  For the identity,
+  // (1·x + 0·y) + 0·z == x for finite y, z (only the sign of a zero x can differ), and a unit
+  // scale multiplies by exactly 1: both are skipped.
+  V3 l = v;
+  if (!(f & OBJ_IDENT_ROT))
+    l = V3{(o.c0[0] * v.x + o.c0[1] * v.y) + o.c0[2] * v.z, (o.c1[0] * v.x + o.c1[1] * v.y) + o.c1[2] * v.z,
+           (o.c2[0] * v.x + o.c2[1] * v.y) + o.c2[2] * v.z};
+  if (!(f & OBJ_UNIT_SCALE)) l = scale(l, o.inv);
   if (f & (OBJ_MIRROR_X | OBJ_MIRROR_Y | OBJ_MIRROR_Z)) {
     if (f & OBJ_MIRROR_X) l.x = rabs(l.x);
     if (f & OBJ_MIRROR_Y) l.y = rabs(l.y);
@@ -237,7 +241,7 @@ MM3E_DEV float object_distance(const DevScene& sc, const DObj& o, V3 p) {
   } else {
     d = prim_distance(kind, o.p, l);
   }
-  d = d * o.scale;
+  if (!(o.flags & OBJ_UNIT_SCALE)) d = d * o.scale;
   if (o.flags & OBJ_ROUND) d = d - o.round;
   if (o.flags & OBJ_ONION) d = rabs(d) - o.onion;
   return d;

@@ -24,6 +24,8 @@ enum : uint32_t {
   OBJ_ROUND = 1u << 11,
   OBJ_ONION = 1u << 12,
   OBJ_BOUNDED = 1u << 13,
+  OBJ_IDENT_ROT = 1u << 14,  // rotation is the identity: to_local skips the matrix product
+  OBJ_UNIT_SCALE = 1u << 15, // scale == 1: to_local and Object::field skip the two multiplies
   OBJ_COMBINE_SHIFT = 16,    // bits 16-17: mm3e_combine of this object
 };
 

@@ -384,6 +384,9 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
     if (o.round != 0.0f) f |= OBJ_ROUND;
     if (o.onion != 0.0f) f |= OBJ_ONION;
     f |= (o.combine & 3u) << OBJ_COMBINE_SHIFT;
+    static const float ident[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    if (std::memcmp(o.rot, ident, sizeof ident) == 0) f |= OBJ_IDENT_ROT;
+    if (o.scale == 1.0f) f |= OBJ_UNIT_SCALE;
     bounds[i] = world_bound(sc, o);
     if (bounds[i].some) {
       f |= OBJ_BOUNDED;

@@ -1,0 +1,448 @@
+// serve.cpp — a render server with continuous batching.
+//
+// Requests (resident compiled scene, camera, size) arrive from any host thread.  A worker
+// thread admits them FIFO into batches while the HBM budget allows (framebuffers for a
+// request: w*h*(12 + 12 + 4) bytes, carved from one slab allocated at start-up), launches each
+// batch as the persistent tile-queue render + batched post (the kernels of batch_kernel.hip)
+// followed by the RGBA8 copies into pinned staging, and keeps up to `max_inflight` batches in
+// flight on separate default-priority streams.  Requests that arrive while a batch runs are
+// admitted into the next batch as soon as a stream slot and memory are free, so their work
+// overlaps the running batch's tail on the device.  Completion is detected with events;
+// latency = completion - submission (steady clock).  A request's image is the same as its
+// per-frame render (batched jobs are bit-exact to mm3e_scene_render_rgba).
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <thread>
+#include <vector>
+
+#include "batch.h"
+#include "launch.h"
+#include "runtime.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+  uint64_t id;
+  mm3e_scene* scene;  // resolved at submission, under the server lock
+  uint32_t scene_id, w, h;
+  mm3e_camera cam;
+  uint8_t* out;
+  Clock::time_point arrival, done;
+  size_t bytes = 0, off = 0;   // slab region
+  uint8_t* staging = nullptr;  // pinned RGBA8
+  int status = MM3E_OK;
+  bool complete = false;
+};
+
+// First-fit allocator over one HBM slab (256-byte granules).
+struct Slab {
+  uint8_t* base = nullptr;
+  size_t size = 0;
+  std::map<size_t, size_t> free_;  // offset -> length
+  void init(uint8_t* b, size_t n) {
+    base = b;
+    size = n;
+    free_.clear();
+    free_[0] = n;
+  }
+  bool alloc(size_t n, size_t* off) {
+    n = (n + 255) & ~size_t(255);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= n) {
+        *off = it->first;
+        size_t rest = it->second - n, roff = it->first + n;
+        free_.erase(it);
+        if (rest) free_[roff] = rest;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(size_t off, size_t n) {
+    n = (n + 255) & ~size_t(255);
+    auto it = free_.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  mm3e::BatchJob* d_jobs = nullptr;
+  mm3e::BatchPost* d_posts = nullptr;
+  uint32_t* d_counter = nullptr;
+  std::vector<mm3e::BatchJob> jobs;
+  std::vector<mm3e::BatchPost> posts;
+  std::vector<Request*> reqs;
+  bool busy = false;
+};
+
+}  // namespace
+
+struct mm3e_server {
+  mm3e_renderer* r = nullptr;
+  mm3e_server_config cfg{};
+  std::vector<mm3e_scene*> scenes;
+  Slab slab;
+  uint8_t* d_slab = nullptr;
+  std::vector<Slot> slots;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request*> pending;
+  std::map<uint64_t, Request*> all;
+  uint64_t next_id = 1;
+  bool stop = false;
+  std::thread worker;
+  uint32_t cus = 256;
+  std::multimap<size_t, uint8_t*> pinned_free;  // reusable pinned staging buffers by size
+  // stats
+  uint64_t batches = 0, completed = 0, rejected = 0, max_batch_seen = 0;
+  uint64_t pixels_done = 0;
+};
+
+namespace {
+
+uint8_t* pinned_get(mm3e_server* S, size_t n) {
+  auto it = S->pinned_free.find(n);
+  if (it != S->pinned_free.end()) {
+    uint8_t* p = it->second;
+    S->pinned_free.erase(it);
+    return p;
+  }
+  uint8_t* p = nullptr;
+  return hipHostMalloc((void**)&p, n, hipHostMallocDefault) == hipSuccess ? p : nullptr;
+}
+void pinned_put(mm3e_server* S, uint8_t* p, size_t n) {
+  if (p) S->pinned_free.emplace(n, p);
+}
+
+bool launch_slot(mm3e_server* S, Slot& sl, std::string& msg) {
+  using namespace mm3e;
+  sl.jobs.clear();
+  sl.posts.clear();
+  // group by scene
+  std::stable_sort(sl.reqs.begin(), sl.reqs.end(),
+                   [&](Request* a, Request* b) { return a->scene_id < b->scene_id; });
+  uint32_t tile = 0, row = 0, max_w = 0, max_r = 1, maxb = 0, stage_max = 0;
+  bool all_fit = true;
+  for (Request* q : sl.reqs) {
+    mm3e_scene* s = q->scene;
+    BatchJob J{};
+    J.sc = s->d_scene;
+    J.fp = make_frame(q->cam, q->w, q->h);
+    if (s->aa <= 1) {
+      J.tw = 16;
+      J.th = 16;
+    } else {
+      tile_shape(s->aa, &J.tw, &J.th);
+    }
+    J.tiles_x = (q->w + J.tw - 1) / J.tw;
+    J.tile_base = tile;
+    tile += J.tiles_x * ((q->h + J.th - 1) / J.th);
+    J.stage_bytes = s->hdr.stage_bytes;
+    stage_max = std::max(stage_max, J.stage_bytes);
+    all_fit = all_fit && J.stage_bytes <= S->r->lds_budget;
+    const size_t px = (size_t)q->w * q->h;
+    uint8_t* base = S->d_slab + q->off;
+    J.hdr = (float*)base;
+    BatchPost P{};
+    P.pp = s->pp;
+    P.pp.width = q->w;
+    P.pp.height = q->h;
+    P.kern = s->d_kern;
+    P.hdr = J.hdr;
+    P.tmp = (float*)(base + px * 12);
+    P.rgba = (uchar4*)(base + px * 24);
+    P.row_base = row;
+    row += q->h;
+    max_w = std::max(max_w, q->w);
+    max_r = std::max(max_r, P.pp.radius);
+    maxb = std::max(maxb, s->bounces);
+    sl.jobs.push_back(J);
+    sl.posts.push_back(P);
+  }
+  const uint32_t lds = all_fit ? stage_max : 0;
+  hipStream_t st = sl.stream;
+  if (!hip_ok(hipMemcpyAsync(sl.d_jobs, sl.jobs.data(), sl.jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice, st),
+              "serve: upload jobs", &msg))
+    return false;
+  if (!hip_ok(hipMemcpyAsync(sl.d_posts, sl.posts.data(), sl.posts.size() * sizeof(BatchPost), hipMemcpyHostToDevice,
+                             st),
+              "serve: upload posts", &msg))
+    return false;
+  if (!hip_ok(launch_batch_reset(st, sl.d_counter), "serve: reset", &msg)) return false;
+  if (!hip_ok(launch_batch_render(maxb, S->cus * S->cfg.blocks_per_cu, st, sl.d_jobs, (uint32_t)sl.jobs.size(), tile,
+                                  sl.d_counter, lds),
+              "serve: render", &msg))
+    return false;
+  if (!hip_ok(launch_batch_post(st, sl.d_posts, (uint32_t)sl.posts.size(), row, max_w, max_r), "serve: post", &msg))
+    return false;
+  for (size_t k = 0; k < sl.reqs.size(); k++) {
+    Request* q = sl.reqs[k];
+    if (!hip_ok(hipMemcpyAsync(q->staging, sl.posts[k].rgba, (size_t)q->w * q->h * 4, hipMemcpyDeviceToHost, st),
+                "serve: readback", &msg))
+      return false;
+  }
+  return hip_ok(hipEventRecord(sl.done, st), "serve: event", &msg);
+}
+
+void worker_main(mm3e_server* S) {
+  hipSetDevice(S->r->device);
+  std::unique_lock<std::mutex> lk(S->mu);
+  while (true) {
+    // 1. reap finished batches
+    bool progressed = false;
+    for (Slot& sl : S->slots) {
+      if (!sl.busy) continue;
+      hipError_t e = hipEventQuery(sl.done);
+      if (e == hipErrorNotReady) continue;
+      const auto now = Clock::now();
+      for (Request* q : sl.reqs) {
+        q->status = e == hipSuccess ? MM3E_OK : MM3E_ERR_DEVICE;
+        if (q->out && e == hipSuccess) std::memcpy(q->out, q->staging, (size_t)q->w * q->h * 4);
+        q->done = now;
+        q->complete = true;
+        S->slab.release(q->off, q->bytes);
+        pinned_put(S, q->staging, (size_t)q->w * q->h * 4);
+        q->staging = nullptr;
+        S->completed++;
+        S->pixels_done += (uint64_t)q->w * q->h;
+      }
+      sl.reqs.clear();
+      sl.busy = false;
+      progressed = true;
+    }
+    if (progressed) S->cv_done.notify_all();
+    // 2. admit pending requests into free slots
+    for (Slot& sl : S->slots) {
+      if (sl.busy || S->pending.empty()) continue;
+      while (!S->pending.empty() && sl.reqs.size() < S->cfg.max_batch_requests) {
+        Request* q = S->pending.front();
+        if (q->bytes > S->slab.size) {  // can never fit: reject
+          S->pending.pop_front();
+          q->status = MM3E_ERR_BUSY;
+          q->done = Clock::now();
+          q->complete = true;
+          S->rejected++;
+          progressed = true;
+          continue;
+        }
+        size_t off;
+        if (!S->slab.alloc(q->bytes, &off)) break;  // wait for memory
+        q->staging = pinned_get(S, (size_t)q->w * q->h * 4);
+        if (!q->staging) {
+          S->slab.release(off, q->bytes);
+          break;
+        }
+        q->off = off;
+        S->pending.pop_front();
+        sl.reqs.push_back(q);
+      }
+      if (sl.reqs.empty()) continue;
+      std::string msg;
+      lk.unlock();
+      bool ok = launch_slot(S, sl, msg);
+      lk.lock();
+      if (!ok) {
+        for (Request* q : sl.reqs) {
+          q->status = MM3E_ERR_DEVICE;
+          q->done = Clock::now();
+          q->complete = true;
+          S->slab.release(q->off, q->bytes);
+          pinned_put(S, q->staging, (size_t)q->w * q->h * 4);
+          q->staging = nullptr;
+        }
+        sl.reqs.clear();
+        S->cv_done.notify_all();
+        continue;
+      }
+      sl.busy = true;
+      S->batches++;
+      S->max_batch_seen = std::max<uint64_t>(S->max_batch_seen, sl.reqs.size());
+    }
+    bool any_busy = false;
+    for (Slot& sl : S->slots) any_busy = any_busy || sl.busy;
+    if (S->stop && !any_busy && S->pending.empty()) break;
+    if (progressed) S->cv_done.notify_all();
+    // 3. sleep briefly while batches run (event polling), or until new work arrives
+    if (any_busy) S->cv_work.wait_for(lk, std::chrono::microseconds(50));
+    else if (S->pending.empty()) S->cv_work.wait_for(lk, std::chrono::milliseconds(5));
+  }
+}
+
+}  // namespace
+
+using namespace mm3e;
+
+extern "C" {
+
+void mm3e_server_default_config(mm3e_server_config* cfg) {
+  if (!cfg) return;
+  cfg->hbm_budget_bytes = (uint64_t)4 << 30;
+  cfg->max_batch_requests = 64;
+  cfg->max_inflight = 2;
+  cfg->queue_capacity = 4096;
+  cfg->blocks_per_cu = 4;
+}
+
+int mm3e_server_create(mm3e_renderer* r, const mm3e_server_config* cfg, mm3e_server** out, char* err, size_t errlen) {
+  if (!r || !out) return MM3E_ERR_INVALID;
+  *out = nullptr;
+  mm3e_server_config c;
+  mm3e_server_default_config(&c);
+  if (cfg) c = *cfg;
+  if (c.max_inflight == 0 || c.max_inflight > 8 || c.max_batch_requests == 0 || c.hbm_budget_bytes < (1u << 20) ||
+      c.blocks_per_cu == 0) {
+    set_err(err, errlen, "server config out of range (max_inflight 1..8, max_batch_requests >= 1, budget >= 1 MiB)");
+    return MM3E_ERR_INVALID;
+  }
+  auto* S = new mm3e_server();
+  S->r = r;
+  S->cfg = c;
+  hipSetDevice(r->device);
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, r->device) == hipSuccess && p.multiProcessorCount > 0) S->cus = p.multiProcessorCount;
+  std::string msg;
+  bool ok = hip_ok(hipMalloc(&S->d_slab, c.hbm_budget_bytes), "hipMalloc(serve slab)", &msg);
+  if (ok) S->slab.init(S->d_slab, c.hbm_budget_bytes);
+  S->slots.resize(c.max_inflight);
+  for (Slot& sl : S->slots) {
+    if (!ok) break;
+    ok = hip_ok(hipStreamCreate(&sl.stream), "hipStreamCreate", &msg) &&
+         hip_ok(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming), "hipEventCreate", &msg) &&
+         hip_ok(hipMalloc(&sl.d_jobs, c.max_batch_requests * sizeof(BatchJob)), "hipMalloc(jobs)", &msg) &&
+         hip_ok(hipMalloc(&sl.d_posts, c.max_batch_requests * sizeof(BatchPost)), "hipMalloc(posts)", &msg) &&
+         hip_ok(hipMalloc(&sl.d_counter, 256), "hipMalloc(counter)", &msg);
+  }
+  if (!ok) {
+    set_err(err, errlen, msg);
+    for (Slot& sl : S->slots) {
+      if (sl.d_jobs) hipFree(sl.d_jobs);
+      if (sl.d_posts) hipFree(sl.d_posts);
+      if (sl.d_counter) hipFree(sl.d_counter);
+      if (sl.done) hipEventDestroy(sl.done);
+      if (sl.stream) hipStreamDestroy(sl.stream);
+    }
+    if (S->d_slab) hipFree(S->d_slab);
+    delete S;
+    return MM3E_ERR_DEVICE;
+  }
+  S->worker = std::thread(worker_main, S);
+  *out = S;
+  return MM3E_OK;
+}
+
+int mm3e_server_add_scene(mm3e_server* S, mm3e_scene* scene, uint32_t* scene_id) {
+  if (!S || !scene || !scene_id || scene->r != S->r) return MM3E_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->scenes.push_back(scene);
+  *scene_id = (uint32_t)(S->scenes.size() - 1);
+  return MM3E_OK;
+}
+
+int mm3e_server_submit(mm3e_server* S, uint32_t scene_id, const mm3e_camera* cam, uint32_t w, uint32_t h, uint8_t* out,
+                       uint64_t* request_id) {
+  if (!S || !cam || !request_id) return MM3E_ERR_INVALID;
+  if (w == 0 || h == 0 || w > 16384 || h > 16384) return MM3E_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (scene_id >= S->scenes.size()) return MM3E_ERR_INVALID;
+  if (S->pending.size() >= S->cfg.queue_capacity) {
+    S->rejected++;
+    return MM3E_ERR_BUSY;  // queue full: the caller may retry
+  }
+  auto* q = new Request();
+  q->id = S->next_id++;
+  q->scene = S->scenes[scene_id];
+  q->scene_id = scene_id;
+  q->w = w;
+  q->h = h;
+  q->cam = *cam;
+  q->out = out;
+  q->bytes = (size_t)w * h * 28;
+  q->arrival = Clock::now();
+  S->pending.push_back(q);
+  S->all[q->id] = q;
+  *request_id = q->id;
+  S->cv_work.notify_one();
+  return MM3E_OK;
+}
+
+int mm3e_server_wait(mm3e_server* S, uint64_t request_id, double* latency_ms) {
+  if (!S) return MM3E_ERR_INVALID;
+  std::unique_lock<std::mutex> lk(S->mu);
+  auto it = S->all.find(request_id);
+  if (it == S->all.end()) return MM3E_ERR_INVALID;
+  Request* q = it->second;
+  S->cv_done.wait(lk, [&] { return q->complete; });
+  if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(q->done - q->arrival).count();
+  const int st = q->status;
+  S->all.erase(it);
+  delete q;
+  return st;
+}
+
+int mm3e_server_poll(mm3e_server* S, uint64_t request_id, int* done) {
+  if (!S || !done) return MM3E_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->all.find(request_id);
+  if (it == S->all.end()) return MM3E_ERR_INVALID;
+  *done = it->second->complete ? 1 : 0;
+  return MM3E_OK;
+}
+
+int mm3e_server_stats(mm3e_server* S, uint64_t* batches, uint64_t* completed, uint64_t* rejected,
+                      uint64_t* max_batch, uint64_t* pixels) {
+  if (!S) return MM3E_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (batches) *batches = S->batches;
+  if (completed) *completed = S->completed;
+  if (rejected) *rejected = S->rejected;
+  if (max_batch) *max_batch = S->max_batch_seen;
+  if (pixels) *pixels = S->pixels_done;
+  return MM3E_OK;
+}
+
+void mm3e_server_destroy(mm3e_server* S) {
+  if (!S) return;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+    S->cv_work.notify_all();
+  }
+  if (S->worker.joinable()) S->worker.join();
+  hipSetDevice(S->r->device);
+  for (Slot& sl : S->slots) {
+    if (sl.stream) hipStreamSynchronize(sl.stream);
+    if (sl.d_jobs) hipFree(sl.d_jobs);
+    if (sl.d_posts) hipFree(sl.d_posts);
+    if (sl.d_counter) hipFree(sl.d_counter);
+    if (sl.done) hipEventDestroy(sl.done);
+    if (sl.stream) hipStreamDestroy(sl.stream);
+  }
+  for (auto& kv : S->all) {
+    if (kv.second->staging) hipHostFree(kv.second->staging);
+    delete kv.second;
+  }
+  for (auto& kv : S->pinned_free) hipHostFree(kv.second);
+  if (S->d_slab) hipFree(S->d_slab);
+  delete S;
+}
+
+}  // extern "C"

@@ -24,7 +24,9 @@ EXPORTS = (
     "mm3e_scene_last_timing", "mm3e_scene_timing", "mm3e_scene_field", "mm3e_scene_march", "mm3e_camera_look_at",
     "mm3e_orbit_camera", "mm3e_write_bmp", "mm3e_scene_check", "mm3e_batch_create", "mm3e_batch_autotune",
     "mm3e_batch_set_camera", "mm3e_batch_launch", "mm3e_batch_wait", "mm3e_batch_read", "mm3e_batch_info",
-    "mm3e_batch_destroy", "mm3e_render_batch",
+    "mm3e_batch_destroy", "mm3e_render_batch", "mm3e_server_default_config", "mm3e_server_create",
+    "mm3e_server_add_scene", "mm3e_server_submit", "mm3e_server_poll", "mm3e_server_wait", "mm3e_server_stats",
+    "mm3e_server_destroy",
 )
 
 _lib = None
@@ -75,6 +77,16 @@ def lib():
         L.mm3e_batch_info.argtypes = [P, P, P, P, P, P, P]
         L.mm3e_batch_destroy.argtypes = [P]
         L.mm3e_render_batch.argtypes = [P, P, C.c_uint32, C.c_char_p, C.c_size_t]
+        L.mm3e_server_default_config.argtypes = [P]
+        L.mm3e_server_default_config.restype = None
+        L.mm3e_server_create.argtypes = [P, P, C.POINTER(P), C.c_char_p, C.c_size_t]
+        L.mm3e_server_add_scene.argtypes = [P, P, C.POINTER(C.c_uint32)]
+        L.mm3e_server_submit.argtypes = [P, C.c_uint32, P, C.c_uint32, C.c_uint32, P, C.POINTER(C.c_uint64)]
+        L.mm3e_server_poll.argtypes = [P, C.c_uint64, C.POINTER(C.c_int)]
+        L.mm3e_server_wait.argtypes = [P, C.c_uint64, C.POINTER(C.c_double)]
+        L.mm3e_server_stats.argtypes = [P, P, P, P, P, P]
+        L.mm3e_server_destroy.argtypes = [P]
+        L.mm3e_server_destroy.restype = None
         _lib = L
     return _lib
 
@@ -346,6 +358,77 @@ class Batch:
     def close(self):
         if getattr(self, "h", None):
             lib().mm3e_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch_requests", C.c_uint32), ("max_inflight", C.c_uint32),
+                ("queue_capacity", C.c_uint32), ("blocks_per_cu", C.c_uint32)]
+
+
+class Server:
+    """Continuous-batching render server over resident compiled scenes (mm3e_server_*)."""
+
+    def __init__(self, renderer: Renderer, **cfg):
+        c = ServerConfig()
+        lib().mm3e_server_default_config(C.byref(c))
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        h = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        rc = lib().mm3e_server_create(renderer.h, C.byref(c), C.byref(h), err, 1024)
+        if rc != 0:
+            raise Mm3eError(rc, err.value.decode())
+        self.h = h
+        self.renderer = renderer
+        self._scenes = []
+        self._outs = {}
+
+    def add_scene(self, g: GpuScene) -> int:
+        sid = C.c_uint32()
+        rc = lib().mm3e_server_add_scene(self.h, g.h, C.byref(sid))
+        if rc != 0:
+            raise Mm3eError(rc, "add_scene failed")
+        self._scenes.append(g)
+        return sid.value
+
+    def submit(self, scene_id, cam, w, h, keep_image=True) -> int:
+        out = np.zeros((h, w, 4), np.uint8) if keep_image else None
+        rid = C.c_uint64()
+        rc = lib().mm3e_server_submit(self.h, scene_id, _p(_cam(cam)), w, h, _p(out), C.byref(rid))
+        if rc != 0:
+            raise Mm3eError(rc, "submit refused (queue full)" if rc == 3 else "submit failed")
+        self._outs[rid.value] = out
+        return rid.value
+
+    def poll(self, rid) -> bool:
+        d = C.c_int(0)
+        lib().mm3e_server_poll(self.h, rid, C.byref(d))
+        return bool(d.value)
+
+    def wait(self, rid):
+        """-> (image or None, latency_ms)."""
+        lat = C.c_double(0)
+        rc = lib().mm3e_server_wait(self.h, rid, C.byref(lat))
+        out = self._outs.pop(rid, None)
+        if rc != 0:
+            raise Mm3eError(rc, f"request {rid} failed")
+        return out, lat.value
+
+    def stats(self):
+        v = [C.c_uint64() for _ in range(5)]
+        lib().mm3e_server_stats(self.h, *[C.byref(x) for x in v])
+        return dict(zip(("batches", "completed", "rejected", "max_batch", "pixels"), [x.value for x in v]))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mm3e_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -159,6 +159,39 @@ void mm3e_batch_destroy(mm3e_batch* b);
 /* One-shot: create, launch, wait, read every job with out != NULL, destroy. */
 int mm3e_render_batch(mm3e_renderer* r, const mm3e_batch_job* jobs, uint32_t n_jobs, char* err, size_t errlen);
 
+/* ---- serving: continuous batching over resident scenes ---------------------------------
+ * Requests (resident compiled scene, camera, size) may be submitted from any thread.  A
+ * worker thread admits them FIFO into batches while the HBM budget allows (a request needs
+ * w*h*28 bytes of framebuffers, carved from one slab allocated at creation), renders each
+ * batch with the persistent tile-queue kernels, and keeps up to max_inflight batches on the
+ * device at once (separate default-priority streams); requests arriving meanwhile join the
+ * next batch.  A request's image equals its per-frame render.  Latency = completion - submit. */
+typedef struct mm3e_server mm3e_server;
+typedef struct mm3e_server_config {
+  uint64_t hbm_budget_bytes;    /* framebuffer slab (default 4 GiB) */
+  uint32_t max_batch_requests;  /* requests per batch (default 64) */
+  uint32_t max_inflight;        /* batches on the device at once, 1..8 (default 2) */
+  uint32_t queue_capacity;      /* pending requests before submit returns MM3E_ERR_BUSY (4096) */
+  uint32_t blocks_per_cu;       /* persistent blocks per CU (default 4) */
+} mm3e_server_config;
+
+void mm3e_server_default_config(mm3e_server_config* cfg);
+int mm3e_server_create(mm3e_renderer* r, const mm3e_server_config* cfg, mm3e_server** out, char* err, size_t errlen);
+/* Register a compiled scene (it stays resident; the caller keeps it alive). */
+int mm3e_server_add_scene(mm3e_server* s, mm3e_scene* scene, uint32_t* scene_id);
+/* Queue a request; out (w*h*4 bytes, may be NULL) is written before the request completes.
+ * MM3E_ERR_BUSY when the queue is full; a request larger than the whole budget completes
+ * with MM3E_ERR_BUSY. */
+int mm3e_server_submit(mm3e_server* s, uint32_t scene_id, const mm3e_camera* cam, uint32_t w, uint32_t h, uint8_t* out,
+                       uint64_t* request_id);
+int mm3e_server_poll(mm3e_server* s, uint64_t request_id, int* done);
+/* Block until the request completes; returns its status and releases it. */
+int mm3e_server_wait(mm3e_server* s, uint64_t request_id, double* latency_ms);
+int mm3e_server_stats(mm3e_server* s, uint64_t* batches, uint64_t* completed, uint64_t* rejected, uint64_t* max_batch,
+                      uint64_t* pixels);
+/* Finishes queued work, then stops the worker and frees device memory. */
+void mm3e_server_destroy(mm3e_server* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -223,3 +223,40 @@ def test_batch_equals_per_frame_renders_across_replays(renderer):
         assert (b.read(i) == plain.render_rgba(cam)).all(), f"after autotune job {i}"
         plain.close()
     b.close()
+
+
+def test_server_results_equal_alone_and_in_mix(renderer):
+    """Continuous batching: requests submitted in a burst (so they share batches and overlap
+    on the device) return exactly the per-frame render of the same request alone."""
+    scenes = [configs.procedural_scene(61 + i, 30 + 20 * i, 200, 120) for i in range(3)]
+    compiled = [renderer.compile(sc.to_ir(), 200, 120) for sc in scenes]
+    srv = hip.Server(renderer, hbm_budget_bytes=256 << 20, max_batch_requests=8, max_inflight=2)
+    ids = [srv.add_scene(g) for g in compiled]
+    rng = configs.SplitMix64(99)
+    sizes = [(200, 120), (96, 64), (160, 90), (200, 120), (64, 48), (128, 72)]
+    reqs = []
+    for k in range(12):
+        si = k % 3
+        w, h = sizes[k % len(sizes)]
+        cam = configs.orbit_pose(rng)
+        reqs.append((si, cam, w, h, srv.submit(ids[si], cam, w, h)))
+    lat = []
+    for si, cam, w, h, rid in reqs:
+        img, ms = srv.wait(rid)
+        lat.append(ms)
+        plain = renderer.compile(scenes[si].to_ir(), w, h)
+        assert (img == plain.render_rgba(cam)).all()
+        plain.close()
+    st = srv.stats()
+    print(f"\n[server] {st} latency ms min={min(lat):.2f} max={max(lat):.2f}", flush=True)
+    assert st["completed"] == 12 and st["batches"] >= 1
+    # alone: one request at a time
+    si, cam, w, h, _ = reqs[0]
+    img, _ = srv.wait(srv.submit(ids[si], cam, w, h))
+    plain = renderer.compile(scenes[si].to_ir(), w, h)
+    assert (img == plain.render_rgba(cam)).all()
+    # a request larger than the whole HBM budget is refused, not hung
+    rid = srv.submit(ids[0], cam, 8000, 8000)
+    with pytest.raises(hip.Mm3eError):
+        srv.wait(rid)
+    srv.close()
