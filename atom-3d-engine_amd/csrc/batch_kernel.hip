@@ -46,22 +46,22 @@ __device__ __forceinline__ void batch_render_body(const BatchJob* __restrict__ j
       const Tables tb{reinterpret_cast<const DObj*>(b), reinterpret_cast<const DSeg*>(b + sc->st_seg),
                       reinterpret_cast<const DNode*>(b + sc->st_node),
                       reinterpret_cast<const uint32_t*>(b + sc->st_unb)};
-      render_tile<TableMap, MM3E_MAXB>(sc, tb, J.fp, J.hdr, J.tw, J.th, tx, ty, smem);
+      render_tile<TableMap, MM3E_MAXB, true>(sc, tb, J.fp, J.hdr, J.tw, J.th, tx, ty, smem);
     } else {
-      render_tile<TableMap, MM3E_MAXB>(sc, global_tables(sc), J.fp, J.hdr, J.tw, J.th, tx, ty, smem);
+      render_tile<TableMap, MM3E_MAXB, true>(sc, global_tables(sc), J.fp, J.hdr, J.tw, J.th, tx, ty, smem);
     }
     __syncthreads();  // smem is reused by the next tile
   }
 }
 
-extern "C" __global__ void __launch_bounds__(256)
+extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
     MM3E_CAT(mm3e_batch_render_lds_b, MM3E_MAXB)(const BatchJob* __restrict__ jobs, uint32_t n_jobs,
                                                  uint32_t total_tiles, uint32_t* __restrict__ counter) {
   extern __shared__ uint4 tables_lds[];
   batch_render_body<true>(jobs, n_jobs, total_tiles, counter, tables_lds);
 }
 
-extern "C" __global__ void __launch_bounds__(256)
+extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
     MM3E_CAT(mm3e_batch_render_table_b, MM3E_MAXB)(const BatchJob* __restrict__ jobs, uint32_t n_jobs,
                                                    uint32_t total_tiles, uint32_t* __restrict__ counter) {
   batch_render_body<false>(jobs, n_jobs, total_tiles, counter, nullptr);

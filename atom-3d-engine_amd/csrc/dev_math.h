@@ -230,12 +230,63 @@ MM3E_DEV V3 local_point(const DObj& o, V3 p) {
   return l;
 }
 
+// csg.rs:95-118 Expr::distance over the postfix program ops[0..n) at the object-local point l.
+// The operand stack lives in per-lane scratch memory (volatile keeps the compiler from
+// promoting it to registers, which would raise the register count of every kernel that can
+// reach this code): CSG objects are rare and the common path stays unaffected.
+MM3E_DEV float csg_distance(const DCsgOp* __restrict__ ops, uint32_t n, V3 l) {
+  volatile float st[kCsgStack];
+  uint32_t sp = 0;
+  V3 pt = l;
+  for (uint32_t i = 0; i < n; i++) {
+    const DCsgOp& o = ops[i];
+    switch (o.op) {
+      case CSG_LEAF:
+        st[sp++] = prim_distance(o.prim, o.p, pt);
+        break;
+      case CSG_ENTER: {  // vec.rs:216-221 Transform::to_local
+        V3 v = pt - ld3(o.pos);
+        pt = scale(V3{(o.c0[0] * v.x + o.c0[1] * v.y) + o.c0[2] * v.z, (o.c1[0] * v.x + o.c1[1] * v.y) + o.c1[2] * v.z,
+                      (o.c2[0] * v.x + o.c2[1] * v.y) + o.c2[2] * v.z},
+                   o.inv);
+        break;
+      }
+      case CSG_LEAVE:
+        st[sp - 1] = st[sp - 1] * o.scale;
+        pt = l;
+        break;
+      case CSG_OFFSET:
+        st[sp - 1] = st[sp - 1] - o.p[0];
+        break;
+      case CSG_SHELL:
+        st[sp - 1] = rabs(st[sp - 1]) - o.p[0];
+        break;
+      default: {
+        const float x = st[--sp], y = st[sp - 1];
+        const float a = o.swap ? x : y, b = o.swap ? y : x;
+        float r;
+        if (o.op == CSG_UNION) r = rmin(a, b);
+        else if (o.op == CSG_INTERSECT) r = rmax(a, b);
+        else if (o.op == CSG_SUBTRACT) r = rmax(a, -b);
+        else r = f_smooth_union(Field{a, 0u}, Field{b, 0u}, o.p[0]).dist;
+        st[sp - 1] = r;
+      }
+    }
+  }
+  return st[0];
+}
+
 // lib.rs:380-397 Object::field.
+// CSG: whether Prim::Csg objects can occur (kernels for scenes without them leave the
+// evaluator out).
+template <bool CSG>
 MM3E_DEV float object_distance(const DevScene& sc, const DObj& o, V3 p) {
   V3 l = local_point(o, p);
   const uint32_t kind = o.flags & OBJ_KIND_MASK;
   float d;
-  if (kind == 12) {
+  if (CSG && kind == 11) {
+    d = csg_distance(sc.csg + __builtin_bit_cast(uint32_t, o.p[0]), __builtin_bit_cast(uint32_t, o.p[1]), l);
+  } else if (kind == 12) {
     uint32_t vid = __builtin_bit_cast(uint32_t, o.p[0]);
     d = sc.n_vol == 0 ? __builtin_inff() : vol_sample(sc.vol[vid % sc.n_vol], sc.vol_data, l);
   } else {
@@ -263,12 +314,13 @@ MM3E_DEV bool far_noop(const DObj& o, V3 p, float thr) {
 }
 
 // accel.rs:187-207 WorldPlan::contrib.
+template <bool CSG>
 MM3E_DEV Field contrib(const DevScene& sc, const DObj& o, V3 p) {
   if (o.flags & OBJ_BOUNDED) {
     float lower = length(p - ld3(o.bc)) - o.bc_r;
     if (lower > sc.slack) return Field{lower, o.mat};
   }
-  return Field{object_distance(sc, o, p), o.mat};
+  return Field{object_distance<CSG>(sc, o, p), o.mat};
 }
 
 #ifdef MM3E_MAP_NOINLINE
@@ -277,33 +329,31 @@ MM3E_DEV Field contrib(const DevScene& sc, const DObj& o, V3 p) {
 #define MM3E_MAP_ATTR MM3E_DEV
 #endif
 
-// accel.rs:209-252 WorldPlan::eval — the reference `world()` field.  Runs of plain unions walk
-// the same bounding-sphere tree in the same near-first order with the same `lower > best`
+// One segment of WorldPlan::eval (accel.rs:209-252) folded into `acc`.  Runs of plain unions
+// walk the same bounding-sphere tree in the same near-first order with the same `lower > best`
 // cut and earliest-index tie break as Bvh::query (accel.rs:79-119); instead of a 64-entry
 // stack each lane keeps one bit per tree level saying which child it visited first, and
 // climbs back through parent links, so traversal state is four registers.
-MM3E_MAP_ATTR Field world_eval(const DevScene& sc, const Tables& tb, V3 p) {
-  Field acc{__builtin_inff(), 0u};
-  bool seeded = false;
-  const uint32_t nseg = sc.n_seg;
-  for (uint32_t si = 0; si < nseg; si++) {
+template <bool CSG>
+MM3E_DEV void seg_step(const DevScene& sc, const Tables& tb, V3 p, uint32_t si, Field& acc, bool& seeded) {
+  {
     const DSeg& seg = tb.seg[si];
     if (seg.kind == 0) {
       const DObj& o = tb.obj[seg.index];
       if (seeded && (o.flags & OBJ_BOUNDED)) {
         float thr = seg.combine == 2 ? -acc.dist : (seg.combine == 1 && seg.k > 0.0f ? acc.dist + seg.k : acc.dist);
-        if (far_noop(o, p, rmax(thr, sc.slack))) continue;
+        if (far_noop(o, p, rmax(thr, sc.slack))) return;
       }
-      Field f = contrib(sc, o, p);
+      Field f = contrib<CSG>(sc, o, p);
       if (!seeded) {
         acc = f;
         seeded = true;
-        continue;
+        return;
       }
       if (seg.combine == 0) acc = f_union(acc, f);
       else if (seg.combine == 1) acc = f_smooth_union(acc, f, seg.k);
       else acc = f_subtract(acc, f);
-      continue;
+      return;
     }
     if (seg.kind == 2) {
       // Every member has an exact SDF inside a true bounding sphere (SDF >= |p - c| - r), so
@@ -316,12 +366,12 @@ MM3E_MAP_ATTR Field world_eval(const DevScene& sc, const Tables& tb, V3 p) {
       for (uint32_t u = 0; u < seg.unb_count; u++) {
         const DObj& o = tb.obj[tb.unb[seg.unb_base + u]];
         if (any && (o.flags & OBJ_BOUNDED) && far_noop(o, p, rmax(bd, sc.slack))) continue;
-        Field f = contrib(sc, o, p);
+        Field f = contrib<CSG>(sc, o, p);
         if (!any || f.dist < bd) { bd = f.dist; bm = f.mat; any = true; }
       }
       acc = Field{bd, bm};
       seeded = true;
-      continue;
+      return;
     }
     float bd;
     int64_t bi;
@@ -334,7 +384,7 @@ MM3E_MAP_ATTR Field world_eval(const DevScene& sc, const Tables& tb, V3 p) {
     for (uint32_t u = 0; u < seg.unb_count; u++) {
       uint32_t i = tb.unb[seg.unb_base + u];
       const DObj& o = tb.obj[i];
-      float d = object_distance(sc, o, p);
+      float d = object_distance<CSG>(sc, o, p);
       if (d < bd || (d == bd && (int64_t)i < bi)) { bd = d; bi = i; bm = o.mat; }
     }
     if (seg.has_tree) {
@@ -358,7 +408,7 @@ MM3E_MAP_ATTR Field world_eval(const DevScene& sc, const Tables& tb, V3 p) {
             const DObj& o = tb.obj[i];
             float d;
             if (lower > sc.slack) d = lower;
-            else d = object_distance(sc, o, p);
+            else d = object_distance<CSG>(sc, o, p);
             if (d < bd || (d == bd && (int64_t)i < bi)) { bd = d; bi = i; bm = o.mat; }
           } else {
             const DNode& L = nodes[n.left];
@@ -395,7 +445,96 @@ MM3E_MAP_ATTR Field world_eval(const DevScene& sc, const Tables& tb, V3 p) {
     acc = Field{bd, bm};
     seeded = true;
   }
-  return acc;
+}
+
+// Segment-grid cell of p (index nx*ny*nz for points outside the grid, NaN included).
+MM3E_DEV uint32_t grid_cell(const DevScene& sc, V3 p) {
+  const float fx = (p.x - sc.grid_lo[0]) * sc.grid_inv;
+  const float fy = (p.y - sc.grid_lo[1]) * sc.grid_inv;
+  const float fz = (p.z - sc.grid_lo[2]) * sc.grid_inv;
+  const uint32_t nx = sc.grid_dims[0], ny = sc.grid_dims[1], nz = sc.grid_dims[2];
+  const bool in = fx >= 0.0f && fy >= 0.0f && fz >= 0.0f && fx < (float)nx && fy < (float)ny && fz < (float)nz;
+  if (!in) return nx * ny * nz;
+  // fx < (float)nx with fx >= 0 truncates to at most nx - 1
+  const uint32_t ix = (uint32_t)fx, iy = (uint32_t)fy, iz = (uint32_t)fz;
+  return (iz * ny + iy) * nx + ix;
+}
+
+// accel.rs:209-252 WorldPlan::eval — the reference `world()` field: the ordered fold of
+// seg_step over every segment.  With a segment grid, after the first segment seeds the
+// accumulator each lane checks that max(acc + kmax, slack) (plus rounding margins) is below a
+// grid level's reach; if every active lane passes, the wave gathers the distinct cells of its
+// lanes (at most four; scalar readlane/ballot), ORs their masks with scalar loads and folds only
+// the segments whose bits are set, in index order.  A clear bit is a segment whose every member
+// is a provable no-op for every point of that cell at that threshold, so the fold is the same.
+// GRID: whether the scene may carry a segment grid (kernels for scenes without one keep the
+// plain fold).
+template <bool GRID, bool CSG>
+MM3E_MAP_ATTR Field world_eval(const DevScene& sc, const Tables& tb, V3 p) {
+  Field acc{__builtin_inff(), 0u};
+  bool seeded = false;
+  const uint32_t nseg = sc.n_seg;
+  if constexpr (!GRID) {
+    for (uint32_t si = 0; si < nseg; si++) seg_step<CSG>(sc, tb, p, si, acc, seeded);
+    return acc;
+  }
+  if (nseg == 0) return acc;
+  const uint32_t* __restrict__ grid = sc.grid;
+  // One call site for seg_step (it carries the tree walk and is inlined): the loop either
+  // steps si through every segment or, once the wave switches to the grid, through the set
+  // bits of the OR of its cells' masks (word w, remaining bits `bits`).
+  bool masked = false;
+  uint32_t W = 0, w = 0, bits = 0;
+  const uint32_t *m0 = nullptr, *m1 = nullptr, *m2 = nullptr, *m3 = nullptr;
+  uint32_t si = 0;
+  for (;;) {
+    seg_step<CSG>(sc, tb, p, si, acc, seeded);
+    if (si == 0 && grid != nullptr) {
+      const float thr = rmax(acc.dist + sc.grid_kmax, sc.slack) * 1.0001f + sc.grid_eps;
+      const bool near0 = thr < sc.grid_m[0];
+      const bool near1 = thr < sc.grid_m[1];
+      if (__builtin_amdgcn_ballot_w64(!near1) == 0) {
+        const uint32_t level = __builtin_amdgcn_ballot_w64(!near0) != 0 ? 1u : 0u;
+        const uint32_t cell = grid_cell(sc, p);
+        uint64_t rem = __builtin_amdgcn_ballot_w64(true);
+        const uint32_t c0 = __builtin_amdgcn_readlane(cell, (uint32_t)__builtin_ctzll(rem));
+        rem &= ~__builtin_amdgcn_ballot_w64(cell == c0);
+        uint32_t c1 = c0, c2 = c0, c3 = c0;
+        if (rem) {
+          c1 = __builtin_amdgcn_readlane(cell, (uint32_t)__builtin_ctzll(rem));
+          rem &= ~__builtin_amdgcn_ballot_w64(cell == c1);
+        }
+        if (rem) {
+          c2 = __builtin_amdgcn_readlane(cell, (uint32_t)__builtin_ctzll(rem));
+          rem &= ~__builtin_amdgcn_ballot_w64(cell == c2);
+        }
+        if (rem) {
+          c3 = __builtin_amdgcn_readlane(cell, (uint32_t)__builtin_ctzll(rem));
+          rem &= ~__builtin_amdgcn_ballot_w64(cell == c3);
+        }
+        if (rem == 0) {
+          masked = true;
+          W = sc.grid_words;
+          m0 = grid + ((uint64_t)c0 * 2 + level) * W;
+          m1 = grid + ((uint64_t)c1 * 2 + level) * W;
+          m2 = grid + ((uint64_t)c2 * 2 + level) * W;
+          m3 = grid + ((uint64_t)c3 * 2 + level) * W;
+          w = 0;
+          bits = (m0[0] | m1[0] | m2[0] | m3[0]) & ~1u;
+        }
+      }
+    }
+    if (masked) {
+      while (bits == 0) {
+        if (++w >= W) return acc;
+        bits = m0[w] | m1[w] | m2[w] | m3[w];
+      }
+      si = w * 32u + (uint32_t)__builtin_ctz(bits);
+      bits &= bits - 1u;
+    } else if (++si >= nseg) {
+      return acc;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -566,86 +705,167 @@ MM3E_DEV V3 normal_dual(const DevScene& sc, const Tables& tb, V3 p) {
 struct Hit {
   bool hit;
   float t;
-  V3 pos, normal;
+  V3 pos;
   uint32_t mat, steps;
 };
 
-template <class Map> MM3E_DEV V3 tetra_normal(const Map& field, float h, V3 p) {  // march.rs:212-223
-  const V3 k0{1.0f, -1.0f, -1.0f}, k1{-1.0f, -1.0f, 1.0f}, k2{-1.0f, 1.0f, -1.0f}, k3{1.0f, 1.0f, 1.0f};
-  V3 g = scale(k0, field(p + scale(k0, h)).dist) + scale(k1, field(p + scale(k1, h)).dist) +
-         scale(k2, field(p + scale(k2, h)).dist) + scale(k3, field(p + scale(k3, h)).dist);
-  return normalize(g);
+// The searches below (march, tetrahedron normal, soft shadow, AO) are written as state
+// records with init/step functions that take one field value per step, so a kernel can run
+// all of them from a single field-evaluation site (one inlined copy of the world fold) and
+// the lanes of a wave share every evaluation whatever search each is in.  The arithmetic of
+// each step is exactly the reference loop body it restates.
+
+// march.rs:212-223 tetrahedron gradient: corner k of (+--, --+, -+-, +++) at p + k*h.
+MM3E_DEV V3 tetra_corner(uint32_t k) {
+  return k == 0 ? V3{1.0f, -1.0f, -1.0f} : k == 1 ? V3{-1.0f, -1.0f, 1.0f} : k == 2 ? V3{-1.0f, 1.0f, -1.0f}
+                                                                                    : V3{1.0f, 1.0f, 1.0f};
+}
+struct TetraState {
+  V3 p, g;
+  float h;
+  uint32_t k;
+};
+MM3E_DEV V3 tetra_init(TetraState& s, V3 p, float h) {
+  s.p = p;
+  s.h = h;
+  s.k = 0;
+  s.g = V3{0.0f, 0.0f, 0.0f};
+  return p + scale(tetra_corner(0), h);
+}
+// Folds corner k's value in left-to-right order; true when the gradient is complete.
+MM3E_DEV bool tetra_step(TetraState& s, float d, V3& q) {
+  const V3 term = scale(tetra_corner(s.k), d);
+  s.g = s.k == 0 ? term : s.g + term;
+  if (++s.k == 4) return true;
+  q = s.p + scale(tetra_corner(s.k), s.h);
+  return false;
 }
 
 // march.rs:120-209 — enhanced sphere tracing with the unconditional overlap guard.
-template <class Map> MM3E_DEV Hit march(const DevScene& sc, const Map& field, V3 ro, V3 rd) {
-  const bool ss1 = (sc.flags & SC_SS1) != 0;
-  float omega = ss1 ? 1.4f : sc.step_scale;
-  float subitize = ss1 ? sc.subitize : 0.0f;
-  bool secant_on = ((sc.flags & SC_SECANT) != 0) && ss1;
-  int boost = 0;  // 0 omega, 1 leap, 2 secant
-  float t = 0.0f, prev_radius = 0.0f, step_len = 0.0f, t_prev = 0.0f, d_prev = __builtin_inff();
-  const uint32_t max_steps = sc.max_steps;
-  Hit h;
+enum : uint32_t { MARCH_EVAL = 0, MARCH_EVAL_MID = 1, MARCH_DONE = 2 };
+struct MarchState {
+  V3 ro, rd;
+  float t, prev_radius, step_len, t_prev, d_prev, omega, subitize;
+  uint32_t i, boost;  // boost: 0 omega, 1 leap, 2 secant
+  bool secant_on;
+  // the main evaluation while the guard's midpoint probe is in flight
+  Field fm;
+  float radius_m, eps_m, gap_lo, gap_hi;
+};
+MM3E_DEV uint32_t march_miss(const DevScene& sc, const MarchState& m, Hit& h) {
   h.hit = false;
-  h.normal = V3{0.0f, 0.0f, 0.0f};
-  uint32_t i = 0;
-  for (; i < max_steps; i++) {
-    V3 p = ro + scale(rd, t);
-    Field f = field(p);
-    float radius = rabs(f.dist);
-    float eps = sc.eps * (1.0f + t * 0.5f) + sc.lod * t;
-    if ((step_len > 0.0f && f.dist < 0.0f) || step_len > radius + prev_radius) {
-      float gap_lo = t_prev + prev_radius;
-      float gap_hi = t - radius;
-      float mid = 0.5f * (gap_lo + gap_hi);
-      bool covered = false;
-      if (f.dist > 0.0f) covered = field(ro + scale(rd, mid)).dist >= 0.5f * (gap_hi - gap_lo);
-      if (!covered) {
-        t = gap_lo;
-        if (boost == 0) omega = rmin(omega, 1.0f);
-        else if (boost == 1) subitize = 0.0f;
-        else secant_on = false;
-        boost = 0;
-        prev_radius = 0.0f;
-        step_len = 0.0f;
-        d_prev = __builtin_inff();
-        continue;
-      }
-    }
-    if (f.dist < eps) {
-      h.hit = true;
-      h.t = t;
-      h.pos = p;
-      h.mat = f.mat;
-      h.steps = i;
-      return h;
-    }
-    step_len = f.dist * omega;
-    boost = 0;
-    if (subitize > 0.0f && f.dist > eps * 6.0f) {
-      step_len = step_len * (1.0f + subitize);
-      boost = 1;
-    }
-    if (secant_on && f.dist < 0.08f && __builtin_isfinite(d_prev)) {
-      float dt = t - t_prev;
-      float dd = f.dist - d_prev;
-      if (dt > 1e-6f && dd < -1e-6f) {
-        step_len = rclamp(-f.dist * dt / dd, f.dist, f.dist * 4.0f);
-        boost = 2;
-      }
-    }
-    prev_radius = radius;
-    d_prev = f.dist;
-    t_prev = t;
-    t = t + step_len;
-    if (t > sc.max_dist) break;
-  }
-  h.t = t;
-  h.pos = ro + scale(rd, t);
+  h.t = m.t;
+  h.pos = m.ro + scale(m.rd, m.t);
   h.mat = 0;
-  h.steps = max_steps;
+  h.steps = sc.max_steps;
+  return MARCH_DONE;
+}
+MM3E_DEV uint32_t march_next(const DevScene& sc, MarchState& m, Hit& h, V3& q) {  // loop increment + test
+  if (++m.i >= sc.max_steps) return march_miss(sc, m, h);
+  q = m.ro + scale(m.rd, m.t);
+  return MARCH_EVAL;
+}
+MM3E_DEV uint32_t march_init(const DevScene& sc, MarchState& m, V3 ro, V3 rd, Hit& h, V3& q) {
+  const bool ss1 = (sc.flags & SC_SS1) != 0;
+  m.ro = ro;
+  m.rd = rd;
+  m.omega = ss1 ? 1.4f : sc.step_scale;
+  m.subitize = ss1 ? sc.subitize : 0.0f;
+  m.secant_on = ((sc.flags & SC_SECANT) != 0) && ss1;
+  m.boost = 0;
+  m.t = 0.0f;
+  m.prev_radius = 0.0f;
+  m.step_len = 0.0f;
+  m.t_prev = 0.0f;
+  m.d_prev = __builtin_inff();
+  m.i = 0;
+  if (sc.max_steps == 0) return march_miss(sc, m, h);
+  q = m.ro + scale(m.rd, m.t);
+  return MARCH_EVAL;
+}
+MM3E_DEV uint32_t march_retreat(const DevScene& sc, MarchState& m, Hit& h, V3& q) {
+  m.t = m.gap_lo;
+  if (m.boost == 0) m.omega = rmin(m.omega, 1.0f);
+  else if (m.boost == 1) m.subitize = 0.0f;
+  else m.secant_on = false;
+  m.boost = 0;
+  m.prev_radius = 0.0f;
+  m.step_len = 0.0f;
+  m.d_prev = __builtin_inff();
+  return march_next(sc, m, h, q);
+}
+MM3E_DEV uint32_t march_finish(const DevScene& sc, MarchState& m, Field f, float radius, float eps, Hit& h, V3& q) {
+  if (f.dist < eps) {
+    h.hit = true;
+    h.t = m.t;
+    h.pos = m.ro + scale(m.rd, m.t);
+    h.mat = f.mat;
+    h.steps = m.i;
+    return MARCH_DONE;
+  }
+  float step_len = f.dist * m.omega;
+  m.boost = 0;
+  if (m.subitize > 0.0f && f.dist > eps * 6.0f) {
+    step_len = step_len * (1.0f + m.subitize);
+    m.boost = 1;
+  }
+  if (m.secant_on && f.dist < 0.08f && __builtin_isfinite(m.d_prev)) {
+    float dt = m.t - m.t_prev;
+    float dd = f.dist - m.d_prev;
+    if (dt > 1e-6f && dd < -1e-6f) {
+      step_len = rclamp(-f.dist * dt / dd, f.dist, f.dist * 4.0f);
+      m.boost = 2;
+    }
+  }
+  m.step_len = step_len;
+  m.prev_radius = radius;
+  m.d_prev = f.dist;
+  m.t_prev = m.t;
+  m.t = m.t + step_len;
+  if (m.t > sc.max_dist) return march_miss(sc, m, h);
+  return march_next(sc, m, h, q);
+}
+// One step given the field at the point march_init / march_step last asked for (`st` is the
+// status that request came with).  Returns the next status; q is the next point to evaluate.
+MM3E_DEV uint32_t march_step(const DevScene& sc, MarchState& m, uint32_t st, Field f, Hit& h, V3& q) {
+  if (st == MARCH_EVAL_MID) {
+    const bool covered = f.dist >= 0.5f * (m.gap_hi - m.gap_lo);
+    if (!covered) return march_retreat(sc, m, h, q);
+    return march_finish(sc, m, m.fm, m.radius_m, m.eps_m, h, q);
+  }
+  const float radius = rabs(f.dist);
+  const float eps = sc.eps * (1.0f + m.t * 0.5f) + sc.lod * m.t;
+  if ((m.step_len > 0.0f && f.dist < 0.0f) || m.step_len > radius + m.prev_radius) {
+    m.gap_lo = m.t_prev + m.prev_radius;
+    m.gap_hi = m.t - radius;
+    if (f.dist > 0.0f) {
+      const float mid = 0.5f * (m.gap_lo + m.gap_hi);
+      m.fm = f;
+      m.radius_m = radius;
+      m.eps_m = eps;
+      q = m.ro + scale(m.rd, mid);
+      return MARCH_EVAL_MID;
+    }
+    return march_retreat(sc, m, h, q);
+  }
+  return march_finish(sc, m, f, radius, eps, h, q);
+}
+
+template <class Map> MM3E_DEV Hit march(const DevScene& sc, const Map& field, V3 ro, V3 rd) {
+  MarchState m;
+  Hit h;
+  V3 q;
+  uint32_t st = march_init(sc, m, ro, rd, h, q);
+  while (st != MARCH_DONE) st = march_step(sc, m, st, field(q), h, q);
   return h;
+}
+
+template <class Map> MM3E_DEV V3 tetra_normal(const Map& field, float h, V3 p) {  // march.rs:212-223
+  TetraState s;
+  V3 q = tetra_init(s, p, h);
+  while (!tetra_step(s, field(q).dist, q)) {
+  }
+  return normalize(s.g);
 }
 
 // Hit normal: dual-number gradient on dual-safe scenes, tetrahedron otherwise (lib.rs:1273-1277).
@@ -654,38 +874,100 @@ template <class Map> MM3E_DEV V3 hit_normal(const DevScene& sc, const Map& field
   return tetra_normal(field, sc.normal_h, p);
 }
 
-// march.rs:227-259
-template <class Map> MM3E_DEV float soft_shadow(const DevScene& sc, const Map& field, V3 o, V3 d, float max_t, float k) {
-  float res = 1.0f;
+// march.rs:227-259 soft shadow.
+struct ShadowState {
+  V3 o, d;
+  float res, t, max_t, k;
+  uint32_t s;
+};
+// false: finished without evaluating (`result` set); true: evaluate q.
+MM3E_DEV bool shadow_init(const DevScene& sc, ShadowState& s, V3 o, V3 d, float max_t, float k, V3& q, float& result) {
+  s.o = o;
+  s.d = d;
+  s.max_t = max_t;
+  s.k = k;
+  s.res = 1.0f;
   float hb = o.x * 127.1f + o.y * 311.7f + o.z * 74.7f;
   float sv = sinf(hb) * 43758.547f;
   float jitter = rabs(sv - __builtin_truncf(sv));  // f32::fract
-  float t = 0.02f + jitter * 0.16f;
-  const uint32_t n = sc.shadow_steps;
-  for (uint32_t s = 0; s < n; s++) {
-    float h = field(o + scale(d, t)).dist;
-    if (h < 0.0008f) return 0.0f;
-    res = rmin(res, k * h / t);
-    t = t + rclamp(h, 0.06f, 0.5f + 0.7f * t);
-    if (t > max_t) break;
+  s.t = 0.02f + jitter * 0.16f;
+  s.s = 0;
+  if (sc.shadow_steps == 0) {
+    result = rclamp(s.res, 0.0f, 1.0f);
+    return false;
   }
-  return rclamp(res, 0.0f, 1.0f);
+  q = s.o + scale(s.d, s.t);
+  return true;
+}
+// true: finished (`result` set); false: evaluate q next.
+MM3E_DEV bool shadow_step(const DevScene& sc, ShadowState& s, float h, V3& q, float& result) {
+  if (h < 0.0008f) {
+    result = 0.0f;
+    return true;
+  }
+  s.res = rmin(s.res, s.k * h / s.t);
+  s.t = s.t + rclamp(h, 0.06f, 0.5f + 0.7f * s.t);
+  if (s.t > s.max_t || ++s.s >= sc.shadow_steps) {
+    result = rclamp(s.res, 0.0f, 1.0f);
+    return true;
+  }
+  q = s.o + scale(s.d, s.t);
+  return false;
+}
+template <class Map> MM3E_DEV float soft_shadow(const DevScene& sc, const Map& field, V3 o, V3 d, float max_t, float k) {
+  ShadowState s;
+  V3 q;
+  float r;
+  if (!shadow_init(sc, s, o, d, max_t, k, q, r)) return r;
+  while (!shadow_step(sc, s, field(q).dist, q, r)) {
+  }
+  return r;
 }
 
-// march.rs:264-280
-template <class Map> MM3E_DEV float ambient_occlusion(const DevScene& sc, const Map& field, V3 p, V3 n) {
+// march.rs:264-280 ambient occlusion.
+struct AoState {
+  V3 p, n;
+  float occ, sca;
+  uint32_t i;
+};
+MM3E_DEV float ao_radius(const DevScene& sc, uint32_t i) {
   const uint32_t ns = sc.ao_samples;
-  if (ns == 0) return 1.0f;
-  float base = rmax(sc.normal_h, 0.01f);
-  float span = (float)((ns > 2u ? ns : 2u) - 1u);
-  float occ = 0.0f, sca = 1.0f;
-  for (uint32_t i = 0; i < ns; i++) {
-    float hr = base + 0.12f * (float)i / span;
-    float d = field(p + scale(n, hr)).dist;
-    occ = occ + (hr - d) * sca;
-    sca = sca * 0.92f;
+  const float base = rmax(sc.normal_h, 0.01f);
+  const float span = (float)((ns > 2u ? ns : 2u) - 1u);
+  return base + 0.12f * (float)i / span;
+}
+MM3E_DEV bool ao_init(const DevScene& sc, AoState& a, V3 p, V3 n, V3& q, float& result) {
+  a.p = p;
+  a.n = n;
+  a.occ = 0.0f;
+  a.sca = 1.0f;
+  a.i = 0;
+  if (sc.ao_samples == 0) {
+    result = 1.0f;
+    return false;
   }
-  return rclamp(1.0f - 2.6f * occ, 0.0f, 1.0f);
+  q = p + scale(n, ao_radius(sc, 0));
+  return true;
+}
+MM3E_DEV bool ao_step(const DevScene& sc, AoState& a, float d, V3& q, float& result) {
+  const float hr = ao_radius(sc, a.i);
+  a.occ = a.occ + (hr - d) * a.sca;
+  a.sca = a.sca * 0.92f;
+  if (++a.i >= sc.ao_samples) {
+    result = rclamp(1.0f - 2.6f * a.occ, 0.0f, 1.0f);
+    return true;
+  }
+  q = a.p + scale(a.n, ao_radius(sc, a.i));
+  return false;
+}
+template <class Map> MM3E_DEV float ambient_occlusion(const DevScene& sc, const Map& field, V3 p, V3 n) {
+  AoState a;
+  V3 q;
+  float r;
+  if (!ao_init(sc, a, p, n, q, r)) return r;
+  while (!ao_step(sc, a, field(q).dist, q, r)) {
+  }
+  return r;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -857,11 +1139,16 @@ MM3E_DEV V3 ibl_irradiance(const DevScene& sc, V3 n) {
 }
 
 // Table-driven world field over the compiled scene (tables in HBM or staged in LDS).
-struct TableMap {
+template <bool GRID, bool CSG>
+struct TableMapT {
   const DevScene* sc;
   Tables tb;
-  MM3E_DEV Field operator()(V3 p) const { return world_eval(*sc, tb, p); }
+  MM3E_DEV Field operator()(V3 p) const { return world_eval<GRID, CSG>(*sc, tb, p); }
 };
+// Every feature (segment grid, CSG objects): the general map.
+using TableMap = TableMapT<true, true>;
+// Scenes with neither: the fold without the grid gather and without the CSG evaluator.
+using SimpleMap = TableMapT<false, false>;
 
 MM3E_DEV Tables global_tables(const DevScene* sc) { return Tables{sc->obj, sc->seg, sc->nodes, sc->unb}; }
 

@@ -2,8 +2,9 @@
 // implementation (`TableMap` walks the compiled scene tables; generated maps specialise it).
 //
 //   render_tile      one lane per (pixel, AA sample): march + shade + reflection chain
-//                    (lib.rs:1223-1360); the block's samples meet in LDS and the first lane of
-//                    each pixel sums them in the reference's sy-major order.
+//                    (lib.rs:1223-1360), nested loops or a per-lane state machine; the block's
+//                    samples meet in LDS and the first lane of each pixel sums them in the
+//                    reference's sy-major order.
 //   gather_rays      GI bake rays (lib.rs:686-711): one lane per (probe, face, ray).
 //   gi_reduce        weighted mean per cube face (gi.rs:327-341).
 //   bloom_h / bloom_v_resolve / resolve_only   post.rs:36-115 + framebuffer.rs:102-113.
@@ -11,6 +12,14 @@
 //   field_probe / march_probe   map / raw-march parity probes (wgsl.rs:641-646 twin).
 #pragma once
 #include "dev_math.h"
+
+// Waves per SIMD the render kernels are register-allocated for: 4 keeps them at <= 128 VGPRs
+// (16 waves per CU), trading a few SGPR spills into VGPR lanes for a quarter more latency
+// hiding than the 3 waves the unconstrained allocation reaches.
+#ifndef MM3E_RENDER_WAVES
+#define MM3E_RENDER_WAVES 4
+#endif
+#define MM3E_RENDER_OCCUPANCY __attribute__((amdgpu_waves_per_eu(MM3E_RENDER_WAVES)))
 
 namespace mm3e {
 
@@ -54,9 +63,10 @@ MM3E_DEV V3 shade_surface(const DevScene& sc, const Map& field, const Hit& h, V3
   return radiance;
 }
 
-// trace (lib.rs:1265-1282).  The reflection recursion is bounded at compile time by MAXB
-// (the scene's `bounces`), so each level is a separate inlined copy and no per-level state
-// goes to scratch: out = fog(base*(1-fr) + trace(reflected)*fr + emissive).
+// trace (lib.rs:1265-1282) as nested loops: the kernel for scenes whose world fold is short.
+// The reflection recursion is bounded at compile time by MAXB (the scene's `bounces`), so each
+// level is a separate inlined copy and no per-level state goes to scratch:
+// out = fog(base*(1-fr) + trace(reflected)*fr + emissive).
 template <class Map, int D, int MAXB>
 MM3E_DEV V3 trace_level(const DevScene& sc, const Map& field, V3 ro, V3 rd) {
   Hit h = march(sc, field, ro, rd);
@@ -79,13 +89,256 @@ MM3E_DEV V3 trace_level(const DevScene& sc, const Map& field, V3 ro, V3 rd) {
   return apply_fog(radiance, ld3(sc.fog), h.t, sc.fog_density);
 }
 
+// ---------------------------------------------------------------------------------------
+// trace / shade_hit (lib.rs:1247-1360) as one per-lane state machine: the kernel for scenes
+// whose world fold is long (segment grid, CSG objects).
+//
+// Every field evaluation of a sample's path — march steps and the overlap-guard probe, the
+// tetrahedron normal, AO taps, soft-shadow steps, the reflected rays' marches — goes through
+// the single `field(ps.q)` site of trace_path, so the kernel holds one inlined copy of the world
+// fold, and the lanes of a wave share each evaluation whichever search they are in.  Per lane
+// the sequence of operations is exactly the recursive reference:
+//   trace(ray, d): miss -> sky(dir); hit -> radiance = shade (AO, GI, IBL, lights with soft
+//   shadows), then if d < bounces and reflectivity > 0: radiance = radiance*(1-fr) +
+//   trace(reflected, d+1)*fr; then + emissive; then fog(t).
+// The pending levels of that recursion (radiance, fr, t, material) wait in a per-lane stack.
+// ---------------------------------------------------------------------------------------
+enum : uint32_t { PH_MARCH = 0, PH_NORMAL = 1, PH_AO = 2, PH_SHADOW = 3, PH_DONE = 4 };
+
+// The searches never overlap in time, so they share storage.
+union SearchState {
+  MarchState m;
+  TetraState tn;
+  AoState ao;
+  ShadowState sh;
+};
+
+struct PathState {
+  uint32_t phase, mst, level, li;
+  V3 q;         // next point to evaluate
+  V3 rd;        // current ray direction
+  Hit h;        // current level's hit
+  V3 n;         // its normal
+  V3 radiance;  // current level's radiance so far; the sample's radiance once PH_DONE
+  float value;  // finished AO / shadow search result
+  SearchState x;
+};
+
+template <int MAXB>
+struct LevelStack {
+  // base radiance xyz, fr, hit t and the material index per pending level; volatile keeps it
+  // in per-lane scratch memory rather than in registers that every iteration would carry
+  volatile float v[MAXB > 0 ? MAXB : 1][5];
+  volatile uint32_t mat[MAXB > 0 ? MAXB : 1];
+};
+
+// Fold finished levels back up the recursion: out = fog(base*(1-fr) + inner*fr + emissive).
+template <int MAXB>
+MM3E_DEV void path_unwind(const DevScene& sc, PathState& ps, LevelStack<MAXB>& stk, V3 r) {
+  while (ps.level > 0) {
+    ps.level--;
+    const volatile float* e = stk.v[ps.level];
+    const V3 base = V3{e[0], e[1], e[2]};
+    const float fr = e[3], t = e[4];
+    const uint32_t mat = stk.mat[ps.level];
+    r = scale(base, 1.0f - fr) + scale(r, fr);
+    r = r + ld3(sc.mat[mat].emissive);
+    r = apply_fog(r, ld3(sc.fog), t, sc.fog_density);
+  }
+  ps.radiance = r;
+  ps.phase = PH_DONE;
+}
+
+template <int MAXB>
+MM3E_DEV void path_march(const DevScene& sc, PathState& ps, V3 ro, V3 rd) {
+  ps.rd = rd;
+  ps.phase = PH_MARCH;
+  ps.mst = march_init(sc, ps.x.m, ro, rd, ps.h, ps.q);
+}
+
+// Lights from ps.li on (lib.rs:1333-1352); returns true when a shadow search needs evaluating.
+template <int MAXB>
+MM3E_DEV bool path_lights(const DevScene& sc, PathState& ps, float shadow_in, bool resume) {
+  const DMat& m = sc.mat[ps.h.mat % sc.n_mat];
+  const V3 albedo = surface_albedo(m, ps.h.pos);
+  const V3 view = scale(ps.rd, -1.0f);
+  for (;;) {
+    if (ps.li >= sc.n_light) return false;
+    const DLight& l = sc.light[ps.li];
+    V3 ld;
+    float dist;
+    light_toward(l, ps.h.pos, ld, dist);
+    float shadow = 1.0f;
+    if (resume) {
+      shadow = shadow_in;
+      resume = false;
+    } else {
+      if (lambert(ps.n, ld) <= 0.0f) {
+        ps.li++;
+        continue;
+      }
+      if (sc.flags & SC_SHADOWS) {
+        const float max_t = rmin(dist, sc.max_dist);
+        const float lift = rmax(sc.normal_h * 2.0f, 0.01f);
+        if (shadow_init(sc, ps.x.sh, ps.h.pos + scale(ps.n, lift), ld, max_t, light_shadow_k(l, dist), ps.q, shadow)) {
+          ps.phase = PH_SHADOW;
+          return true;
+        }
+      }
+    }
+    if (shadow > 0.0f) {
+      const V3 surf = brdf(ps.n, ld, view, albedo, m.metallic, m.roughness, m.specular);
+      ps.radiance = ps.radiance + scale(cmul(surf, ld3(l.color)), shadow * light_atten(l, dist));
+    }
+    ps.li++;
+  }
+}
+
+// After the lights: reflect (push this level) or finish the level and unwind.
+template <int MAXB>
+MM3E_DEV void path_level_end(const DevScene& sc, PathState& ps, LevelStack<MAXB>& stk) {
+  const uint32_t mi = ps.h.mat % sc.n_mat;
+  const DMat& m = sc.mat[mi];
+  if (MAXB > 0 && ps.level < (uint32_t)MAXB && ps.level < sc.bounces && m.reflectivity > 0.0f) {
+    const V3 view = scale(ps.rd, -1.0f);
+    const float c = rmax(dot(ps.n, view), 0.0f);
+    const float fr = fresnel_schlick(c, m.reflectivity);
+    const V3 rdir = normalize(reflect(ps.rd, ps.n));
+    const V3 rorg = ps.h.pos + scale(ps.n, rmax(sc.normal_h * 2.0f, 0.02f));
+    volatile float* e = stk.v[ps.level];
+    e[0] = ps.radiance.x;
+    e[1] = ps.radiance.y;
+    e[2] = ps.radiance.z;
+    e[3] = fr;
+    e[4] = ps.h.t;
+    stk.mat[ps.level] = mi;
+    ps.level++;
+    path_march<MAXB>(sc, ps, rorg, rdir);
+    return;
+  }
+  V3 r = ps.radiance + ld3(m.emissive);
+  r = apply_fog(r, ld3(sc.fog), ps.h.t, sc.fog_density);
+  path_unwind<MAXB>(sc, ps, stk, r);
+}
+
+// Surface shading up to the lights (lib.rs:1286-1332): albedo, AO, GI, IBL, ambient.
+template <int MAXB>
+MM3E_DEV void path_shade_after_ao(const DevScene& sc, PathState& ps, LevelStack<MAXB>& stk, float occ) {
+  const DMat& m = sc.mat[ps.h.mat % sc.n_mat];
+  const V3 albedo = surface_albedo(m, ps.h.pos);
+  V3 gi = V3{0.0f, 0.0f, 0.0f};
+  if (sc.flags & SC_HAS_GI) gi = gi_sample(sc, ps.h.pos, ps.n);
+  const V3 amb = ibl_irradiance(sc, ps.n) + gi + ld3(sc.ambient);
+  ps.radiance = scale(cmul(albedo, amb), occ);
+  ps.li = 0;
+  if (path_lights<MAXB>(sc, ps, 0.0f, false)) return;
+  path_level_end<MAXB>(sc, ps, stk);
+}
+
+template <int MAXB>
+MM3E_DEV void path_shade(const DevScene& sc, PathState& ps, LevelStack<MAXB>& stk) {
+  float occ = 1.0f;
+  if (sc.flags & SC_AO) {
+    if (ao_init(sc, ps.x.ao, ps.h.pos, ps.n, ps.q, occ)) {
+      ps.phase = PH_AO;
+      return;
+    }
+  }
+  path_shade_after_ao<MAXB>(sc, ps, stk, occ);
+}
+
+template <int MAXB>
+MM3E_DEV void path_hit(const DevScene& sc, const Tables& tb, PathState& ps, LevelStack<MAXB>& stk) {
+  if (!ps.h.hit) {
+    path_unwind<MAXB>(sc, ps, stk, sky(ps.rd, ld3(sc.sun)));
+    return;
+  }
+  if (sc.flags & SC_DUAL_SAFE) {
+    ps.n = normal_dual(sc, tb, ps.h.pos);
+    path_shade<MAXB>(sc, ps, stk);
+    return;
+  }
+  ps.q = tetra_init(ps.x.tn, ps.h.pos, sc.normal_h);
+  ps.phase = PH_NORMAL;
+}
+
+// One step of the lane's current search with the field value at ps.q.  Returns true when
+// the search has finished; its result then waits in the search state (march: ps.h; normal:
+// ps.x.tn.g; AO / shadow: ps.occ_or_shadow) until the lane's transition runs.
+template <int MAXB>
+MM3E_DEV bool path_search_step(const DevScene& sc, PathState& ps, Field f) {
+  switch (ps.phase) {
+    case PH_MARCH:
+      ps.mst = march_step(sc, ps.x.m, ps.mst, f, ps.h, ps.q);
+      return ps.mst == MARCH_DONE;
+    case PH_NORMAL:
+      return tetra_step(ps.x.tn, f.dist, ps.q);
+    case PH_AO:
+      return ao_step(sc, ps.x.ao, f.dist, ps.q, ps.value);
+    default:  // PH_SHADOW
+      return shadow_step(sc, ps.x.sh, f.dist, ps.q, ps.value);
+  }
+}
+
+// The work between searches (shading, lights, reflection set-up, unwinding), run for a lane
+// whose search has finished; it ends by starting the next search or with PH_DONE.
+template <int MAXB>
+MM3E_DEV void path_transition(const DevScene& sc, const Tables& tb, PathState& ps, LevelStack<MAXB>& stk) {
+  switch (ps.phase) {
+    case PH_MARCH:
+      path_hit<MAXB>(sc, tb, ps, stk);
+      return;
+    case PH_NORMAL:
+      ps.n = normalize(ps.x.tn.g);
+      path_shade<MAXB>(sc, ps, stk);
+      return;
+    case PH_AO:
+      path_shade_after_ao<MAXB>(sc, ps, stk, ps.value);
+      return;
+    default:  // PH_SHADOW
+      if (path_lights<MAXB>(sc, ps, ps.value, true)) return;
+      path_level_end<MAXB>(sc, ps, stk);
+      return;
+  }
+}
+
+// Lanes whose search finished wait until at least this many lanes of the wave are waiting (or
+// no lane is searching any more) and then run their transitions together: the shading code
+// executes a few times per wave instead of once per finishing lane, while every field
+// evaluation is still shared by all searching lanes whatever search they are in.
+#ifndef MM3E_PATH_GATE
+#define MM3E_PATH_GATE 1
+#endif
+
 template <class Map, int MAXB>
-MM3E_DEV V3 trace_path(const DevScene& sc, const Map& field, V3 ro, V3 rd) {
-  return trace_level<Map, 0, MAXB>(sc, field, ro, rd);
+MM3E_DEV V3 trace_sm(const DevScene& sc, const Map& field, V3 ro, V3 rd) {
+  PathState ps;
+  LevelStack<MAXB> stk;
+  ps.level = 0;
+  path_march<MAXB>(sc, ps, ro, rd);
+  bool waiting = ps.mst == MARCH_DONE;
+  for (;;) {
+    const bool searching = ps.phase != PH_DONE && !waiting;
+    if (searching) waiting = path_search_step<MAXB>(sc, ps, field(ps.q));
+    const uint64_t wait_mask = __builtin_amdgcn_ballot_w64(waiting);
+    const uint64_t search_mask = __builtin_amdgcn_ballot_w64(ps.phase != PH_DONE && !waiting);
+    if (wait_mask != 0 && (search_mask == 0 || __builtin_popcountll(wait_mask) >= MM3E_PATH_GATE)) {
+      if (waiting) {
+        path_transition<MAXB>(sc, field.tb, ps, stk);
+        // the only search that can finish before its first evaluation is a march with
+        // max_steps == 0; every other search either asks for a point or completes inside
+        // the transition
+        waiting = ps.phase == PH_MARCH && ps.mst == MARCH_DONE;
+      }
+    }
+    if (__builtin_amdgcn_ballot_w64(ps.phase != PH_DONE) == 0) break;
+  }
+  return ps.radiance;
 }
 
 // Render one tile: blockDim.x = tw*th*aa*aa lanes; lane -> (pixel in tile, sample).
-template <class Map, int MAXB>
+// SM selects the state-machine path (long world folds) over the nested one.
+template <class Map, int MAXB, bool SM>
 MM3E_DEV void render_tile(const DevScene* __restrict__ scp, const Tables& tb, const FrameParams& fp,
                           float* __restrict__ hdr, uint32_t tw, uint32_t th, uint32_t tile_x, uint32_t tile_y,
                           float* smem) {
@@ -104,7 +357,8 @@ MM3E_DEV void render_tile(const DevScene* __restrict__ scp, const Tables& tb, co
     const float oy = ((float)sy + 0.5f) / (float)aa;
     V3 ro, rd;
     camera_ray(fp, (float)px + ox, (float)py + oy, ro, rd);
-    rad = trace_path<Map, MAXB>(sc, field, ro, rd);
+    if constexpr (SM) rad = trace_sm<Map, MAXB>(sc, field, ro, rd);
+    else rad = trace_level<Map, 0, MAXB>(sc, field, ro, rd);
   }
   if (ns == 1) {
     if (valid) {

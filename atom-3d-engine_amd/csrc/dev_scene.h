@@ -72,6 +72,35 @@ struct alignas(16) DSeg {
   uint32_t pad;
 };
 
+// One instruction of an object-local CSG expression (mm3e-kit/src/csg.rs:95-118) compiled to
+// postfix: leaves push a distance, binary ops pop two and push one.  The evaluation point
+// lives in three registers: ENTER maps it into a Transform node's frame, LEAVE scales the
+// child's distance and resets the point to the object-local point, after which the ancestor
+// ENTERs are replayed (bit-identical to the recursion's saved point).
+enum : uint32_t {
+  CSG_LEAF = 0,       // push prim_distance(prim, p, point)
+  CSG_ENTER = 1,      // point = to_local(xform, point)
+  CSG_LEAVE = 2,      // top *= scale; point = object-local point
+  CSG_UNION = 3,
+  CSG_INTERSECT = 4,
+  CSG_SUBTRACT = 5,
+  CSG_SMOOTH = 6,     // k = p[0]
+  CSG_OFFSET = 7,     // top -= p[0]
+  CSG_SHELL = 8,      // top = |top| - p[0]
+};
+constexpr uint32_t kCsgStack = 10;  // postfix depth of any expression of <= 1024 nodes (csg.rs:13-14)
+
+struct alignas(16) DCsgOp {
+  uint32_t op, prim;
+  uint32_t swap;     // binary: operands were emitted b-first (deeper child first)
+  uint32_t pad;
+  float p[8];
+  float c0[3], inv;  // Transform::rot columns and 1/scale (0 for |scale| <= 1e-12)
+  float c1[3], scale;
+  float c2[3], pad1;
+  float pos[3], pad2;
+};
+
 struct alignas(16) DMat {  // mm3e-kit/src/color.rs:37-45
   float albedo[3];
   float metallic;
@@ -137,6 +166,18 @@ struct alignas(16) DevScene {
   const float* gi;       // [probe][6 faces][4] (RGB + pad), probe = (k*ny + j)*nx + i
   const DVol* vol;
   const float* vol_data;
+  // Segment candidate grid (grid == nullptr when absent).  Cubic cells of side 1/grid_inv
+  // from grid_lo; cell (x, y, z) has index (z * ny + y) * nx + x and index nx*ny*nz stands for
+  // every point outside the grid.  Each cell holds two bit masks over the segments (level 0 and
+  // level 1, grid_words words each, [cell][level][word]): bit s is clear only if segment s's
+  // members all lie farther than grid_m[level] from every point of the cell.  Built only for
+  // scenes without subtract segments, where the accumulator never increases along the fold.
+  float grid_lo[3], grid_inv;
+  uint32_t grid_dims[3], grid_words;
+  float grid_m[2], grid_kmax, grid_eps;
+  const uint32_t* grid;
+  // Postfix CSG programs; a CSG object's p[0] / p[1] hold its first op and op count (bits).
+  const DCsgOp* csg;
 };
 
 // Per-frame camera record (mm3e-kit/src/camera.rs:39-47) plus the image size.

@@ -100,13 +100,14 @@ extern "C" __global__ void __launch_bounds__(256)
 namespace mm3e {
 
 hipError_t launch_render_table(uint32_t maxb, dim3 grid, dim3 block, hipStream_t st, const DevScene* sc,
-                               uint32_t stage_bytes, const FrameParams& fp, float* hdr, uint32_t tw, uint32_t th) {
+                               uint32_t stage_bytes, bool general, const FrameParams& fp, float* hdr, uint32_t tw,
+                               uint32_t th) {
   switch (maxb) {
-    case 0: return launch_render_table_b0(grid, block, st, sc, stage_bytes, fp, hdr, tw, th);
-    case 1: return launch_render_table_b1(grid, block, st, sc, stage_bytes, fp, hdr, tw, th);
-    case 2: return launch_render_table_b2(grid, block, st, sc, stage_bytes, fp, hdr, tw, th);
-    case 3: return launch_render_table_b3(grid, block, st, sc, stage_bytes, fp, hdr, tw, th);
-    case 4: return launch_render_table_b4(grid, block, st, sc, stage_bytes, fp, hdr, tw, th);
+    case 0: return launch_render_table_b0(grid, block, st, sc, stage_bytes, general, fp, hdr, tw, th);
+    case 1: return launch_render_table_b1(grid, block, st, sc, stage_bytes, general, fp, hdr, tw, th);
+    case 2: return launch_render_table_b2(grid, block, st, sc, stage_bytes, general, fp, hdr, tw, th);
+    case 3: return launch_render_table_b3(grid, block, st, sc, stage_bytes, general, fp, hdr, tw, th);
+    case 4: return launch_render_table_b4(grid, block, st, sc, stage_bytes, general, fp, hdr, tw, th);
     default: return hipErrorInvalidValue;
   }
 }

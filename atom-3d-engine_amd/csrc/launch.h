@@ -27,8 +27,8 @@ hipError_t launch_batch_reset(hipStream_t st, uint32_t* counter);
 
 #define MM3E_DECL_RENDER(B)                                                                          \
   hipError_t launch_render_table_b##B(dim3 grid, dim3 block, hipStream_t st, const DevScene* sc, \
-                                      uint32_t stage_bytes, const FrameParams& fp, float* hdr, uint32_t tw,  \
-                                      uint32_t th);
+                                      uint32_t stage_bytes, bool general, const FrameParams& fp, float* hdr, \
+                                      uint32_t tw, uint32_t th);
 MM3E_DECL_RENDER(0)
 MM3E_DECL_RENDER(1)
 MM3E_DECL_RENDER(2)
@@ -36,9 +36,13 @@ MM3E_DECL_RENDER(3)
 MM3E_DECL_RENDER(4)
 #undef MM3E_DECL_RENDER
 
-// stage_bytes > 0 selects the variant that copies the scene tables into LDS first.
+// stage_bytes > 0 selects the variant that copies the scene tables into LDS first; general
+// selects the kernels with the segment grid and CSG (scene_is_general).
 hipError_t launch_render_table(uint32_t maxb, dim3 grid, dim3 block, hipStream_t st, const DevScene* sc,
-                               uint32_t stage_bytes, const FrameParams& fp, float* hdr, uint32_t tw, uint32_t th);
+                               uint32_t stage_bytes, bool general, const FrameParams& fp, float* hdr, uint32_t tw,
+                               uint32_t th);
+// Whether a compiled scene needs the general kernels: it carries a segment grid or CSG programs.
+inline bool scene_is_general(const DevScene& h) { return h.grid != nullptr || h.csg != nullptr; }
 hipError_t launch_gi_gather_table(dim3 grid, dim3 block, hipStream_t st, const DevScene* sc, uint32_t stage_bytes,
                                   const GiBakeParams& bp, float4* out, uint32_t first_probe, uint32_t n_probes);
 hipError_t launch_gi_reduce(dim3 grid, dim3 block, hipStream_t st, const GiBakeParams& bp, const float4* rays,

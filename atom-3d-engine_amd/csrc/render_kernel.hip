@@ -1,6 +1,10 @@
-// render_kernel.hip — the per-pixel render kernel over the table-driven world field for one
-// reflection depth.  Compiled once per MM3E_MAXB (0..4) into separate objects so the variants
-// build in parallel; the runtime picks the variant equal to the scene's `bounces`.
+// render_kernel.hip — the per-pixel render kernels for one reflection depth.  Compiled once per
+// MM3E_MAXB (0..4) into separate objects so the variants build in parallel; the runtime picks
+// the variant equal to the scene's `bounces`, and per scene one of two families:
+//   mm3e_render_{table,lds}_bN     scenes without a segment grid or CSG objects: the world fold
+//                                  without those features, march/shade as nested loops;
+//   mm3e_render_{table,lds}_x_bN   every other scene: the general fold (segment grid, CSG) and
+//                                  the single-evaluation-site state machine.
 #include <hip/hip_runtime.h>
 
 #include "dev_render.h"
@@ -14,32 +18,54 @@
 #define MM3E_CAT(a, b) MM3E_CAT2(a, b)
 #define KNAME MM3E_CAT(mm3e_render_table_b, MM3E_MAXB)
 #define KNAME_LDS MM3E_CAT(mm3e_render_lds_b, MM3E_MAXB)
+#define KNAME_X MM3E_CAT(mm3e_render_table_x_b, MM3E_MAXB)
+#define KNAME_LDS_X MM3E_CAT(mm3e_render_lds_x_b, MM3E_MAXB)
 #define LNAME MM3E_CAT(launch_render_table_b, MM3E_MAXB)
 
 using namespace mm3e;
 
 // Scene tables read from HBM (scenes whose tables exceed the LDS budget).
-extern "C" __global__ void __launch_bounds__(256)
+extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
     KNAME(const DevScene* __restrict__ sc, FrameParams fp, float* __restrict__ hdr, uint32_t tw, uint32_t th) {
   __shared__ float smem[256 * 3];
-  render_tile<TableMap, MM3E_MAXB>(sc, global_tables(sc), fp, hdr, tw, th, blockIdx.x, blockIdx.y, smem);
+  render_tile<SimpleMap, MM3E_MAXB, false>(sc, global_tables(sc), fp, hdr, tw, th, blockIdx.x, blockIdx.y, smem);
 }
 
 // Scene tables staged in LDS by each block (dynamic LDS = DevScene::stage_bytes).
-extern "C" __global__ void __launch_bounds__(256)
+extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
     KNAME_LDS(const DevScene* __restrict__ sc, FrameParams fp, float* __restrict__ hdr, uint32_t tw, uint32_t th,
               uint32_t stage_bytes) {
   __shared__ float smem[256 * 3];
   extern __shared__ uint4 tables_lds[];
   const Tables tb = stage_tables(sc, tables_lds, stage_bytes);
-  render_tile<TableMap, MM3E_MAXB>(sc, tb, fp, hdr, tw, th, blockIdx.x, blockIdx.y, smem);
+  render_tile<SimpleMap, MM3E_MAXB, false>(sc, tb, fp, hdr, tw, th, blockIdx.x, blockIdx.y, smem);
+}
+
+extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
+    KNAME_X(const DevScene* __restrict__ sc, FrameParams fp, float* __restrict__ hdr, uint32_t tw, uint32_t th) {
+  __shared__ float smem[256 * 3];
+  render_tile<TableMap, MM3E_MAXB, true>(sc, global_tables(sc), fp, hdr, tw, th, blockIdx.x, blockIdx.y, smem);
+}
+
+extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
+    KNAME_LDS_X(const DevScene* __restrict__ sc, FrameParams fp, float* __restrict__ hdr, uint32_t tw, uint32_t th,
+                uint32_t stage_bytes) {
+  __shared__ float smem[256 * 3];
+  extern __shared__ uint4 tables_lds[];
+  const Tables tb = stage_tables(sc, tables_lds, stage_bytes);
+  render_tile<TableMap, MM3E_MAXB, true>(sc, tb, fp, hdr, tw, th, blockIdx.x, blockIdx.y, smem);
 }
 
 namespace mm3e {
-hipError_t LNAME(dim3 grid, dim3 block, hipStream_t st, const DevScene* sc, uint32_t stage_bytes,
+hipError_t LNAME(dim3 grid, dim3 block, hipStream_t st, const DevScene* sc, uint32_t stage_bytes, bool general,
                  const FrameParams& fp, float* hdr, uint32_t tw, uint32_t th) {
-  if (stage_bytes) hipLaunchKernelGGL(KNAME_LDS, grid, block, stage_bytes, st, sc, fp, hdr, tw, th, stage_bytes);
-  else hipLaunchKernelGGL(KNAME, grid, block, 0, st, sc, fp, hdr, tw, th);
+  if (general) {
+    if (stage_bytes) hipLaunchKernelGGL(KNAME_LDS_X, grid, block, stage_bytes, st, sc, fp, hdr, tw, th, stage_bytes);
+    else hipLaunchKernelGGL(KNAME_X, grid, block, 0, st, sc, fp, hdr, tw, th);
+  } else {
+    if (stage_bytes) hipLaunchKernelGGL(KNAME_LDS, grid, block, stage_bytes, st, sc, fp, hdr, tw, th, stage_bytes);
+    else hipLaunchKernelGGL(KNAME, grid, block, 0, st, sc, fp, hdr, tw, th);
+  }
   return hipGetLastError();
 }
 }  // namespace mm3e

@@ -133,7 +133,7 @@ static void patch_gi(mm3e_scene* s, DevScene& d) {
 
 bool upload_tables(mm3e_scene* s, const std::vector<mm3e_ir_object>& xo, const std::vector<mm3e_ir_material>& xm,
                    uint8_t** d_buf, size_t* cap, DevScene* hdr_out, std::string& msg) {
-  const uint64_t need = blob_size(s->host, xo.size(), xm.size());
+  const uint64_t need = blob_size(s->host, xo, xm);
   if (*d_buf == nullptr || *cap < need) {
     if (*d_buf) hipFree(*d_buf);
     *d_buf = nullptr;
@@ -150,19 +150,25 @@ bool upload_tables(mm3e_scene* s, const std::vector<mm3e_ir_object>& xo, const s
   if (!hip_ok(hipStreamSynchronize(s->r->stream), "upload scene tables", &msg)) return false;
   if (hdr_out) *hdr_out = b.header;
   const uint32_t stage = b.header.stage_bytes <= s->r->lds_budget ? b.header.stage_bytes : 0;
-  if (*d_buf == s->d_tables) s->stage_bytes = stage;
-  else s->dyn_stage_bytes = stage;
+  if (*d_buf == s->d_tables) {
+    s->stage_bytes = stage;
+    s->general = scene_is_general(b.header);
+  } else {
+    s->dyn_stage_bytes = stage;
+    s->dyn_general = scene_is_general(b.header);
+  }
   return true;
 }
 
 bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std::string& msg) {
   hipStream_t st = s->r->stream;
   const uint32_t stage = dsc == s->d_scene ? s->stage_bytes : s->dyn_stage_bytes;
+  const bool general = dsc == s->d_scene ? s->general : s->dyn_general;
   dim3 grid((fp.width + s->tw - 1) / s->tw, (fp.height + s->th - 1) / s->th);
   dim3 block(s->tw * s->th * s->aa * s->aa);
   hipEvent_t* ev = s->ev[s->frames % mm3e_scene::kRing];
   hipEventRecord(ev[0], st);
-  if (!hip_ok(launch_render_table(s->bounces, grid, block, st, dsc, stage, fp, s->d_hdr, s->tw, s->th),
+  if (!hip_ok(launch_render_table(s->bounces, grid, block, st, dsc, stage, general, fp, s->d_hdr, s->tw, s->th),
               "render launch", &msg))
     return false;
   hipEventRecord(ev[1], st);
@@ -295,7 +301,7 @@ int mm3e_scene_compile(mm3e_renderer* r, const void* ir, size_t ir_len, uint32_t
   DevScene hdr;
   if (s->host.hdr.gi_mode == MM3E_GI_BAKED) {
     // point d_gi at the in-blob array after upload
-    const uint64_t need = blob_size(s->host, 0, 0);
+    const uint64_t need = blob_size(s->host, {}, {});
     ok = hip_ok(hipMalloc((void**)&s->d_tables, need), "hipMalloc(scene tables)", &msg);
     if (ok) {
       s->tables_cap = need;
@@ -308,6 +314,7 @@ int mm3e_scene_compile(mm3e_renderer* r, const void* ir, size_t ir_len, uint32_t
       ok = hip_ok(hipMemcpy(s->d_tables, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice), "upload tables", &msg);
       hdr = b.header;
       s->stage_bytes = b.header.stage_bytes <= r->lds_budget ? b.header.stage_bytes : 0;
+      s->general = scene_is_general(b.header);
     }
   } else {
     ok = upload_tables(s, {}, {}, &s->d_tables, &s->tables_cap, &hdr, msg);
@@ -471,6 +478,26 @@ int mm3e_scene_render_linear(mm3e_scene* s, const mm3e_camera* cam, float* out, 
   if (!hip_ok(hipMemcpyAsync(out, s->d_hdr, need * 4, hipMemcpyDeviceToHost, s->r->stream), "readback", &msg))
     return MM3E_ERR_DEVICE;
   return hip_ok(hipStreamSynchronize(s->r->stream), "frame", &msg) ? MM3E_OK : MM3E_ERR_DEVICE;
+}
+
+int mm3e_scene_info(const mm3e_scene* s, uint32_t* n_objects, uint32_t* n_segments, uint32_t* grid_cells,
+                    uint32_t* grid_words, uint32_t* stage_bytes) {
+  if (!s) return MM3E_ERR_INVALID;
+  const mm3e::DevScene& d = s->hdr;
+  if (n_objects) *n_objects = d.n_obj;
+  if (n_segments) *n_segments = d.n_seg;
+  if (grid_cells) *grid_cells = d.grid ? d.grid_dims[0] * d.grid_dims[1] * d.grid_dims[2] : 0u;
+  if (grid_words) *grid_words = d.grid ? d.grid_words : 0u;
+  if (stage_bytes) *stage_bytes = s->stage_bytes;
+  return MM3E_OK;
+}
+
+int mm3e_scene_kernel_name(const mm3e_scene* s, char* buf, size_t buflen) {
+  if (!s || !buf || buflen == 0) return MM3E_ERR_INVALID;
+  const char* base = s->general ? (s->stage_bytes ? "mm3e_render_lds_x_b" : "mm3e_render_table_x_b")
+                                 : (s->stage_bytes ? "mm3e_render_lds_b" : "mm3e_render_table_b");
+  std::snprintf(buf, buflen, "%s%u", base, s->bounces);
+  return MM3E_OK;
 }
 
 int mm3e_scene_last_timing(mm3e_scene* s, float* render_ms, float* post_ms) {

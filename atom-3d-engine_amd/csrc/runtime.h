@@ -45,6 +45,8 @@ struct mm3e_scene {
   bool timing_valid = false;
   uint32_t stage_bytes = 0;            // LDS bytes of the staged tables (0: read from HBM)
   uint32_t dyn_stage_bytes = 0;        // the same for the tables with dynamic spheres appended
+  bool general = false;                // the scene needs the general kernels (segment grid / CSG)
+  bool dyn_general = false;            // the same for the tables with dynamic spheres appended
 };
 
 namespace mm3e {

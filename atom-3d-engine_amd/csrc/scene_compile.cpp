@@ -19,6 +19,14 @@ namespace mm3e {
 #endif
 // Longest run folded linearly; longer runs walk their tree (O(log n) per evaluation).
 static const size_t kLinearRunMax = MM3E_LINEAR_RUN_MAX;
+// Segment candidate grid: built for scenes with at least this many segments; about
+// kGridTargetCells cells; level-0 / level-1 masks keep segments within slack + kGridNear /
+// slack + kGridFar of a cell.
+static const size_t kGridMinSegments = 8;
+static const double kGridTargetCells = 32768.0;
+static const double kGridMaxDim = 256.0;
+static const double kGridNear = 0.25;
+static const double kGridFar = 2.0;
 
 namespace {
 
@@ -53,6 +61,140 @@ Sphere enclose(F3 c1, float r1, F3 c2, float r2) {  // accel.rs:26-36
   return Sphere{true, {c.x, c.y, c.z}, radius};
 }
 
+}  // namespace
+
+Sphere csg_bound(const HostScene& sc, uint32_t ni) {  // csg.rs:120-150
+  const mm3e_ir_csg_node& n = sc.csg_nodes[ni];
+  const float* p = n.p;
+  auto s0 = [](float r) { return Sphere{true, {0.0f, 0.0f, 0.0f}, r}; };
+  const Sphere none{false, {0, 0, 0}, 0};
+  switch (n.op) {
+    case MM3E_CSG_PRIM:
+      switch (n.prim) {
+        case MM3E_PRIM_SPHERE: return s0(p[0]);
+        case MM3E_PRIM_BOX:
+        case MM3E_PRIM_ROUND_BOX: return s0(len(f3(p)));
+        case MM3E_PRIM_TORUS: return s0(p[0] + p[1]);
+        case MM3E_PRIM_CYLINDER: return s0(std::sqrt(p[0] * p[0] + p[1] * p[1]));
+        case MM3E_PRIM_CAPSULE: {
+          F3 a = f3(p), b = f3(p + 3);
+          F3 c = scl(add(a, b), 0.5f);
+          return Sphere{true, {c.x, c.y, c.z}, len(sub(a, b)) * 0.5f + p[6]};
+        }
+        case MM3E_PRIM_CONE: return Sphere{true, {0.0f, p[2] * 0.5f, 0.0f}, p[2] * 0.5f + std::fmax(p[0], p[1])};
+        case MM3E_PRIM_HEX_PRISM: return s0(std::sqrt(4.0f * p[0] * p[0] / 3.0f + p[1] * p[1]));
+        default: return none;  // ellipsoid, octahedron, plane: no proven underestimate
+      }
+    case MM3E_CSG_TRANSFORM: {
+      Sphere c = csg_bound(sc, n.a);
+      if (!c.some) return c;
+      F3 w = add(f3(n.pos), mul_vec(n.rot, scl(f3(c.c), n.scale)));  // vec.rs:190-192 to_world
+      return Sphere{true, {w.x, w.y, w.z}, c.r * n.scale};
+    }
+    case MM3E_CSG_UNION:
+    case MM3E_CSG_SMOOTH_UNION: {
+      Sphere a = csg_bound(sc, n.a);
+      if (!a.some) return none;
+      Sphere b = csg_bound(sc, n.b);
+      if (!b.some) return none;
+      Sphere e = enclose(f3(a.c), a.r, f3(b.c), b.r);
+      if (n.op == MM3E_CSG_SMOOTH_UNION) e.r += p[0] * 0.25f;
+      return e;
+    }
+    case MM3E_CSG_INTERSECT: {
+      Sphere a = csg_bound(sc, n.a), b = csg_bound(sc, n.b);
+      if (a.some && b.some) return a.r <= b.r ? a : b;
+      return a.some ? a : b;
+    }
+    case MM3E_CSG_SUBTRACT: return csg_bound(sc, n.a);
+    case MM3E_CSG_OFFSET: {
+      Sphere c = csg_bound(sc, n.a);
+      if (c.some) c.r = std::fmax(c.r + p[0], 0.0f);
+      return c;
+    }
+    case MM3E_CSG_SHELL: {
+      Sphere c = csg_bound(sc, n.a);
+      if (c.some) c.r = c.r + p[0];
+      return c;
+    }
+  }
+  return none;
+}
+
+namespace {
+// Registers a subtree needs in postfix order when the deeper child goes first (Sethi-Ullman).
+uint32_t csg_need(const HostScene& sc, uint32_t ni) {
+  const mm3e_ir_csg_node& n = sc.csg_nodes[ni];
+  if (n.op == MM3E_CSG_PRIM) return 1;
+  if (n.op == MM3E_CSG_TRANSFORM || n.op == MM3E_CSG_OFFSET || n.op == MM3E_CSG_SHELL) return csg_need(sc, n.a);
+  const uint32_t a = csg_need(sc, n.a), b = csg_need(sc, n.b);
+  return a == b ? a + 1 : std::max(a, b);
+}
+
+DCsgOp csg_xform_op(uint32_t op, const mm3e_ir_csg_node& n) {
+  DCsgOp o{};
+  o.op = op;
+  std::memcpy(o.c0, n.rot, 12);
+  std::memcpy(o.c1, n.rot + 3, 12);
+  std::memcpy(o.c2, n.rot + 6, 12);
+  std::memcpy(o.pos, n.pos, 12);
+  o.scale = n.scale;
+  o.inv = std::fabs(n.scale) > 1e-12f ? 1.0f / n.scale : 0.0f;  // vec.rs:216-221
+  return o;
+}
+
+void csg_emit(const HostScene& sc, uint32_t ni, std::vector<uint32_t>& frames, std::vector<DCsgOp>& out) {
+  const mm3e_ir_csg_node& n = sc.csg_nodes[ni];
+  switch (n.op) {
+    case MM3E_CSG_PRIM: {
+      DCsgOp o{};
+      o.op = CSG_LEAF;
+      o.prim = n.prim;
+      std::memcpy(o.p, n.p, 32);
+      out.push_back(o);
+      return;
+    }
+    case MM3E_CSG_TRANSFORM: {
+      out.push_back(csg_xform_op(CSG_ENTER, n));
+      frames.push_back(ni);
+      csg_emit(sc, n.a, frames, out);
+      frames.pop_back();
+      out.push_back(csg_xform_op(CSG_LEAVE, n));
+      for (uint32_t f : frames) out.push_back(csg_xform_op(CSG_ENTER, sc.csg_nodes[f]));  // back to the parent frame
+      return;
+    }
+    case MM3E_CSG_OFFSET:
+    case MM3E_CSG_SHELL: {
+      csg_emit(sc, n.a, frames, out);
+      DCsgOp o{};
+      o.op = n.op == MM3E_CSG_OFFSET ? CSG_OFFSET : CSG_SHELL;
+      o.p[0] = n.p[0];
+      out.push_back(o);
+      return;
+    }
+    default: {
+      const bool swap = csg_need(sc, n.b) > csg_need(sc, n.a);
+      csg_emit(sc, swap ? n.b : n.a, frames, out);
+      csg_emit(sc, swap ? n.a : n.b, frames, out);
+      DCsgOp o{};
+      o.op = n.op == MM3E_CSG_UNION ? CSG_UNION
+             : n.op == MM3E_CSG_INTERSECT ? CSG_INTERSECT
+             : n.op == MM3E_CSG_SUBTRACT ? CSG_SUBTRACT : CSG_SMOOTH;
+      o.swap = swap ? 1u : 0u;
+      o.p[0] = n.p[0];
+      out.push_back(o);
+    }
+  }
+}
+}  // namespace
+
+void csg_compile(const HostScene& sc, uint32_t id, std::vector<DCsgOp>& out) {
+  std::vector<uint32_t> frames;
+  csg_emit(sc, sc.csg_roots[id], frames, out);
+}
+
+namespace {
+
 Sphere local_bound(const HostScene& sc, const mm3e_ir_object& o) {  // lib.rs:264-292
   const float* p = o.p;
   auto s0 = [](float r) { return Sphere{true, {0.0f, 0.0f, 0.0f}, r}; };
@@ -72,6 +214,11 @@ Sphere local_bound(const HostScene& sc, const mm3e_ir_object& o) {  // lib.rs:26
     case MM3E_PRIM_ELLIPSOID: return s0(std::fmax(std::fmax(p[0], p[1]), p[2]));
     case MM3E_PRIM_OCTAHEDRON: return s0(p[0]);
     case MM3E_PRIM_HEX_PRISM: return s0(std::sqrt(p[0] * p[0] + p[1] * p[1]));
+    case MM3E_PRIM_CSG: {
+      uint32_t id;
+      std::memcpy(&id, &o.p[0], 4);
+      return csg_bound(sc, sc.csg_roots[id]);
+    }
     case MM3E_PRIM_VOLUME: {
       if (sc.volumes.empty()) return Sphere{false, {0, 0, 0}, 0};
       uint32_t vid;
@@ -166,7 +313,9 @@ bool parse_scene_ir(const void* ir, size_t len_, HostScene& sc, std::string& err
       !span(h.off_materials, (uint64_t)h.n_materials * sizeof(mm3e_ir_material)) ||
       !span(h.off_lights, (uint64_t)h.n_lights * sizeof(mm3e_ir_light)) ||
       !span(h.off_volumes, (uint64_t)h.n_volumes * sizeof(mm3e_ir_volume)) ||
-      !span(h.off_volume_data, (uint64_t)h.n_volume_floats * 4)) {
+      !span(h.off_volume_data, (uint64_t)h.n_volume_floats * 4) ||
+      !span(h.off_csg_nodes, (uint64_t)h.n_csg_nodes * sizeof(mm3e_ir_csg_node)) ||
+      !span(h.off_csg_roots, (uint64_t)h.n_csg_exprs * 4)) {
     err = "scene IR array lies outside the blob";
     return false;
   }
@@ -184,14 +333,43 @@ bool parse_scene_ir(const void* ir, size_t len_, HostScene& sc, std::string& err
   if (h.n_volumes) std::memcpy(sc.volumes.data(), b + h.off_volumes, h.n_volumes * sizeof(mm3e_ir_volume));
   sc.volume_data.resize(h.n_volume_floats);
   if (h.n_volume_floats) std::memcpy(sc.volume_data.data(), b + h.off_volume_data, (size_t)h.n_volume_floats * 4);
+  sc.csg_nodes.resize(h.n_csg_nodes);
+  if (h.n_csg_nodes) std::memcpy(sc.csg_nodes.data(), b + h.off_csg_nodes, h.n_csg_nodes * sizeof(mm3e_ir_csg_node));
+  sc.csg_roots.resize(h.n_csg_exprs);
+  if (h.n_csg_exprs) std::memcpy(sc.csg_roots.data(), b + h.off_csg_roots, (size_t)h.n_csg_exprs * 4);
+  // csg.rs:160-235 budgets: depth <= 64 (root = 1), <= 1024 nodes; children must exist.
+  for (uint32_t e = 0; e < h.n_csg_exprs; e++) {
+    uint32_t count = 0;
+    std::vector<std::pair<uint32_t, uint32_t>> stack{{sc.csg_roots[e], 1u}};
+    while (!stack.empty()) {
+      auto [ni, depth] = stack.back();
+      stack.pop_back();
+      if (ni >= h.n_csg_nodes || depth > 64 || ++count > 1024) {
+        err = "CSG expression exceeds depth/node budget";
+        return false;
+      }
+      const mm3e_ir_csg_node& n = sc.csg_nodes[ni];
+      if (n.op > MM3E_CSG_SHELL || (n.op == MM3E_CSG_PRIM && n.prim > MM3E_PRIM_PLANE)) {
+        err = "invalid CSG primitive or modifier parameters";
+        return false;
+      }
+      if (n.op == MM3E_CSG_PRIM) continue;
+      stack.push_back({n.a, depth + 1});
+      if (n.op >= MM3E_CSG_UNION && n.op <= MM3E_CSG_SMOOTH_UNION) stack.push_back({n.b, depth + 1});
+    }
+  }
   for (const auto& o : sc.objects) {
     if (o.prim == MM3E_PRIM_SURFACE) {  // wgsl.rs:45-47
       err = "native triangle surfaces are not supported by the GPU backend; use the CPU renderer";
       return false;
     }
     if (o.prim == MM3E_PRIM_CSG) {
-      err = "object-local CSG expressions (Prim::Csg) are not supported by the MI355X render path";
-      return false;
+      uint32_t id;
+      std::memcpy(&id, &o.p[0], 4);
+      if (id >= h.n_csg_exprs) {  // lib.rs:234 expect("unregistered CSG expression")
+        err = "unregistered CSG expression";
+        return false;
+      }
     }
     if (o.prim > MM3E_PRIM_SURFACE) {
       err = "unknown primitive kind in scene IR";
@@ -323,21 +501,136 @@ std::vector<float> gaussian_kernel(uint32_t radius) {  // post.rs:78-92
   return k;
 }
 
+// Segment candidate grid.  Every segment gets a sphere enclosing the bounding spheres of all
+// the objects it folds (computed in double and rounded outwards), or none when one of them is
+// unbounded (then the segment is a candidate everywhere).  A cell's level-L mask keeps the
+// segments whose sphere comes within m[L] of the cell, with the cell grown by a margin that
+// covers the float rounding of the device's cell index.  The device may drop the clear bits of
+// a lane's cell once max(acc + kmax, slack) < m[L] (after margins): every member of a dropped
+// segment then has a bounding-sphere lower bound above the threshold at which folding it is a
+// no-op (union: lower > acc; smooth union: lower - acc > k so h == 1; tree walks prune or keep
+// acc), and since acc only decreases along a fold without subtract segments it stays so.
+void build_seg_grid(const std::vector<DSeg>& segs, const std::vector<std::vector<uint32_t>>& members,
+                    const std::vector<Sphere>& bounds, float slack, SegGrid& g) {
+  g = SegGrid{};
+  const size_t n = segs.size();
+  if (n < kGridMinSegments) return;
+  double kmax = 0.0;
+  for (const DSeg& sg : segs) {
+    if (sg.kind == 0 && sg.combine == MM3E_COMBINE_SUBTRACT) return;
+    if (sg.kind == 0 && sg.combine == MM3E_COMBINE_SMOOTH && sg.k > 0.0f) kmax = std::max(kmax, (double)sg.k);
+  }
+  struct S { bool always; double c[3], r; };
+  std::vector<S> sph(n);
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  bool any_bounded = false;
+  for (size_t s = 0; s < n; s++) {
+    S& q = sph[s];
+    q.always = false;
+    double bl[3] = {1e300, 1e300, 1e300}, bh[3] = {-1e300, -1e300, -1e300};
+    for (uint32_t i : members[s]) {
+      if (!bounds[i].some || !std::isfinite(bounds[i].r)) { q.always = true; break; }
+      for (int a = 0; a < 3; a++) {
+        bl[a] = std::min(bl[a], (double)bounds[i].c[a] - (double)bounds[i].r);
+        bh[a] = std::max(bh[a], (double)bounds[i].c[a] + (double)bounds[i].r);
+      }
+    }
+    if (q.always || members[s].empty()) { q.always = true; continue; }
+    for (int a = 0; a < 3; a++) q.c[a] = 0.5 * (bl[a] + bh[a]);
+    double r = 0.0;
+    for (uint32_t i : members[s]) {
+      double dx = q.c[0] - bounds[i].c[0], dy = q.c[1] - bounds[i].c[1], dz = q.c[2] - bounds[i].c[2];
+      r = std::max(r, std::sqrt(dx * dx + dy * dy + dz * dz) + (double)bounds[i].r);
+    }
+    q.r = r * (1.0 + 1e-9) + 1e-9;
+    if (!std::isfinite(q.r)) { q.always = true; continue; }
+    any_bounded = true;
+    for (int a = 0; a < 3; a++) {
+      lo[a] = std::min(lo[a], q.c[a] - q.r);
+      hi[a] = std::max(hi[a], q.c[a] + q.r);
+    }
+  }
+  if (!any_bounded) return;
+  const double m0 = (double)slack + kGridNear, m1 = (double)slack + kGridFar;
+  double ext = 0.0;
+  for (int a = 0; a < 3; a++) ext = std::max(ext, hi[a] - lo[a] + 2.0 * m1);
+  const double pad = 0.01 + 1e-5 * ext;
+  double vol = 1.0;
+  for (int a = 0; a < 3; a++) {
+    lo[a] -= m1 + pad;
+    hi[a] += m1 + pad;
+    vol *= hi[a] - lo[a];
+  }
+  double cs = std::max(std::cbrt(vol / (double)kGridTargetCells), 1e-3 * ext);
+  uint32_t dims[3];
+  for (int a = 0; a < 3; a++) {
+    double d = std::ceil((hi[a] - lo[a]) / cs);
+    dims[a] = (uint32_t)std::min(std::max(d, 1.0), (double)kGridMaxDim);
+  }
+  // widen the cell if a dimension hit the cap so the grid still covers [lo, hi]
+  for (int a = 0; a < 3; a++) cs = std::max(cs, (hi[a] - lo[a]) / dims[a]);
+  const float csf = (float)cs;
+  const float inv = 1.0f / csf;
+  const double csd = 1.0 / (double)inv;  // the cell side the device's index actually uses
+  const uint64_t n_cells = (uint64_t)dims[0] * dims[1] * dims[2];
+  const uint32_t W = (uint32_t)((n + 31) / 32);
+  g.words.assign((n_cells + 1) * 2 * W, 0u);
+  const double e = 1e-3 * csd + 1e-5 * ext;  // index rounding margin
+  float lof[3];
+  for (int a = 0; a < 3; a++) lof[a] = (float)lo[a];
+  for (size_t s = 0; s < n; s++) {
+    const uint32_t word = (uint32_t)(s / 32), bit = 1u << (s % 32);
+    if (sph[s].always) {
+      for (uint64_t c = 0; c <= n_cells; c++) {
+        g.words[(c * 2 + 0) * W + word] |= bit;
+        g.words[(c * 2 + 1) * W + word] |= bit;
+      }
+      continue;
+    }
+    const S& q = sph[s];
+    int64_t r0[3], r1[3];
+    for (int a = 0; a < 3; a++) {
+      r0[a] = (int64_t)std::floor((q.c[a] - q.r - m1 - e - (double)lof[a]) / csd) - 1;
+      r1[a] = (int64_t)std::floor((q.c[a] + q.r + m1 + e - (double)lof[a]) / csd) + 1;
+      r0[a] = std::max<int64_t>(r0[a], 0);
+      r1[a] = std::min<int64_t>(r1[a], (int64_t)dims[a] - 1);
+    }
+    for (int64_t z = r0[2]; z <= r1[2]; z++)
+      for (int64_t y = r0[1]; y <= r1[1]; y++)
+        for (int64_t x = r0[0]; x <= r1[0]; x++) {
+          const int64_t ix[3] = {x, y, z};
+          double d2 = 0.0;
+          for (int a = 0; a < 3; a++) {
+            const double blo = (double)lof[a] + ix[a] * csd - e, bhi = (double)lof[a] + (ix[a] + 1) * csd + e;
+            const double t = q.c[a] < blo ? blo - q.c[a] : (q.c[a] > bhi ? q.c[a] - bhi : 0.0);
+            d2 += t * t;
+          }
+          const double tag = std::sqrt(d2) - q.r;
+          const uint64_t c = ((uint64_t)z * dims[1] + (uint64_t)y) * dims[0] + (uint64_t)x;
+          if (tag <= m0) g.words[(c * 2 + 0) * W + word] |= bit;
+          if (tag <= m1) g.words[(c * 2 + 1) * W + word] |= bit;
+        }
+  }
+  for (int a = 0; a < 3; a++) {
+    g.lo[a] = lof[a];
+    g.dims[a] = dims[a];
+  }
+  g.inv = inv;
+  g.n_words = W;
+  g.m[0] = (float)m0;
+  g.m[1] = (float)m1;
+  g.kmax = (float)kmax;
+  // absolute margin for the device's float bounding-sphere lower bounds inside the grid
+  g.eps = (float)(1e-4 + 4e-6 * (ext + std::fabs(lo[0]) + std::fabs(lo[1]) + std::fabs(lo[2])));
+}
+
 // Layout: [DevScene][objects][segments][nodes][unbounded][materials][lights][volumes]
-//         [volume data][gi]
-uint64_t blob_size(const HostScene& sc, size_t extra_objects, size_t extra_materials) {
-  const uint64_t n_obj = sc.objects.size() + extra_objects;
-  uint64_t s = align16(sizeof(DevScene));
-  s += align16(n_obj * sizeof(DObj));
-  s += align16((n_obj + 1) * sizeof(DSeg));
-  s += align16((2 * n_obj + 1) * sizeof(DNode));
-  s += align16((n_obj + 1) * sizeof(uint32_t));
-  s += align16((sc.materials.size() + extra_materials) * sizeof(DMat));
-  s += align16((sc.lights.size() + 1) * sizeof(DLight));
-  s += align16((sc.volumes.size() + 1) * sizeof(DVol));
-  s += align16((sc.volume_data.size() + 1) * sizeof(float));
-  if (sc.hdr.gi_mode == MM3E_GI_BAKED) s += align16((uint64_t)sc.hdr.gi_dims[0] * sc.hdr.gi_dims[1] * sc.hdr.gi_dims[2] * 24 * 4);
-  return s;
+//         [volume data][gi][segment grid][csg programs]
+uint64_t blob_size(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_objects,
+                   const std::vector<mm3e_ir_material>& extra_materials) {
+  Blob b;
+  build_blob(sc, extra_objects, extra_materials, 1, 0, 0, nullptr, false, b);
+  return b.bytes.size();
 }
 
 void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_objects,
@@ -349,6 +642,15 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
   mats.insert(mats.end(), extra_materials.begin(), extra_materials.end());
   const mm3e_ir_header& h = sc.hdr;
   const uint64_t n_obj = objs.size();
+
+  // CSG expressions -> postfix programs (one per registered expression)
+  std::vector<DCsgOp> csg_ops;
+  std::vector<uint32_t> csg_start(sc.csg_roots.size()), csg_count(sc.csg_roots.size());
+  for (uint32_t e = 0; e < sc.csg_roots.size(); e++) {
+    csg_start[e] = (uint32_t)csg_ops.size();
+    csg_compile(sc, e, csg_ops);
+    csg_count[e] = (uint32_t)csg_ops.size() - csg_start[e];
+  }
 
   // objects + bounds
   std::vector<DObj> dobj(n_obj);
@@ -366,6 +668,12 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
     d.mat = o.mat;
     std::memcpy(d.pos, o.pos, 12);
     std::memcpy(d.p, o.p, 32);
+    if (o.prim == MM3E_PRIM_CSG) {
+      uint32_t id;
+      std::memcpy(&id, &o.p[0], 4);
+      std::memcpy(&d.p[0], &csg_start[id], 4);
+      std::memcpy(&d.p[1], &csg_count[id], 4);
+    }
     std::memcpy(d.rep, o.repeat, 12);
     d.twist = o.twist;
     std::memcpy(d.elo, o.elongate, 12);
@@ -403,6 +711,7 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
   std::vector<DNode> nodes;
   std::vector<uint32_t> unb;
   std::vector<uint32_t> run;
+  std::vector<std::vector<uint32_t>> seg_members;  // objects folded by each segment
   // An object is "exact-bounded" when its field value never drops below its bounding-sphere
   // lower bound |p - c| - r: an exact Euclidean SDF (sphere, box, rounded box, torus, capped
   // cylinder, capsule, round cone with |r1 - r2| < h) placed by a rigid transform with positive
@@ -432,6 +741,7 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
       for (uint32_t i : run) unb.push_back(i);
       s.unb_count = (uint32_t)run.size();
       segs.push_back(s);
+      seg_members.push_back(run);
       run.clear();
       return;
     }
@@ -453,6 +763,7 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
       s.index = (uint32_t)root;
     }
     segs.push_back(s);
+    seg_members.push_back(run);
     run.clear();
   };
   for (uint32_t i = 0; i < n_obj; i++) {
@@ -466,9 +777,14 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
       s.combine = objs[i].combine;
       s.k = objs[i].k;
       segs.push_back(s);
+      seg_members.push_back(std::vector<uint32_t>{i});
     }
   }
   close_run();
+
+  // Segment candidate grid (dynamic-sphere frames rebuild their blob per frame and skip it).
+  SegGrid grid;
+  if (extra_objects.empty()) build_seg_grid(segs, seg_members, bounds, slack, grid);
 
   std::vector<DMat> dmat(mats.size());
   for (size_t i = 0; i < mats.size(); i++) {
@@ -522,6 +838,15 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
     gi_bytes = (uint64_t)h.gi_dims[0] * h.gi_dims[1] * h.gi_dims[2] * 24 * 4;
     off += align16(gi_bytes);
   }
+  uint64_t o_grid = 0, o_csg = 0;
+  if (!grid.words.empty()) {
+    o_grid = off;
+    off += align16(grid.words.size() * sizeof(uint32_t));
+  }
+  if (!csg_ops.empty()) {
+    o_csg = off;
+    off += align16(csg_ops.size() * sizeof(DCsgOp));
+  }
   out.bytes.assign(off, 0);
   put(out.bytes, o_obj, dobj.data(), dobj.size());
   put(out.bytes, o_seg, segs.data(), segs.size());
@@ -541,6 +866,8 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
     }
     put(out.bytes, o_gi, g4.data(), g4.size());
   }
+  if (o_grid) put(out.bytes, o_grid, grid.words.data(), grid.words.size());
+  if (o_csg) put(out.bytes, o_csg, csg_ops.data(), csg_ops.size());
   out.gi_offset = o_gi;
   out.gi_bytes = gi_bytes;
 
@@ -597,6 +924,20 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
   d.st_seg = (uint32_t)(o_seg - o_obj);
   d.st_node = (uint32_t)(o_node - o_obj);
   d.st_unb = (uint32_t)(o_unb - o_obj);
+  if (o_grid) {
+    for (int a = 0; a < 3; a++) {
+      d.grid_lo[a] = grid.lo[a];
+      d.grid_dims[a] = grid.dims[a];
+    }
+    d.grid_inv = grid.inv;
+    d.grid_words = grid.n_words;
+    d.grid_m[0] = grid.m[0];
+    d.grid_m[1] = grid.m[1];
+    d.grid_kmax = grid.kmax;
+    d.grid_eps = grid.eps;
+    d.grid = (const uint32_t*)(dev_base + o_grid);
+  }
+  if (o_csg) d.csg = (const DCsgOp*)(dev_base + o_csg);
   std::memcpy(out.bytes.data(), &d, sizeof d);
 }
 

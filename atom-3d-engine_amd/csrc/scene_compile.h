@@ -25,6 +25,8 @@ struct HostScene {
   std::vector<mm3e_ir_volume> volumes;
   std::vector<float> volume_data;
   std::vector<float> gi_cubes;  // [probe][6][3] when hdr.gi_mode == MM3E_GI_BAKED
+  std::vector<mm3e_ir_csg_node> csg_nodes;
+  std::vector<uint32_t> csg_roots;   // expression id -> root node
 };
 
 // Parse + validate the IR (the GPU backend's checks, mm3e-gpu/src/lib.rs:138-142 and
@@ -36,6 +38,10 @@ struct Sphere {
   float c[3];
   float r;
 };
+// csg.rs:120-150 Expr::bound of the expression rooted at `node`.
+Sphere csg_bound(const HostScene& sc, uint32_t node);
+// Compile expression `id` to the postfix program evaluated by csg_distance on the device.
+void csg_compile(const HostScene& sc, uint32_t id, std::vector<DCsgOp>& out);
 // Object::world_bound (lib.rs:470-496).
 Sphere world_bound(const HostScene& sc, const mm3e_ir_object& o);
 // gi::bounds_of over all bounded objects with pad 1.5 (lib.rs:668-671, gi.rs:192-203).
@@ -57,7 +63,18 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
                 const std::vector<mm3e_ir_material>& extra_materials, uint32_t aa, uint32_t bounces,
                 uint64_t dev_base, const float* gi_dev, bool with_gi_flag, Blob& out);
 // Size the blob would have (for allocation before the base address is known).
-uint64_t blob_size(const HostScene& sc, size_t extra_objects, size_t extra_materials);
+uint64_t blob_size(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_objects,
+                   const std::vector<mm3e_ir_material>& extra_materials);
+
+// Segment candidate grid as laid out in the blob (DevScene::grid*).
+struct SegGrid {
+  float lo[3] = {0, 0, 0}, inv = 0;
+  uint32_t dims[3] = {0, 0, 0}, n_words = 0;
+  float m[2] = {0, 0}, kmax = 0, eps = 0;
+  std::vector<uint32_t> words;  // [cell (+1 outside)][level][word]
+};
+void build_seg_grid(const std::vector<DSeg>& segs, const std::vector<std::vector<uint32_t>>& members,
+                    const std::vector<Sphere>& bounds, float slack, SegGrid& g);
 
 // gi.rs:38-50 axis_dirs for the six cube faces: xyz + weight per ray.
 void gi_axis_dirs(uint32_t samples, float dirs[6][5][4], uint32_t* rays_per_face);

@@ -361,3 +361,80 @@ def c5_scene(seed: int = 12, width: int = 3840, height: int = 2160, n_prims: int
     cam = orbit_camera((0.0, 1.0, 0.0), rng.uniform(16.0, 20.0), rng.uniform(-math.pi, math.pi), 0.35,
                        math.radians(55.0))
     return sc, cam
+
+
+# --------------------------------------------------------------------------------------------
+# object-local CSG (Prim::Csg, mm3e-kit/src/csg.rs): every node kind, nested transforms
+# --------------------------------------------------------------------------------------------
+
+def csg_scene(width: int = 320, height: int = 180, seed: int = 7) -> Tuple[Scene, np.ndarray]:
+    """A floor plus four CSG objects covering every expression node: a drilled, smooth-blended
+    sphere; a shelled, plane-clipped cap (csg.rs:264-274); a three-level transform chain with
+    offset; a balanced union tree (8 leaves) that needs four operand registers."""
+    from .scene import CsgNode, CSG_PRIM, CSG_TRANSFORM, CSG_UNION, CSG_INTERSECT, CSG_SUBTRACT
+    from .scene import CSG_SMOOTH_UNION, CSG_OFFSET, CSG_SHELL
+    from .scene import PRIM_SPHERE, PRIM_BOX, PRIM_TORUS, PRIM_PLANE, PRIM_CAPSULE, PRIM_CYLINDER
+    from .scene import PRIM_HEX_PRISM, PRIM_CONE, PRIM_ELLIPSOID, PRIM_OCTAHEDRON, PRIM_ROUND_BOX
+    rng = SplitMix64(seed)
+    sc = Scene(width, height)
+    sc.aa = 1
+    floor = sc.material(Material.solid((0.8, 0.8, 0.8)).checkered().roughness(0.7))
+    mats = [sc.material(Material.solid((rng.uniform(0.1, 0.9), rng.uniform(0.1, 0.9), rng.uniform(0.1, 0.9)))
+                        .roughness(rng.uniform(0.2, 0.8)).metallic(1.0 if i == 2 else 0.0)) for i in range(4)]
+    sc.add(Object(Prim.plane((0.0, 1.0, 0.0), 0.0), Transform(), floor))
+    T = lambda pos, rot=(0.0, 0.0, 0.0), s=1.0: Transform(pos=v3(pos), rot=from_euler(*rot), scale=s)
+    # 1: (sphere - rotated box) smooth-union torus
+    e1 = sc.csg([CsgNode(CSG_PRIM, PRIM_SPHERE, p=(1.0,)),
+                 CsgNode(CSG_PRIM, PRIM_BOX, p=(0.55, 0.55, 0.55)),
+                 CsgNode(CSG_TRANSFORM, a=1, xform=T((0.6, 0.4, 0.5), (0.3, 0.5, 0.1))),
+                 CsgNode(CSG_SUBTRACT, a=0, b=2),
+                 CsgNode(CSG_PRIM, PRIM_TORUS, p=(0.7, 0.12)),
+                 CsgNode(CSG_TRANSFORM, a=4, xform=T((0.0, -0.6, 0.0), (0.2, 0.0, 0.0), 1.2)),
+                 CsgNode(CSG_SMOOTH_UNION, a=3, b=5, p=(0.25,))], root=6)
+    sc.add(Object(Prim.csg(e1), T((-2.2, 1.1, 0.0), (0.0, 0.4, 0.0)), mats[0]))
+    # 2: shell(sphere) clipped by a plane (an open cap), then offset
+    e2 = sc.csg([CsgNode(CSG_PRIM, PRIM_SPHERE, p=(1.0,)),
+                 CsgNode(CSG_SHELL, a=0, p=(0.05,)),
+                 CsgNode(CSG_PRIM, PRIM_PLANE, p=(0.0, -1.0, 0.0, 0.3)),
+                 CsgNode(CSG_INTERSECT, a=1, b=2),
+                 CsgNode(CSG_OFFSET, a=3, p=(0.02,))], root=4)
+    sc.add(Object(Prim.csg(e2), T((0.0, 1.2, 1.5), (0.4, 0.0, 0.2), 0.9), mats[1]))
+    # 3: three nested transforms around a capsule, unioned with a cylinder at the outer level
+    e3 = sc.csg([CsgNode(CSG_PRIM, PRIM_CAPSULE, p=(0.0, -0.4, 0.0, 0.0, 0.4, 0.0, 0.2)),
+                 CsgNode(CSG_TRANSFORM, a=0, xform=T((0.3, 0.0, 0.0), (0.0, 0.0, 0.7), 0.9)),
+                 CsgNode(CSG_TRANSFORM, a=1, xform=T((0.0, 0.2, 0.1), (0.5, 0.0, 0.0), 1.1)),
+                 CsgNode(CSG_TRANSFORM, a=2, xform=T((-0.1, 0.0, 0.3), (0.0, 0.9, 0.0), 0.8)),
+                 CsgNode(CSG_PRIM, PRIM_CYLINDER, p=(0.6, 0.25)),
+                 CsgNode(CSG_UNION, a=3, b=4),
+                 CsgNode(CSG_OFFSET, a=5, p=(-0.03,))], root=6)
+    sc.add(Object(Prim.csg(e3), T((2.2, 0.9, 0.2), (0.1, 0.2, 0.3)), mats[2]))
+    # 4: balanced tree of 8 leaves (all analytic kinds), right child deeper at the root
+    leaves = [CsgNode(CSG_PRIM, PRIM_SPHERE, p=(0.3,)), CsgNode(CSG_PRIM, PRIM_BOX, p=(0.25, 0.2, 0.3)),
+              CsgNode(CSG_PRIM, PRIM_HEX_PRISM, p=(0.25, 0.2)), CsgNode(CSG_PRIM, PRIM_CONE, p=(0.25, 0.1, 0.5)),
+              CsgNode(CSG_PRIM, PRIM_ELLIPSOID, p=(0.3, 0.2, 0.25)), CsgNode(CSG_PRIM, PRIM_OCTAHEDRON, p=(0.3,)),
+              CsgNode(CSG_PRIM, PRIM_ROUND_BOX, p=(0.2, 0.2, 0.2, 0.05)), CsgNode(CSG_PRIM, PRIM_TORUS, p=(0.25, 0.08))]
+    nodes = list(leaves)
+    placed = []
+    for i in range(8):
+        nodes.append(CsgNode(CSG_TRANSFORM, a=i, xform=T(((i % 4) * 0.45 - 0.7, (i // 4) * 0.5, 0.0))))
+        placed.append(len(nodes) - 1)
+    lvl = placed
+    ops = [CSG_UNION, CSG_SMOOTH_UNION, CSG_UNION, CSG_INTERSECT]
+    k = 0
+    while len(lvl) > 1:
+        nxt = []
+        for j in range(0, len(lvl), 2):
+            op = ops[k % len(ops)] if len(lvl) > 2 else CSG_UNION
+            k += 1
+            nodes.append(CsgNode(op, a=lvl[j], b=lvl[j + 1], p=(0.1,)))
+            nxt.append(len(nodes) - 1)
+        lvl = nxt
+    nodes.append(CsgNode(CSG_PRIM, PRIM_SPHERE, p=(0.2,)))
+    small = len(nodes) - 1
+    nodes.append(CsgNode(CSG_UNION, a=small, b=lvl[0]))   # shallow left, deep right: swapped order
+    e4 = sc.csg(nodes, root=len(nodes) - 1)
+    sc.add(Object(Prim.csg(e4), T((0.0, 0.6, -1.8), (0.0, 0.3, 0.0), 1.3), mats[3]))
+    sc.light(Light.directional_light(normalize((0.5, 0.8, 0.3)), (1.3, 1.3, 1.3)).soft(0.05))
+    sc.light(Light.point((2.0, 3.5, 2.0), (4.0, 3.6, 3.2)))
+    cam = look_at((0.0, 3.0, -6.5), (0.0, 0.8, 0.0), (0.0, 1.0, 0.0), math.radians(55.0))
+    return sc, cam

@@ -26,7 +26,8 @@ EXPORTS = (
     "mm3e_batch_set_camera", "mm3e_batch_launch", "mm3e_batch_wait", "mm3e_batch_read", "mm3e_batch_info",
     "mm3e_batch_destroy", "mm3e_render_batch", "mm3e_server_default_config", "mm3e_server_create",
     "mm3e_server_add_scene", "mm3e_server_submit", "mm3e_server_poll", "mm3e_server_wait", "mm3e_server_stats",
-    "mm3e_server_destroy",
+    "mm3e_server_destroy", "mm3e_scene_kernel_name",
+    "mm3e_scene_info",
 )
 
 _lib = None
@@ -60,6 +61,8 @@ def lib():
         L.mm3e_scene_gi_raw.argtypes = [P, P, P, P, P, P]
         L.mm3e_scene_last_timing.argtypes = [P, P, P]
         L.mm3e_scene_timing.argtypes = [P, C.c_int, P, P, P]
+        L.mm3e_scene_kernel_name.argtypes = [P, C.c_char_p, C.c_size_t]
+        L.mm3e_scene_info.argtypes = [P, P, P, P, P, P]
         L.mm3e_scene_field.argtypes = [P, P, C.c_uint32, P, P]
         L.mm3e_scene_march.argtypes = [P, P, C.c_uint32, P, P, P, P, P]
         L.mm3e_camera_look_at.argtypes = [P, P, P, C.c_float, P]
@@ -242,6 +245,21 @@ class GpuScene:
         if rc != 0:
             raise Mm3eError(rc, "no frame timed yet")
         return a.value, b.value
+
+    def info(self):
+        v = [C.c_uint32() for _ in range(5)]
+        rc = lib().mm3e_scene_info(self.h, *[C.byref(x) for x in v])
+        if rc != 0:
+            raise Mm3eError(rc, "info failed")
+        return dict(zip(("objects", "segments", "grid_cells", "grid_words", "stage_bytes"), [x.value for x in v]))
+
+    def kernel_name(self) -> str:
+        """Render kernel that render_only / render_rgba dispatch for this scene."""
+        buf = C.create_string_buffer(64)
+        rc = lib().mm3e_scene_kernel_name(self.h, buf, 64)
+        if rc != 0:
+            raise Mm3eError(rc, "kernel_name failed")
+        return buf.value.decode()
 
     def timing(self, reset=True):
         """(frames, mean render-kernel ms, mean post ms) over the frames since the last reset."""

@@ -14,6 +14,17 @@ counted by the instrumented oracle on the same scene and camera) divided by its 
 HIP-event duration over the timed frames, against MI355X's vector FP32 peak.
 cpu_baseline: the CPU oracle (a C++ restatement of the reference's multithreaded CPU
 renderer) timed on this host on one full frame of the same workload.
+
+Two more legs ride on the same line (``--legs`` picks them):
+  batched: 64 cameras over 8 seeded procedural scenes (20-200 primitives, resolutions from
+           {320x180, 640x360, 1280x720, 1920x1080}) rendered once as 64 plain per-frame launches
+           (each followed by a wait, as render() does) and once as one autotuned persistent
+           tile-queue launch replayed from a hipGraph; Mpix/s of both and their ratio.
+  served:  requests over 32 resident scenes (16^3 GI probes each; 854x480 50 %, 1280x720 35 %,
+           1920x1080 15 %) arriving as a seeded Poisson stream at offered rates set from the
+           plain path's measured capacity, plus a burst; p50/p99 latency (arrival -> RGBA8 in
+           host memory) and Mpix/s for the continuous-batching server and for a FIFO server
+           that renders each request by a plain per-frame launch.
 """
 import argparse
 import json
@@ -48,12 +59,164 @@ def cpu_baseline(ir, cam, threads, frames=2):
     return s.width * s.height / dt / 1e6, dt
 
 
+def pct(xs, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if xs else None
+
+
+def batched_leg(hip, configs, r, reps):
+    """64 plain per-frame launches vs one persistent tile-queue launch over the same jobs."""
+    jobs = configs.c3_jobs()
+    scenes = {}
+    compiled = []
+    t0 = time.perf_counter()
+    for sc, cam, (w, h) in jobs:
+        key = (id(sc), w, h)
+        if key not in scenes:
+            scenes[key] = r.compile(sc.to_ir(), w, h)
+        compiled.append((scenes[key], cam, (w, h)))
+    px = sum(w * h for _, _, (w, h) in jobs)
+    log(f"batched: {len(jobs)} jobs, {px / 1e6:.1f} Mpix, {len(scenes)} compiled (scene, size) pairs in "
+        f"{time.perf_counter() - t0:.2f} s")
+    # plain: one launch per frame, each waited for (render_only + wait_idle)
+    for g, cam, _ in compiled[:4]:
+        g.render_only(cam)
+        r.wait_idle()
+    plain = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        for g, cam, _ in compiled:
+            g.render_only(cam)
+            r.wait_idle()
+        plain.append(time.perf_counter() - t0)
+    plain_s = min(plain)
+    b = hip.Batch(r, compiled)
+    t0 = time.perf_counter()
+    report = b.autotune(reps=1)
+    tune_s = time.perf_counter() - t0
+    info = b.info()
+    log(f"batched: autotune {tune_s:.1f} s -> tile {info['tile']}, {info['blocks_per_cu']} blocks/CU")
+    b.launch()
+    b.wait()
+    wall, gpu = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        b.launch()
+        gpu.append(b.wait())
+        wall.append(time.perf_counter() - t0)
+    batched_s = min(wall)
+    b.close()
+    for g in scenes.values():
+        g.close()
+    out = {"workload": "64 cameras over 8 seeded procedural scenes (20-200 primitives, 30 % smooth, 20 % domain "
+                       "ops), per-camera resolution uniform over {320x180, 640x360, 1280x720, 1920x1080}, aa=1, "
+                       "1 reflection bounce",
+           "jobs": len(jobs), "mpix_per_batch": round(px / 1e6, 3),
+           "plain_mpix_s": round(px / plain_s / 1e6, 3), "plain_s": round(plain_s, 4),
+           "batched_mpix_s": round(px / batched_s / 1e6, 3), "batched_s": round(batched_s, 4),
+           "batched_gpu_ms": round(min(gpu), 3), "speedup_vs_plain": round(plain_s / batched_s, 3),
+           "tile": list(info["tile"]), "blocks_per_cu": info["blocks_per_cu"], "autotune_s": round(tune_s, 2)}
+    log(f"batched: plain {out['plain_mpix_s']} Mpix/s, batched {out['batched_mpix_s']} Mpix/s "
+        f"({out['speedup_vs_plain']}x)")
+    return out
+
+
+def served_leg(hip, configs, r, n_req, n_scenes=32):
+    """Poisson request stream: continuous-batching server vs a FIFO server of plain launches."""
+    import numpy as np
+    t0 = time.perf_counter()
+    scs = configs.c4_scenes(n_scenes=n_scenes)
+    rng = configs.SplitMix64(11)
+    reqs = [configs.c4_request(rng, n_scenes) for _ in range(n_req)]
+    need = sorted({(si, w, h) for si, _, (w, h) in reqs})
+    plain_sc = {}
+    for si, w, h in need:
+        plain_sc[(si, w, h)] = r.compile(scs[si].to_ir(), w, h)
+    base = [r.compile(sc.to_ir(), 854, 480) for sc in scs]
+    log(f"served: {n_scenes} resident scenes + {len(plain_sc)} plain (scene, size) pairs compiled with device GI "
+        f"bakes in {time.perf_counter() - t0:.2f} s; {n_req} requests")
+    px_req = [w * h for _, _, (w, h) in reqs]
+    bufs = {(w, h): np.zeros((h, w, 4), np.uint8) for _, _, (w, h) in reqs}
+    for (si, w, h), g in list(plain_sc.items())[:3]:
+        g.render_rgba(reqs[0][1])
+
+    def plain_run(arrivals):
+        lat = []
+        t_start = time.perf_counter()
+        for (si, cam, (w, h)), a in zip(reqs, arrivals):
+            now = time.perf_counter() - t_start
+            if a > now:
+                time.sleep(a - now)
+            plain_sc[(si, w, h)].render_rgba(cam)
+            lat.append((time.perf_counter() - t_start - a) * 1e3)
+        span = time.perf_counter() - t_start
+        return lat, span
+
+    srv = hip.Server(r, hbm_budget_bytes=8 << 30, max_batch_requests=32, max_inflight=2)
+    ids = [srv.add_scene(g) for g in base]
+
+    def served_run(arrivals):
+        rids = []
+        t_start = time.perf_counter()
+        for (si, cam, (w, h)), a in zip(reqs, arrivals):
+            now = time.perf_counter() - t_start
+            if a > now:
+                time.sleep(a - now)
+            rids.append(srv.submit(ids[si], cam, w, h, keep_image=False))
+        lat = [srv.wait(x)[1] for x in rids]
+        span = time.perf_counter() - t_start
+        return lat, span
+
+    served_run([0.0] * len(reqs))  # warm the server's pinned staging and slab
+    burst = [0.0] * len(reqs)
+    plain_lat, plain_span = plain_run(burst)
+    served_lat, served_span = served_run(burst)
+    px = sum(px_req)
+    plain_cap_qps = len(reqs) / plain_span
+    points = []
+    prng = np.random.default_rng(11)
+    gaps = prng.exponential(1.0, len(reqs))
+    for frac in (0.5, 0.9):
+        qps = plain_cap_qps * frac
+        arr = list(np.cumsum(gaps / qps) - gaps[0] / qps)
+        pl, ps = plain_run(arr)
+        sl, ss = served_run(arr)
+        points.append({"offered_qps": round(qps, 2),
+                       "plain": {"mpix_s": round(px / ps / 1e6, 3), "p50_ms": round(pct(pl, 50), 2),
+                                 "p99_ms": round(pct(pl, 99), 2)},
+                       "served": {"mpix_s": round(px / ss / 1e6, 3), "p50_ms": round(pct(sl, 50), 2),
+                                  "p99_ms": round(pct(sl, 99), 2)}})
+        log(f"served: {qps:.1f} req/s offered: plain p50/p99 {pct(pl, 50):.1f}/{pct(pl, 99):.1f} ms, "
+            f"served p50/p99 {pct(sl, 50):.1f}/{pct(sl, 99):.1f} ms")
+    st = srv.stats()
+    srv.close()
+    for g in list(plain_sc.values()) + base:
+        g.close()
+    out = {"workload": f"{n_req} requests over {n_scenes} resident seeded procedural scenes (20-200 primitives, "
+                       "16^3 GI probes baked on device), resolution 854x480 50 % / 1280x720 35 % / 1920x1080 15 %, "
+                       "seeded orbit poses; latency = arrival -> RGBA8 in host memory",
+           "requests": len(reqs), "mpix_requested": round(px / 1e6, 3),
+           "burst": {"plain": {"mpix_s": round(px / plain_span / 1e6, 3), "p50_ms": round(pct(plain_lat, 50), 2),
+                               "p99_ms": round(pct(plain_lat, 99), 2)},
+                     "served": {"mpix_s": round(px / served_span / 1e6, 3),
+                                "p50_ms": round(pct(served_lat, 50), 2), "p99_ms": round(pct(served_lat, 99), 2)},
+                     "served_vs_plain": round(plain_span / served_span, 3)},
+           "poisson": points, "server_stats": st}
+    log(f"served: burst plain {out['burst']['plain']['mpix_s']} Mpix/s, served {out['burst']['served']['mpix_s']} "
+        f"Mpix/s ({out['burst']['served_vs_plain']}x)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--legs", default="plain,batched,served",
+                    help="comma list of plain (always run), batched, served")
+    ap.add_argument("--batched-reps", type=int, default=3)
+    ap.add_argument("--served-requests", type=int, default=48)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device copy (torch), used to scale rocprof "
                          "FETCH_SIZE/WRITE_SIZE readings to bytes")
@@ -105,7 +268,7 @@ def main():
     assert fl["width"] == W and fl["height"] == H
     achieved = fl["flops_per_frame"] / (render_ms * 1e-3) / 1e12
     roofline = {
-        "bound": "valu", "kernel": "mm3e_render_table_b1", "achieved": round(achieved, 3),
+        "bound": "valu", "kernel": g.kernel_name(), "achieved": round(achieved, 3),
         "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
         "frac_nonpacked_peak": round(achieved / PEAK_FP32_NONPACKED_TFLOPS, 4),
         "flops_per_launch": fl["flops_per_frame"], "kernel_ms": round(render_ms, 4),
@@ -136,6 +299,11 @@ def main():
                                "sample": f"full c2 960x540 frame (aa=2), 2 timed frames after 1 warm-up, "
                                          f"{cdt:.2f} s/frame"}
         log(f"cpu baseline {cv:.4f} Mpix/s ({cdt:.2f} s/frame)")
+    legs = set(args.legs.split(","))
+    if "batched" in legs:
+        out["batched"] = batched_leg(hip, configs, r, args.batched_reps)
+    if "served" in legs:
+        out["served"] = served_leg(hip, configs, r, args.served_requests)
     print(json.dumps(out), flush=True)
     g.close()
     r.close()

@@ -102,6 +102,14 @@ int mm3e_scene_render_aov(mm3e_scene* s, const mm3e_camera* cam, uint8_t* hit, u
 /* Raw GI probe volume of the compiled scene (GiVolume::raw, gi.rs:133-135): min, size, dims
  * and, when cubes != NULL, the [probe][6][3] irradiance cubes.  *n_probes = 0 if no GI. */
 int mm3e_scene_gi_raw(mm3e_scene* s, float* min3, float* size3, uint32_t* dims3, float* cubes, uint64_t* n_probes);
+/* Compiled layout: objects, world-plan segments (accel.rs:122-125), cells and words per
+ * mask of the segment candidate grid (0 when the scene has none) and LDS bytes of the
+ * staged tables (0 when they are read from HBM). */
+int mm3e_scene_info(const mm3e_scene* s, uint32_t* n_objects, uint32_t* n_segments, uint32_t* grid_cells,
+                    uint32_t* grid_words, uint32_t* stage_bytes);
+/* Name of the render kernel the plain per-frame launch of this scene dispatches (the
+ * LDS-staged or the HBM-table variant, for the scene's reflection depth). */
+int mm3e_scene_kernel_name(const mm3e_scene* s, char* buf, size_t buflen);
 /* Last frame's device time of the render kernel and of the post kernels, in milliseconds. */
 int mm3e_scene_last_timing(mm3e_scene* s, float* render_ms, float* post_ms);
 /* HIP-event kernel times summed over the frames launched since the last reset (at most the

@@ -54,6 +54,30 @@ def test_field_parity(renderer, which):
     assert (mg == mo).mean() >= 0.999
 
 
+@pytest.mark.parametrize("seed,n", [(31, 150), (44, 200)])
+def test_field_parity_segment_grid(renderer, seed, n):
+    """Spatially coherent probes (64 points per wave within a 0.3-unit box, centres from the
+    floor to above the scene and beyond the grid) so waves take the segment-grid path at both
+    mask levels; every value must match the oracle's full world-plan fold."""
+    sc = configs.procedural_scene(seed, n, 64, 36)
+    g, o = both(renderer, sc)
+    info = g.info()
+    assert info["grid_cells"] > 0 and info["segments"] >= 8
+    rng = np.random.default_rng(seed)
+    centres = np.concatenate([rng.uniform([-8, -0.3, -8], [8, 3.5, 8], (300, 3)),
+                              rng.uniform([-14, -1, -14], [14, 9, 14], (60, 3))])
+    pts = (centres[:, None, :] + rng.uniform(-0.15, 0.15, (len(centres), 64, 3))).reshape(-1, 3).astype(np.float32)
+    dg, mg = g.field(pts)
+    do, mo = o.field(pts)
+    tol = 5e-5 * np.maximum(np.abs(do), 1.0)
+    bad = np.abs(dg - do) > tol
+    report(f"field grid seed={seed} n={n}", {**info, "max_abs_err": float(np.abs(dg - do).max()),
+                                             "bad": int(bad.sum()), "mat_agree": float((mg == mo).mean()),
+                                             "exact": float((dg == do).mean())})
+    assert bad.mean() <= 1e-3
+    assert (mg == mo).mean() >= 0.999
+
+
 def test_march_parity(renderer):
     sc, cam = configs.c1_scene()
     g, o = both(renderer, sc)
@@ -114,6 +138,23 @@ def test_procedural_image_parity(renderer, seed, n):
     rng = configs.SplitMix64(seed)
     cam = configs.orbit_pose(rng)
     image_parity(renderer, sc, cam, f"procedural seed={seed} n={n}")
+
+
+def test_csg_objects_image_and_field_parity(renderer):
+    """Object-local CSG (Prim::Csg): every node kind, nested transforms, a tree evaluated with
+    its deeper child first; image/AOV bars and field probes on and around the objects."""
+    sc, cam = configs.csg_scene(320, 180)
+    g, o, _, _ = image_parity(renderer, sc, cam, "csg 320x180")
+    rng = np.random.default_rng(9)
+    pts = np.concatenate([field_points(8000, [-3.5, -0.2, -3.0], [3.5, 2.6, 2.8], 9),
+                          rng.normal([0.0, 0.6, -1.8], 0.6, (4000, 3)).astype(np.float32)])
+    dg, mg = g.field(pts)
+    do, mo = o.field(pts)
+    tol = 5e-5 * np.maximum(np.abs(do), 1.0)
+    bad = np.abs(dg - do) > tol
+    report("field csg", {"max_abs_err": float(np.abs(dg - do).max()), "bad": int(bad.sum()),
+                         "exact": float((dg == do).mean())})
+    assert bad.mean() <= 1e-3 and (mg == mo).mean() >= 0.999
 
 
 def test_reflections_fog_two_bounces(renderer):
