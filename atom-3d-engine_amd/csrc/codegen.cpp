@@ -1,0 +1,143 @@
+// codegen.cpp — scene -> HIP source.
+//
+// The reference compiles each Scene into WGSL text with the scene baked in
+// (mm3e-gpu/src/wgsl.rs:34-362: constants, CSG helpers, material switch, ordered `map`).  The
+// twin here emits the world fold of WorldPlan::eval (accel.rs:209-252) for one compiled scene:
+// every segment in plan order, each object handed to the same device helpers the table walk
+// uses (seg_one / run_member / contrib / object_distance) as a compile-time constant `DObj`, so
+// the compiler folds the transform, the primitive switch, the modifier branches and the
+// bounding sphere into immediates while every floating-point operation stays the one the table
+// path (and the reference) performs, in the same order.  Union runs that the reference walks
+// through a bounding-sphere tree keep the table walk (seg_step on that segment index).  The
+// segment stepping, with or without the segment grid, is world_eval's.
+// Constants are written as bit patterns, so the text round-trips every float exactly.
+#include "codegen.h"
+
+#include <cstdio>
+#include <cstring>
+
+namespace mm3e {
+
+namespace {
+
+void put_f(std::string& o, float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  char b[48];
+  std::snprintf(b, sizeof b, "__builtin_bit_cast(float, 0x%08xu)", u);
+  o += b;
+}
+void put_u(std::string& o, uint32_t u) {
+  char b[24];
+  std::snprintf(b, sizeof b, "0x%08xu", u);
+  o += b;
+}
+void put_fa(std::string& o, const float* a, int n) {
+  o += "{";
+  for (int i = 0; i < n; i++) {
+    if (i) o += ", ";
+    put_f(o, a[i]);
+  }
+  o += "}";
+}
+
+// Aggregate initialiser in DObj member order (dev_scene.h).
+void put_obj(std::string& o, const DObj& d) {
+  o += "{";
+  put_fa(o, d.c0, 3); o += ", "; put_f(o, d.inv); o += ", ";
+  put_fa(o, d.c1, 3); o += ", "; put_f(o, d.scale); o += ", ";
+  put_fa(o, d.c2, 3); o += ", "; put_u(o, d.mat); o += ", ";
+  put_fa(o, d.pos, 3); o += ", "; put_u(o, d.flags); o += ", ";
+  put_fa(o, d.p, 8); o += ", ";
+  put_fa(o, d.rep, 3); o += ", "; put_f(o, d.twist); o += ", ";
+  put_fa(o, d.elo, 3); o += ", "; put_f(o, d.bend); o += ", ";
+  put_f(o, d.round); o += ", "; put_f(o, d.onion); o += ", "; put_f(o, d.bc_r); o += ", "; put_f(o, d.k); o += ", ";
+  put_fa(o, d.bc, 3); o += ", 0u}";
+}
+
+bool has_csg(const Blob& b) { return b.header.csg != nullptr; }
+
+}  // namespace
+
+bool generated_needs_tables(const Blob& b) {
+  for (const DSeg& s : b.segs)
+    if (s.kind == 1) return true;
+  return false;
+}
+
+bool generated_prefers_sm(const Blob& b) { return b.objs.size() > 24 || b.header.grid != nullptr; }
+
+std::string emit_scene_source(const Blob& b, uint32_t maxb, bool sm) {
+  const char* csg = has_csg(b) ? "true" : "false";
+  std::string o;
+  o.reserve(4096 + b.objs.size() * 900);
+  o += "// generated: world fold of one compiled scene (objects: " + std::to_string(b.objs.size()) +
+       ", segments: " + std::to_string(b.segs.size()) + ")\n";
+  o += "#ifndef __HIPCC_RTC__\n#include <hip/hip_runtime.h>\n#endif\n#include \"dev_render.h\"\nnamespace mm3e {\n";
+  o += "struct GenMap {\n  const DevScene* sc;\n  Tables tb;\n  MM3E_DEV Field operator()(V3 p) const;\n};\n";
+  // The fold is the loop of world_eval with the per-segment body specialised: a uniform switch
+  // on the segment index holds each One / linear-run segment's code once; tree runs share the
+  // table walk in the default case.  The stepping (every segment, or the set bits of the
+  // wave's grid masks) is world_eval's.
+  o += "MM3E_DEV Field GenMap::operator()(V3 p) const {\n  const DevScene& S = *sc;\n";
+  o += "  Field acc{__builtin_inff(), 0u};\n  bool seeded = false;\n";
+  const size_t n = b.segs.size();
+  if (n == 0) {
+    o += "  return acc;\n}\n";
+  } else {
+    o += "  const uint32_t nseg = " + std::to_string(n) + "u;\n";
+    // The empty asm makes the point look modified on every iteration: otherwise the compiler
+    // hoists every segment's point-dependent work (p - centre, ...) out of the loop and keeps
+    // hundreds of values live across it.
+    o += "  GridSel g;\n  g.on = false;\n  uint32_t w = 0, bits = 0, si = 0;\n  for (;;) {\n"
+         "    V3 q = p;\n    asm volatile(\"\" : \"+v\"(q.x), \"+v\"(q.y), \"+v\"(q.z));\n    switch (si) {\n";
+    for (size_t si = 0; si < n; si++) {
+      const DSeg& g = b.segs[si];
+      if (g.kind == 1) continue;
+      o += "      case " + std::to_string(si) + "u: {\n";
+      if (g.kind == 0) {
+        o += "        constexpr DObj o = ";
+        put_obj(o, b.objs[g.index]);
+        o += ";\n        seg_one<";
+        o += csg;
+        o += ">(S, o, " + std::to_string(g.combine) + "u, ";
+        put_f(o, g.k);
+        o += ", q, acc, seeded);\n";
+      } else {
+        o += "        float bd;\n        uint32_t bm;\n        bool any;\n        run_begin(acc, seeded, bd, bm, any);\n";
+        for (uint32_t u = 0; u < g.unb_count; u++) {
+          o += "        {\n          constexpr DObj o = ";
+          put_obj(o, b.objs[b.unb[g.unb_base + u]]);
+          o += ";\n          run_member<";
+          o += csg;
+          o += ">(S, o, q, bd, bm, any);\n        }\n";
+        }
+        o += "        run_end(acc, seeded, bd, bm);\n";
+      }
+      o += "        break;\n      }\n";
+    }
+    o += "      default:\n        seg_step<";
+    o += csg;
+    o += ">(S, tb, q, si, acc, seeded);\n    }\n";
+    if (b.header.grid != nullptr) {
+      o += "    if (si == 0) {\n      g = grid_select(S, q, acc);\n      if (g.on) bits = grid_word(g, 0) & ~1u;\n    }\n"
+           "    if (g.on) {\n      while (bits == 0) {\n        if (++w >= g.W) return acc;\n"
+           "        bits = grid_word(g, w);\n      }\n      si = w * 32u + (uint32_t)__builtin_ctz(bits);\n"
+           "      bits &= bits - 1u;\n      continue;\n    }\n";
+    }
+    o += "    if (++si >= nseg) return acc;\n  }\n}\n";
+  }
+  o += "}  // namespace mm3e\n";
+  o += "using namespace mm3e;\n";
+  o += "extern \"C\" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY\n"
+       "    mm3e_gen_render(const DevScene* __restrict__ sc, FrameParams fp, float* __restrict__ hdr, uint32_t tw,\n"
+       "                    uint32_t th, uint32_t stage_bytes) {\n"
+       "  __shared__ float smem[256 * 3];\n"
+       "  extern __shared__ uint4 tables_lds[];\n"
+       "  const Tables tb = stage_bytes ? stage_tables(sc, tables_lds, stage_bytes) : global_tables(sc);\n";
+  o += "  render_tile<GenMap, " + std::to_string(maxb) + ", " + (sm ? "true" : "false") +
+       ">(sc, tb, fp, hdr, tw, th, blockIdx.x, blockIdx.y, smem);\n}\n";
+  return o;
+}
+
+}  // namespace mm3e

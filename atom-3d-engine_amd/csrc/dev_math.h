@@ -329,6 +329,48 @@ MM3E_DEV Field contrib(const DevScene& sc, const DObj& o, V3 p) {
 #define MM3E_MAP_ATTR MM3E_DEV
 #endif
 
+// Segment::One (accel.rs:209-235): fold object o with its combine into the accumulator.
+template <bool CSG>
+MM3E_DEV void seg_one(const DevScene& sc, const DObj& o, uint32_t combine, float k, V3 p, Field& acc, bool& seeded) {
+  if (seeded && (o.flags & OBJ_BOUNDED)) {
+    float thr = combine == 2 ? -acc.dist : (combine == 1 && k > 0.0f ? acc.dist + k : acc.dist);
+    if (far_noop(o, p, rmax(thr, sc.slack))) return;
+  }
+  Field f = contrib<CSG>(sc, o, p);
+  if (!seeded) {
+    acc = f;
+    seeded = true;
+    return;
+  }
+  if (combine == 0) acc = f_union(acc, f);
+  else if (combine == 1) acc = f_smooth_union(acc, f, k);
+  else acc = f_subtract(acc, f);
+}
+
+// A union run whose members all have an exact SDF inside a true bounding sphere
+// (SDF >= |p - c| - r): the tree walk can only skip objects that would not have won, so the
+// run's result is the plain index-order minimum (strict < keeps the earlier object, and the
+// seeded accumulator wins ties, as in accel.rs:236-247).
+MM3E_DEV void run_begin(const Field& acc, bool seeded, float& bd, uint32_t& bm, bool& any) {
+  bd = seeded ? acc.dist : __builtin_inff();
+  bm = seeded ? acc.mat : 0u;
+  any = seeded;
+}
+template <bool CSG>
+MM3E_DEV void run_member(const DevScene& sc, const DObj& o, V3 p, float& bd, uint32_t& bm, bool& any) {
+  if (any && (o.flags & OBJ_BOUNDED) && far_noop(o, p, rmax(bd, sc.slack))) return;
+  Field f = contrib<CSG>(sc, o, p);
+  if (!any || f.dist < bd) {
+    bd = f.dist;
+    bm = f.mat;
+    any = true;
+  }
+}
+MM3E_DEV void run_end(Field& acc, bool& seeded, float bd, uint32_t bm) {
+  acc = Field{bd, bm};
+  seeded = true;
+}
+
 // One segment of WorldPlan::eval (accel.rs:209-252) folded into `acc`.  Runs of plain unions
 // walk the same bounding-sphere tree in the same near-first order with the same `lower > best`
 // cut and earliest-index tie break as Bvh::query (accel.rs:79-119); instead of a 64-entry
@@ -339,38 +381,16 @@ MM3E_DEV void seg_step(const DevScene& sc, const Tables& tb, V3 p, uint32_t si, 
   {
     const DSeg& seg = tb.seg[si];
     if (seg.kind == 0) {
-      const DObj& o = tb.obj[seg.index];
-      if (seeded && (o.flags & OBJ_BOUNDED)) {
-        float thr = seg.combine == 2 ? -acc.dist : (seg.combine == 1 && seg.k > 0.0f ? acc.dist + seg.k : acc.dist);
-        if (far_noop(o, p, rmax(thr, sc.slack))) return;
-      }
-      Field f = contrib<CSG>(sc, o, p);
-      if (!seeded) {
-        acc = f;
-        seeded = true;
-        return;
-      }
-      if (seg.combine == 0) acc = f_union(acc, f);
-      else if (seg.combine == 1) acc = f_smooth_union(acc, f, seg.k);
-      else acc = f_subtract(acc, f);
+      seg_one<CSG>(sc, tb.obj[seg.index], seg.combine, seg.k, p, acc, seeded);
       return;
     }
     if (seg.kind == 2) {
-      // Every member has an exact SDF inside a true bounding sphere (SDF >= |p - c| - r), so
-      // the tree walk can only skip objects that would not have won: the run's result is the
-      // plain index-order minimum (strict < keeps the earlier object, and the seeded
-      // accumulator wins ties, as in accel.rs:236-247).
-      float bd = seeded ? acc.dist : __builtin_inff();
-      uint32_t bm = seeded ? acc.mat : 0u;
-      bool any = seeded;
-      for (uint32_t u = 0; u < seg.unb_count; u++) {
-        const DObj& o = tb.obj[tb.unb[seg.unb_base + u]];
-        if (any && (o.flags & OBJ_BOUNDED) && far_noop(o, p, rmax(bd, sc.slack))) continue;
-        Field f = contrib<CSG>(sc, o, p);
-        if (!any || f.dist < bd) { bd = f.dist; bm = f.mat; any = true; }
-      }
-      acc = Field{bd, bm};
-      seeded = true;
+      float bd;
+      uint32_t bm;
+      bool any;
+      run_begin(acc, seeded, bd, bm, any);
+      for (uint32_t u = 0; u < seg.unb_count; u++) run_member<CSG>(sc, tb.obj[tb.unb[seg.unb_base + u]], p, bd, bm, any);
+      run_end(acc, seeded, bd, bm);
       return;
     }
     float bd;
@@ -460,6 +480,51 @@ MM3E_DEV uint32_t grid_cell(const DevScene& sc, V3 p) {
   return (iz * ny + iy) * nx + ix;
 }
 
+// The wave's segment masks once the first segment has seeded `acc` (see world_eval): `on` is
+// false (fold every segment) unless every active lane's threshold is inside a grid level and
+// the lanes' points fall in at most four distinct cells.  All fields are wave-uniform.
+struct GridSel {
+  bool on;
+  uint32_t W;
+  const uint32_t *m0, *m1, *m2, *m3;
+};
+MM3E_DEV GridSel grid_select(const DevScene& sc, V3 p, const Field& acc) {
+  GridSel g;
+  g.on = false;
+  const float thr = rmax(acc.dist + sc.grid_kmax, sc.slack) * 1.0001f + sc.grid_eps;
+  const bool near0 = thr < sc.grid_m[0];
+  const bool near1 = thr < sc.grid_m[1];
+  if (__builtin_amdgcn_ballot_w64(!near1) != 0) return g;
+  const uint32_t level = __builtin_amdgcn_ballot_w64(!near0) != 0 ? 1u : 0u;
+  const uint32_t cell = grid_cell(sc, p);
+  uint64_t rem = __builtin_amdgcn_ballot_w64(true);
+  const uint32_t c0 = __builtin_amdgcn_readlane(cell, (uint32_t)__builtin_ctzll(rem));
+  rem &= ~__builtin_amdgcn_ballot_w64(cell == c0);
+  uint32_t c1 = c0, c2 = c0, c3 = c0;
+  if (rem) {
+    c1 = __builtin_amdgcn_readlane(cell, (uint32_t)__builtin_ctzll(rem));
+    rem &= ~__builtin_amdgcn_ballot_w64(cell == c1);
+  }
+  if (rem) {
+    c2 = __builtin_amdgcn_readlane(cell, (uint32_t)__builtin_ctzll(rem));
+    rem &= ~__builtin_amdgcn_ballot_w64(cell == c2);
+  }
+  if (rem) {
+    c3 = __builtin_amdgcn_readlane(cell, (uint32_t)__builtin_ctzll(rem));
+    rem &= ~__builtin_amdgcn_ballot_w64(cell == c3);
+  }
+  if (rem != 0) return g;
+  const uint32_t* __restrict__ grid = sc.grid;
+  g.on = true;
+  g.W = sc.grid_words;
+  g.m0 = grid + ((uint64_t)c0 * 2 + level) * g.W;
+  g.m1 = grid + ((uint64_t)c1 * 2 + level) * g.W;
+  g.m2 = grid + ((uint64_t)c2 * 2 + level) * g.W;
+  g.m3 = grid + ((uint64_t)c3 * 2 + level) * g.W;
+  return g;
+}
+MM3E_DEV uint32_t grid_word(const GridSel& g, uint32_t w) { return g.m0[w] | g.m1[w] | g.m2[w] | g.m3[w]; }
+
 // accel.rs:209-252 WorldPlan::eval — the reference `world()` field: the ordered fold of
 // seg_step over every segment.  With a segment grid, after the first segment seeds the
 // accumulator each lane checks that max(acc + kmax, slack) (plus rounding margins) is below a
@@ -479,55 +544,23 @@ MM3E_MAP_ATTR Field world_eval(const DevScene& sc, const Tables& tb, V3 p) {
     return acc;
   }
   if (nseg == 0) return acc;
-  const uint32_t* __restrict__ grid = sc.grid;
   // One call site for seg_step (it carries the tree walk and is inlined): the loop either
   // steps si through every segment or, once the wave switches to the grid, through the set
   // bits of the OR of its cells' masks (word w, remaining bits `bits`).
-  bool masked = false;
-  uint32_t W = 0, w = 0, bits = 0;
-  const uint32_t *m0 = nullptr, *m1 = nullptr, *m2 = nullptr, *m3 = nullptr;
+  GridSel g;
+  g.on = false;
+  uint32_t w = 0, bits = 0;
   uint32_t si = 0;
   for (;;) {
     seg_step<CSG>(sc, tb, p, si, acc, seeded);
-    if (si == 0 && grid != nullptr) {
-      const float thr = rmax(acc.dist + sc.grid_kmax, sc.slack) * 1.0001f + sc.grid_eps;
-      const bool near0 = thr < sc.grid_m[0];
-      const bool near1 = thr < sc.grid_m[1];
-      if (__builtin_amdgcn_ballot_w64(!near1) == 0) {
-        const uint32_t level = __builtin_amdgcn_ballot_w64(!near0) != 0 ? 1u : 0u;
-        const uint32_t cell = grid_cell(sc, p);
-        uint64_t rem = __builtin_amdgcn_ballot_w64(true);
-        const uint32_t c0 = __builtin_amdgcn_readlane(cell, (uint32_t)__builtin_ctzll(rem));
-        rem &= ~__builtin_amdgcn_ballot_w64(cell == c0);
-        uint32_t c1 = c0, c2 = c0, c3 = c0;
-        if (rem) {
-          c1 = __builtin_amdgcn_readlane(cell, (uint32_t)__builtin_ctzll(rem));
-          rem &= ~__builtin_amdgcn_ballot_w64(cell == c1);
-        }
-        if (rem) {
-          c2 = __builtin_amdgcn_readlane(cell, (uint32_t)__builtin_ctzll(rem));
-          rem &= ~__builtin_amdgcn_ballot_w64(cell == c2);
-        }
-        if (rem) {
-          c3 = __builtin_amdgcn_readlane(cell, (uint32_t)__builtin_ctzll(rem));
-          rem &= ~__builtin_amdgcn_ballot_w64(cell == c3);
-        }
-        if (rem == 0) {
-          masked = true;
-          W = sc.grid_words;
-          m0 = grid + ((uint64_t)c0 * 2 + level) * W;
-          m1 = grid + ((uint64_t)c1 * 2 + level) * W;
-          m2 = grid + ((uint64_t)c2 * 2 + level) * W;
-          m3 = grid + ((uint64_t)c3 * 2 + level) * W;
-          w = 0;
-          bits = (m0[0] | m1[0] | m2[0] | m3[0]) & ~1u;
-        }
-      }
+    if (si == 0 && sc.grid != nullptr) {
+      g = grid_select(sc, p, acc);
+      if (g.on) bits = grid_word(g, 0) & ~1u;
     }
-    if (masked) {
+    if (g.on) {
       while (bits == 0) {
-        if (++w >= W) return acc;
-        bits = m0[w] | m1[w] | m2[w] | m3[w];
+        if (++w >= g.W) return acc;
+        bits = grid_word(g, w);
       }
       si = w * 32u + (uint32_t)__builtin_ctz(bits);
       bits &= bits - 1u;

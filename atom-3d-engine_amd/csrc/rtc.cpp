@@ -1,0 +1,101 @@
+// rtc.cpp — hipRTC compile + module cache for generated scene kernels.
+#include "rtc.h"
+
+#include <hip/hiprtc.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "codegen.h"
+
+namespace mm3e {
+
+namespace {
+
+struct Entry {
+  std::mutex mu;       // held while compiling, so concurrent requests for one source wait
+  bool done = false;
+  bool ok = false;
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+  std::string err;
+};
+
+std::mutex g_mu;
+std::map<std::pair<int, std::string>, std::shared_ptr<Entry>> g_cache;
+
+bool compile(const std::string& src, std::vector<char>& code, std::string& err) {
+  std::vector<const char*> hs, hn;
+  for (int i = 0; i < kEmbeddedHeaderCount; i++) {
+    hs.push_back(kEmbeddedHeaders[i].text);
+    hn.push_back(kEmbeddedHeaders[i].name);
+  }
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "mm3e_gen.hip", (int)hs.size(), hs.data(), hn.data()) !=
+      HIPRTC_SUCCESS) {
+    err = "hiprtcCreateProgram failed";
+    return false;
+  }
+  const char* opts[] = {"--offload-arch=gfx950",
+                        "-O3",
+                        "-std=c++17",
+                        "-ffp-contract=off",
+                        "-fhip-fp32-correctly-rounded-divide-sqrt",
+                        "-Xclang",
+                        "-target-feature",
+                        "-Xclang",
+                        "-packed-fp32-ops"};
+  const hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
+  if (r != HIPRTC_SUCCESS) {
+    size_t n = 0;
+    hiprtcGetProgramLogSize(prog, &n);
+    std::string log(n, '\0');
+    if (n) hiprtcGetProgramLog(prog, &log[0]);
+    err = std::string("hipRTC compile of the generated scene kernel failed: ") + hiprtcGetErrorString(r) + "\n" +
+          log.substr(0, 4000);
+    hiprtcDestroyProgram(&prog);
+    return false;
+  }
+  size_t n = 0;
+  hiprtcGetCodeSize(prog, &n);
+  code.resize(n);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  return true;
+}
+
+}  // namespace
+
+bool rtc_kernel(const std::string& src, int device, hipFunction_t* fn, std::string& err) {
+  std::shared_ptr<Entry> e;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto& slot = g_cache[{device, src}];
+    if (!slot) slot = std::make_shared<Entry>();
+    e = slot;
+  }
+  std::lock_guard<std::mutex> lk(e->mu);
+  if (!e->done) {
+    std::vector<char> code;
+    e->ok = compile(src, code, e->err);
+    if (e->ok && hipModuleLoadData(&e->mod, code.data()) != hipSuccess) {
+      e->ok = false;
+      e->err = "hipModuleLoadData of the generated scene kernel failed";
+    }
+    if (e->ok && hipModuleGetFunction(&e->fn, e->mod, "mm3e_gen_render") != hipSuccess) {
+      e->ok = false;
+      e->err = "generated module has no mm3e_gen_render";
+    }
+    e->done = true;
+  }
+  if (!e->ok) {
+    err = e->err;
+    return false;
+  }
+  *fn = e->fn;
+  return true;
+}
+
+}  // namespace mm3e

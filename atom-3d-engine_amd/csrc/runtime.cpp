@@ -16,7 +16,9 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "codegen.h"
 #include "launch.h"
+#include "rtc.h"
 
 namespace mm3e {
 
@@ -160,6 +162,60 @@ bool upload_tables(mm3e_scene* s, const std::vector<mm3e_ir_object>& xo, const s
   return true;
 }
 
+// ---- scene code generation ----------------------------------------------------------------
+static void gen_run(mm3e::GenJob& j) {
+  j.state.store(MM3E_CODEGEN_QUEUED);
+  hipSetDevice(j.device);
+  hipFunction_t fn = nullptr;
+  std::string err;
+  const bool ok = rtc_kernel(j.src, j.device, &fn, err);
+  {
+    std::lock_guard<std::mutex> lk(j.mu);
+    j.fn = fn;
+    j.err = err;
+    j.state.store(ok ? MM3E_CODEGEN_READY : MM3E_CODEGEN_FAILED, std::memory_order_release);
+  }
+  j.cv.notify_all();
+}
+
+static void gen_worker(mm3e_renderer* r) {
+  for (;;) {
+    std::shared_ptr<GenJob> j;
+    {
+      std::unique_lock<std::mutex> lk(r->gen_mu);
+      r->gen_cv.wait(lk, [&] { return r->gen_stop || !r->gen_queue.empty(); });
+      if (r->gen_stop) return;
+      j = r->gen_queue.front();
+      r->gen_queue.pop_front();
+    }
+    gen_run(*j);
+  }
+}
+
+// Emit the scene's generated kernel and compile it (SYNC) or queue it (BACKGROUND).
+static void gen_schedule(mm3e_scene* s) {
+  mm3e_renderer* r = s->r;
+  Blob b;
+  build_blob(s->host, {}, {}, s->aa, s->bounces, (uint64_t)(uintptr_t)s->d_tables, s->d_gi, false, b);
+  auto j = std::make_shared<GenJob>();
+  j->src = emit_scene_source(b, s->bounces, generated_prefers_sm(b));
+  j->device = r->device;
+  s->gen_stage = generated_needs_tables(b) ? s->stage_bytes : 0u;
+  s->gen = j;
+  if (r->codegen == MM3E_CODEGEN_SYNC) {
+    gen_run(*j);
+    return;
+  }
+  std::lock_guard<std::mutex> lk(r->gen_mu);
+  if (r->gen_threads.empty()) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const unsigned n = std::max(1u, std::min(8u, hw / 2));
+    for (unsigned i = 0; i < n; i++) r->gen_threads.emplace_back(gen_worker, r);
+  }
+  r->gen_queue.push_back(j);
+  r->gen_cv.notify_one();
+}
+
 bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std::string& msg) {
   hipStream_t st = s->r->stream;
   const uint32_t stage = dsc == s->d_scene ? s->stage_bytes : s->dyn_stage_bytes;
@@ -168,9 +224,20 @@ bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std
   dim3 block(s->tw * s->th * s->aa * s->aa);
   hipEvent_t* ev = s->ev[s->frames % mm3e_scene::kRing];
   hipEventRecord(ev[0], st);
-  if (!hip_ok(launch_render_table(s->bounces, grid, block, st, dsc, stage, general, fp, s->d_hdr, s->tw, s->th),
-              "render launch", &msg))
+  const mm3e::GenJob* gj = s->gen.get();
+  if (dsc == s->d_scene && gj && gj->state.load(std::memory_order_acquire) == MM3E_CODEGEN_READY) {
+    const DevScene* sc_arg = dsc;
+    FrameParams fp_arg = fp;
+    float* hdr_arg = s->d_hdr;
+    uint32_t tw = s->tw, th = s->th, gstage = s->gen_stage;
+    void* args[] = {&sc_arg, &fp_arg, &hdr_arg, &tw, &th, &gstage};
+    if (!hip_ok(hipModuleLaunchKernel(gj->fn, grid.x, grid.y, 1, block.x, 1, 1, gstage, st, args, nullptr),
+                "generated render launch", &msg))
+      return false;
+  } else if (!hip_ok(launch_render_table(s->bounces, grid, block, st, dsc, stage, general, fp, s->d_hdr, s->tw, s->th),
+                     "render launch", &msg)) {
     return false;
+  }
   hipEventRecord(ev[1], st);
   if (!hip_ok(launch_post(st, s->pp, s->d_kern, s->d_hdr, s->d_tmp, s->d_rgba), "post launch", &msg)) return false;
   hipEventRecord(ev[2], st);
@@ -233,6 +300,12 @@ int mm3e_renderer_create(const char* adapter_filter, mm3e_renderer** out, char* 
   }
   auto* r = new mm3e_renderer();
   r->device = dev;
+  if (const char* cg = std::getenv("MM3E_CODEGEN")) {
+    const std::string m = lower(cg);
+    if (m == "off" || m == "0") r->codegen = MM3E_CODEGEN_OFF;
+    else if (m == "sync" || m == "2") r->codegen = MM3E_CODEGEN_SYNC;
+    else r->codegen = MM3E_CODEGEN_BACKGROUND;
+  }
   r->name = describe(dev);
   // Tables staged in LDS share the block's LDS with the 3 KiB sample-reduction buffer; keep
   // the copy to 56 KiB so three 256-lane blocks still fit one CU.
@@ -258,6 +331,12 @@ int mm3e_renderer_wait_idle(mm3e_renderer* r) {
 
 void mm3e_renderer_destroy(mm3e_renderer* r) {
   if (!r) return;
+  {
+    std::lock_guard<std::mutex> lk(r->gen_mu);
+    r->gen_stop = true;
+  }
+  r->gen_cv.notify_all();
+  for (auto& t : r->gen_threads) t.join();  // a compile in progress finishes first
   hipSetDevice(r->device);
   hipStreamSynchronize(r->stream);
   hipStreamDestroy(r->stream);
@@ -350,6 +429,7 @@ int mm3e_scene_compile(mm3e_renderer* r, const void* ir, size_t ir_len, uint32_t
     mm3e_scene_destroy(s);
     return MM3E_ERR_DEVICE;
   }
+  if (r->codegen != MM3E_CODEGEN_OFF) gen_schedule(s);
   *out = s;
   return MM3E_OK;
 }
@@ -494,6 +574,10 @@ int mm3e_scene_info(const mm3e_scene* s, uint32_t* n_objects, uint32_t* n_segmen
 
 int mm3e_scene_kernel_name(const mm3e_scene* s, char* buf, size_t buflen) {
   if (!s || !buf || buflen == 0) return MM3E_ERR_INVALID;
+  if (s->gen && s->gen->state.load(std::memory_order_acquire) == MM3E_CODEGEN_READY) {
+    std::snprintf(buf, buflen, "mm3e_gen_render");
+    return MM3E_OK;
+  }
   const char* base = s->general ? (s->stage_bytes ? "mm3e_render_lds_x_b" : "mm3e_render_table_x_b")
                                  : (s->stage_bytes ? "mm3e_render_lds_b" : "mm3e_render_table_b");
   std::snprintf(buf, buflen, "%s%u", base, s->bounces);
@@ -733,6 +817,48 @@ int mm3e_write_bmp(const uint8_t* rgba, uint32_t w, uint32_t h, uint8_t* out, si
     }
     for (size_t i = 0; i < pad; i++) *q++ = 0;
   }
+  return MM3E_OK;
+}
+
+int mm3e_renderer_set_codegen(mm3e_renderer* r, int mode) {
+  if (!r || mode < MM3E_CODEGEN_OFF || mode > MM3E_CODEGEN_SYNC) return MM3E_ERR_INVALID;
+  r->codegen = mode;
+  return MM3E_OK;
+}
+
+int mm3e_scene_codegen_status(mm3e_scene* s, int wait, int* state, char* err, size_t errlen) {
+  if (!s || !state) return MM3E_ERR_INVALID;
+  mm3e::GenJob* j = s->gen.get();
+  if (!j) {
+    *state = MM3E_CODEGEN_NONE;
+    return MM3E_OK;
+  }
+  std::unique_lock<std::mutex> lk(j->mu);
+  if (wait)
+    j->cv.wait(lk, [&] {
+      const int st = j->state.load();
+      return st == MM3E_CODEGEN_READY || st == MM3E_CODEGEN_FAILED;
+    });
+  *state = j->state.load();
+  if (*state == MM3E_CODEGEN_FAILED) set_err(err, errlen, j->err);
+  return MM3E_OK;
+}
+
+int mm3e_scene_emit_source(const void* ir, size_t ir_len, char* src, size_t* src_len) {
+  if (!src_len) return MM3E_ERR_INVALID;
+  HostScene hs;
+  std::string msg;
+  if (!parse_scene_ir(ir, ir_len, hs, msg)) return MM3E_ERR_INVALID;
+  Blob b;
+  build_blob(hs, {}, {}, hs.hdr.aa == 0 ? 1u : hs.hdr.aa, hs.hdr.bounces, 0, nullptr, false, b);
+  const std::string text = emit_scene_source(b, hs.hdr.bounces, generated_prefers_sm(b));
+  const size_t need = text.size() + 1;
+  if (!src || *src_len < need) {
+    *src_len = need;
+    return MM3E_ERR_INVALID;
+  }
+  std::memcpy(src, text.c_str(), need);
+  *src_len = need;
   return MM3E_OK;
 }
 

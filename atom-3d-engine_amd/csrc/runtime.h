@@ -2,13 +2,31 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <condition_variable>
+#include <deque>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mm3e_hip.h"
 #include "dev_render.h"
 #include "scene_compile.h"
+
+namespace mm3e {
+// One scene's generated kernel: compiled by the renderer's code-generation workers.
+struct GenJob {
+  std::string src;
+  int device = 0;
+  std::atomic<int> state{MM3E_CODEGEN_QUEUED};
+  hipFunction_t fn = nullptr;
+  std::string err;
+  std::mutex mu;
+  std::condition_variable cv;
+};
+}  // namespace mm3e
 
 struct mm3e_renderer {
   int device = 0;
@@ -16,6 +34,13 @@ struct mm3e_renderer {
   std::string name;
   hipStream_t stream = nullptr;  // default priority
   std::mutex mu;
+  // scene code generation: mode, queue and worker threads (started on first use)
+  int codegen = MM3E_CODEGEN_BACKGROUND;
+  std::mutex gen_mu;
+  std::condition_variable gen_cv;
+  std::deque<std::shared_ptr<mm3e::GenJob>> gen_queue;
+  std::vector<std::thread> gen_threads;
+  bool gen_stop = false;
 };
 
 struct mm3e_scene {
@@ -47,6 +72,8 @@ struct mm3e_scene {
   uint32_t dyn_stage_bytes = 0;        // the same for the tables with dynamic spheres appended
   bool general = false;                // the scene needs the general kernels (segment grid / CSG)
   bool dyn_general = false;            // the same for the tables with dynamic spheres appended
+  std::shared_ptr<mm3e::GenJob> gen;   // generated kernel (null when code generation is off)
+  uint32_t gen_stage = 0;              // LDS bytes the generated kernel stages (0: none needed)
 };
 
 namespace mm3e {

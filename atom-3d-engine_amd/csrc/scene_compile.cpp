@@ -870,6 +870,9 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
   if (o_csg) put(out.bytes, o_csg, csg_ops.data(), csg_ops.size());
   out.gi_offset = o_gi;
   out.gi_bytes = gi_bytes;
+  out.objs = dobj;
+  out.segs = segs;
+  out.unb = unb;
 
   DevScene& d = out.header;
   std::memset(&d, 0, sizeof d);

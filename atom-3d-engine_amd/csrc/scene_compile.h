@@ -58,6 +58,10 @@ struct Blob {
   DevScene header;
   uint64_t gi_offset = 0;   // offset of the GI array in `bytes` (0 if none in blob)
   uint64_t gi_bytes = 0;
+  // host copies of the world-plan tables (what the code generator folds into constants)
+  std::vector<DObj> objs;
+  std::vector<DSeg> segs;
+  std::vector<uint32_t> unb;
 };
 void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_objects,
                 const std::vector<mm3e_ir_material>& extra_materials, uint32_t aa, uint32_t bounces,

@@ -27,8 +27,11 @@ EXPORTS = (
     "mm3e_batch_destroy", "mm3e_render_batch", "mm3e_server_default_config", "mm3e_server_create",
     "mm3e_server_add_scene", "mm3e_server_submit", "mm3e_server_poll", "mm3e_server_wait", "mm3e_server_stats",
     "mm3e_server_destroy", "mm3e_scene_kernel_name",
-    "mm3e_scene_info",
+    "mm3e_scene_info", "mm3e_renderer_set_codegen", "mm3e_scene_codegen_status", "mm3e_scene_emit_source",
 )
+
+CODEGEN_OFF, CODEGEN_BACKGROUND, CODEGEN_SYNC = 0, 1, 2
+CODEGEN_NONE, CODEGEN_QUEUED, CODEGEN_READY, CODEGEN_FAILED = 0, 1, 2, 3
 
 _lib = None
 
@@ -63,6 +66,9 @@ def lib():
         L.mm3e_scene_timing.argtypes = [P, C.c_int, P, P, P]
         L.mm3e_scene_kernel_name.argtypes = [P, C.c_char_p, C.c_size_t]
         L.mm3e_scene_info.argtypes = [P, P, P, P, P, P]
+        L.mm3e_renderer_set_codegen.argtypes = [P, C.c_int]
+        L.mm3e_scene_codegen_status.argtypes = [P, C.c_int, P, C.c_char_p, C.c_size_t]
+        L.mm3e_scene_emit_source.argtypes = [P, C.c_size_t, C.c_char_p, C.POINTER(C.c_size_t)]
         L.mm3e_scene_field.argtypes = [P, P, C.c_uint32, P, P]
         L.mm3e_scene_march.argtypes = [P, P, C.c_uint32, P, P, P, P, P]
         L.mm3e_camera_look_at.argtypes = [P, P, P, C.c_float, P]
@@ -137,6 +143,17 @@ def scene_check(ir: bytes, width: int, height: int):
     return None if rc == 0 else err.value.decode()
 
 
+def emit_source(ir: bytes) -> str:
+    """wgsl::build_shader_checked twin: the generated HIP source of a scene IR."""
+    n = C.c_size_t(0)
+    lib().mm3e_scene_emit_source(ir, len(ir), None, C.byref(n))
+    buf = C.create_string_buffer(n.value)
+    rc = lib().mm3e_scene_emit_source(ir, len(ir), buf, C.byref(n))
+    if rc != 0:
+        raise Mm3eError(rc, "scene IR rejected")
+    return buf.value.decode()
+
+
 def list_adapters() -> str:
     buf = C.create_string_buffer(4096)
     lib().mm3e_list_adapters(buf, 4096)
@@ -162,6 +179,12 @@ class Renderer:
         rc = lib().mm3e_renderer_wait_idle(self.h)
         if rc != 0:
             raise Mm3eError(rc, "wait_idle failed")
+
+    def set_codegen(self, mode: int):
+        """CODEGEN_OFF / CODEGEN_BACKGROUND / CODEGEN_SYNC for scenes compiled afterwards."""
+        rc = lib().mm3e_renderer_set_codegen(self.h, mode)
+        if rc != 0:
+            raise Mm3eError(rc, "bad codegen mode")
 
     def compile(self, ir: bytes, width: int, height: int) -> "GpuScene":
         return GpuScene(self, ir, width, height)
@@ -245,6 +268,15 @@ class GpuScene:
         if rc != 0:
             raise Mm3eError(rc, "no frame timed yet")
         return a.value, b.value
+
+    def codegen_status(self, wait=False):
+        """(state, message): CODEGEN_NONE / QUEUED / READY / FAILED of the scene's generated kernel."""
+        st = C.c_int(0)
+        err = C.create_string_buffer(4096)
+        rc = lib().mm3e_scene_codegen_status(self.h, 1 if wait else 0, C.byref(st), err, 4096)
+        if rc != 0:
+            raise Mm3eError(rc, "codegen_status failed")
+        return st.value, err.value.decode()
 
     def info(self):
         v = [C.c_uint32() for _ in range(5)]

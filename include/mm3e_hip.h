@@ -132,6 +132,27 @@ void mm3e_orbit_camera(const float target[3], float radius, float yaw, float pit
  * only *out_len is set to the required size. */
 int mm3e_write_bmp(const uint8_t* rgba, uint32_t w, uint32_t h, uint8_t* out, size_t* out_len);
 
+/* ---- scene code generation ------------------------------------------------------------
+ * The twin of the reference's WGSL code generator (mm3e-gpu/src/wgsl.rs:34-362): each compiled
+ * scene's world fold is emitted as HIP source with the scene's constants folded in and compiled
+ * at run time (hipRTC) into a render kernel for that scene.  The generated kernel performs the
+ * same floating-point operations as the table-driven kernels, so images are identical; only
+ * speed differs.  Modes: OFF (table kernels only), BACKGROUND (default: compile on worker
+ * threads, the scene renders with the table kernels until its kernel is ready), SYNC (compile
+ * inside mm3e_scene_compile).  The environment variable MM3E_CODEGEN=off|background|sync sets
+ * the default for new renderers. */
+enum { MM3E_CODEGEN_OFF = 0, MM3E_CODEGEN_BACKGROUND = 1, MM3E_CODEGEN_SYNC = 2 };
+enum { MM3E_CODEGEN_NONE = 0, MM3E_CODEGEN_QUEUED = 1, MM3E_CODEGEN_READY = 2, MM3E_CODEGEN_FAILED = 3 };
+/* Mode for scenes compiled afterwards on this renderer. */
+int mm3e_renderer_set_codegen(mm3e_renderer* r, int mode);
+/* State of a scene's generated kernel (MM3E_CODEGEN_NONE/QUEUED/READY/FAILED); wait != 0
+ * blocks until it is READY or FAILED.  On FAILED the compiler's message goes to err. */
+int mm3e_scene_codegen_status(mm3e_scene* s, int wait, int* state, char* err, size_t errlen);
+/* wgsl::build_shader_checked twin (wgsl.rs:34-102): the generated HIP source of a scene IR (as
+ * mm3e_scene_compile would build it).  With src == NULL or *src_len too small, *src_len is set
+ * to the size needed (including the terminating NUL) and MM3E_ERR_INVALID is returned. */
+int mm3e_scene_emit_source(const void* scene_ir, size_t ir_len, char* src, size_t* src_len);
+
 /* ---- batched rendering ----------------------------------------------------------------
  * Many (compiled scene, camera, size) jobs rendered by one persistent launch that pulls
  * tiles from a global atomic queue, followed by one batched bloom/tonemap over all jobs;

@@ -15,8 +15,11 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def renderer():
-    """One HIP renderer for the whole GPU session (the library has no CPU fallback)."""
+    """One HIP renderer for the whole GPU session (the library has no CPU fallback).  Its scenes
+    render with the table-driven kernels (code generation off); the generated kernels are
+    checked against them in test_codegen.py."""
     from mm3e import hip
     r = hip.Renderer()
+    r.set_codegen(hip.CODEGEN_OFF)
     yield r
     r.close()
