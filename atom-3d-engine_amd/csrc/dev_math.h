@@ -332,7 +332,7 @@ MM3E_DEV Field contrib(const DevScene& sc, const DObj& o, V3 p) {
 // Segment::One (accel.rs:209-235): fold object o with its combine into the accumulator.
 template <bool CSG>
 MM3E_DEV void seg_one(const DevScene& sc, const DObj& o, uint32_t combine, float k, V3 p, Field& acc, bool& seeded) {
-  if (seeded && (o.flags & OBJ_BOUNDED)) {
+  if (seeded && (o.flags & (OBJ_BOUNDED | OBJ_SKIP))) {
     float thr = combine == 2 ? -acc.dist : (combine == 1 && k > 0.0f ? acc.dist + k : acc.dist);
     if (far_noop(o, p, rmax(thr, sc.slack))) return;
   }
@@ -358,7 +358,7 @@ MM3E_DEV void run_begin(const Field& acc, bool seeded, float& bd, uint32_t& bm, 
 }
 template <bool CSG>
 MM3E_DEV void run_member(const DevScene& sc, const DObj& o, V3 p, float& bd, uint32_t& bm, bool& any) {
-  if (any && (o.flags & OBJ_BOUNDED) && far_noop(o, p, rmax(bd, sc.slack))) return;
+  if (any && (o.flags & (OBJ_BOUNDED | OBJ_SKIP)) && far_noop(o, p, rmax(bd, sc.slack))) return;
   Field f = contrib<CSG>(sc, o, p);
   if (!any || f.dist < bd) {
     bd = f.dist;
@@ -404,6 +404,8 @@ MM3E_DEV void seg_step(const DevScene& sc, const Tables& tb, V3 p, uint32_t si, 
     for (uint32_t u = 0; u < seg.unb_count; u++) {
       uint32_t i = tb.unb[seg.unb_base + u];
       const DObj& o = tb.obj[i];
+      // a skip sphere proves d > bd, so neither the < nor the tie test can fire
+      if ((o.flags & OBJ_SKIP) && far_noop(o, p, rmax(bd, sc.slack))) continue;
       float d = object_distance<CSG>(sc, o, p);
       if (d < bd || (d == bd && (int64_t)i < bi)) { bd = d; bi = i; bm = o.mat; }
     }

@@ -27,6 +27,10 @@ enum : uint32_t {
   OBJ_IDENT_ROT = 1u << 14,  // rotation is the identity: to_local skips the matrix product
   OBJ_UNIT_SCALE = 1u << 15, // scale == 1: to_local and Object::field skip the two multiplies
   OBJ_COMBINE_SHIFT = 16,    // bits 16-17: mm3e_combine of this object
+  // The reference leaves the object unbounded (bend: lib.rs:478-480), but bc / bc_r hold a
+  // sphere that provably lies below its field (see skip_bound in scene_compile.cpp): used only
+  // to prove that folding the object changes nothing, never as its field value.
+  OBJ_SKIP = 1u << 18,
 };
 
 // One placed object (mm3e-orchestrator/src/lib.rs:330-336) with its world bounding sphere

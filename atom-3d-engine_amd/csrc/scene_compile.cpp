@@ -439,6 +439,47 @@ Sphere world_bound(const HostScene& sc, const mm3e_ir_object& o) {  // lib.rs:47
   return Sphere{true, {wc.x, wc.y, wc.z}, radius * o.scale + std::fmax(o.round, 0.0f) + std::fabs(o.onion)};
 }
 
+// A sphere below the field of an object the reference leaves unbounded because of a bend
+// (lib.rs:478-480).  Mirror, twist and bend map the local point by reflections and rotations,
+// which keep |l|; elongation moves it by at most |h|; so for a primitive whose SDF is exact
+// (SDF(q) >= |q - c| - r for its local bound (c, r)) the object's field is at least
+// (|l| - |h| - |c| - r) * scale - max(round, 0) - |onion|, and |l| * scale = |p - pos| for a
+// rigid transform with positive uniform scale.  Repetition folds space and has no such bound.
+// The radius is computed in double and rounded up; the device's far test adds its own margin
+// for the float rounding of the object's evaluation.
+Sphere skip_bound(const mm3e_ir_object& o) {
+  const Sphere none{false, {0, 0, 0}, 0};
+  if (o.bend == 0.0f || nonzero3(o.repeat) || !(o.scale > 0.0f) || !std::isfinite(o.scale)) return none;
+  double c = 0.0, r = 0.0;
+  const float* p = o.p;
+  auto n3 = [](double x, double y, double z) { return std::sqrt(x * x + y * y + z * z); };
+  switch (o.prim) {
+    case MM3E_PRIM_SPHERE: r = p[0]; break;
+    case MM3E_PRIM_BOX: r = n3(p[0], p[1], p[2]); break;
+    case MM3E_PRIM_ROUND_BOX:  // sdf.rs:44-46: exact while the radius fits every half extent
+      if (!(p[3] >= 0.0f && p[3] <= std::min(p[0], std::min(p[1], p[2])))) return none;
+      r = n3(p[0], p[1], p[2]);
+      break;
+    case MM3E_PRIM_TORUS: r = (double)p[0] + p[1]; break;
+    case MM3E_PRIM_CYLINDER: r = n3(p[0], p[1], 0.0); break;
+    case MM3E_PRIM_CAPSULE:
+      c = n3(0.5 * ((double)p[0] + p[3]), 0.5 * ((double)p[1] + p[4]), 0.5 * ((double)p[2] + p[5]));
+      r = 0.5 * n3((double)p[0] - p[3], (double)p[1] - p[4], (double)p[2] - p[5]) + p[6];
+      break;
+    case MM3E_PRIM_CONE:
+      if (!(std::fabs(p[0] - p[1]) < p[2])) return none;
+      c = 0.5 * std::fabs((double)p[2]);
+      r = 0.5 * std::fabs((double)p[2]) + std::max((double)p[0], (double)p[1]);
+      break;
+    default: return none;
+  }
+  const double h = n3(o.elongate[0], o.elongate[1], o.elongate[2]);
+  double rad = (h + c + r) * (double)o.scale + std::max((double)o.round, 0.0) + std::fabs((double)o.onion);
+  rad = rad * (1.0 + 1e-6) + 1e-6;
+  if (!std::isfinite(rad) || !(rad > 0.0)) return none;
+  return Sphere{true, {o.pos[0], o.pos[1], o.pos[2]}, (float)rad};
+}
+
 bool gi_bounds(const HostScene& sc, float lo[3], float hi[3]) {
   F3 l{INFINITY, INFINITY, INFINITY}, h{-INFINITY, -INFINITY, -INFINITY};
   bool any = false;
@@ -654,7 +695,8 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
 
   // objects + bounds
   std::vector<DObj> dobj(n_obj);
-  std::vector<Sphere> bounds(n_obj);
+  std::vector<Sphere> bounds(n_obj);       // Object::world_bound (the reference's plan and prune)
+  std::vector<Sphere> grid_bounds(n_obj);  // the same, or a skip sphere (candidate grid only)
   float max_k = 0.0f;
   for (uint64_t i = 0; i < n_obj; i++) {
     const mm3e_ir_object& o = objs[i];
@@ -696,10 +738,19 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
     if (std::memcmp(o.rot, ident, sizeof ident) == 0) f |= OBJ_IDENT_ROT;
     if (o.scale == 1.0f) f |= OBJ_UNIT_SCALE;
     bounds[i] = world_bound(sc, o);
+    grid_bounds[i] = bounds[i];
     if (bounds[i].some) {
       f |= OBJ_BOUNDED;
       std::memcpy(d.bc, bounds[i].c, 12);
       d.bc_r = bounds[i].r;
+    } else {
+      Sphere k = skip_bound(o);
+      if (k.some) {
+        f |= OBJ_SKIP;
+        std::memcpy(d.bc, k.c, 12);
+        d.bc_r = k.r;
+        grid_bounds[i] = k;
+      }
     }
     d.flags = f;
     max_k = std::fmax(max_k, o.combine == MM3E_COMBINE_SMOOTH ? o.k : 0.0f);
@@ -723,9 +774,11 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
     if (!bounds[i].some) return true;
     if (!(o.scale > 0.0f) || o.twist != 0.0f) return false;
     switch (o.prim) {
-      case MM3E_PRIM_SPHERE: case MM3E_PRIM_BOX: case MM3E_PRIM_ROUND_BOX: case MM3E_PRIM_TORUS:
+      case MM3E_PRIM_SPHERE: case MM3E_PRIM_BOX: case MM3E_PRIM_TORUS:
       case MM3E_PRIM_CYLINDER: case MM3E_PRIM_CAPSULE:
         return true;
+      case MM3E_PRIM_ROUND_BOX:  // sdf.rs:44-46: exact while the radius fits every half extent
+        return o.p[3] >= 0.0f && o.p[3] <= std::min(o.p[0], std::min(o.p[1], o.p[2]));
       case MM3E_PRIM_CONE: return std::fabs(o.p[0] - o.p[1]) < o.p[2];
       default: return false;
     }
@@ -784,7 +837,7 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
 
   // Segment candidate grid (dynamic-sphere frames rebuild their blob per frame and skip it).
   SegGrid grid;
-  if (extra_objects.empty()) build_seg_grid(segs, seg_members, bounds, slack, grid);
+  if (extra_objects.empty()) build_seg_grid(segs, seg_members, grid_bounds, slack, grid);
 
   std::vector<DMat> dmat(mats.size());
   for (size_t i = 0; i < mats.size(); i++) {

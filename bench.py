@@ -6,7 +6,8 @@ A step is one frame of the 960x540 Cornell GI workload (seed 2: five box walls, 
 primitives incl. a smooth union and a twist, emissive bar, sphere light, 16^3 GI probes baked
 on the device, soft shadows, AO, one reflection bounce, bloom + tonemap, aa = 2) rendered by
 the plain per-frame launch path (`render_only`: render kernel + bloom/tonemap kernels) with
-the scene and framebuffers resident in HBM.  W untimed frames, then K timed frames between
+the scene and framebuffers resident in HBM; the render kernel is the scene's generated
+kernel (scene -> HIP source -> hipRTC), which the bench waits for before timing.  W untimed frames, then K timed frames between
 two device synchronisations.  One JSON line on stdout; progress on stderr.
 
 roofline: the render kernel's algorithmic FP32 work per frame (tests/golden/flops.json,
@@ -25,6 +26,8 @@ Two more legs ride on the same line (``--legs`` picks them):
            plain path's measured capacity, plus a burst; p50/p99 latency (arrival -> RGBA8 in
            host memory) and Mpix/s for the continuous-batching server and for a FIFO server
            that renders each request by a plain per-frame launch.
+  stress:  (not in the default set) one 3840x2160 frame of the 1000-primitive scene with a 128^3
+           sampled volume, 32^3 device-baked GI, 2 bounces and fog; ms/frame.
 """
 import argparse
 import json
@@ -76,8 +79,10 @@ def batched_leg(hip, configs, r, reps):
             scenes[key] = r.compile(sc.to_ir(), w, h)
         compiled.append((scenes[key], cam, (w, h)))
     px = sum(w * h for _, _, (w, h) in jobs)
-    log(f"batched: {len(jobs)} jobs, {px / 1e6:.1f} Mpix, {len(scenes)} compiled (scene, size) pairs in "
-        f"{time.perf_counter() - t0:.2f} s")
+    for g in scenes.values():
+        g.codegen_status(wait=True)  # plain launches use each scene's generated kernel
+    log(f"batched: {len(jobs)} jobs, {px / 1e6:.1f} Mpix, {len(scenes)} compiled (scene, size) pairs (with their "
+        f"generated kernels) in {time.perf_counter() - t0:.2f} s")
     # plain: one launch per frame, each waited for (render_only + wait_idle)
     for g, cam, _ in compiled[:4]:
         g.render_only(cam)
@@ -133,8 +138,10 @@ def served_leg(hip, configs, r, n_req, n_scenes=32):
     for si, w, h in need:
         plain_sc[(si, w, h)] = r.compile(scs[si].to_ir(), w, h)
     base = [r.compile(sc.to_ir(), 854, 480) for sc in scs]
+    for g in plain_sc.values():
+        g.codegen_status(wait=True)
     log(f"served: {n_scenes} resident scenes + {len(plain_sc)} plain (scene, size) pairs compiled with device GI "
-        f"bakes in {time.perf_counter() - t0:.2f} s; {n_req} requests")
+        f"bakes and generated kernels in {time.perf_counter() - t0:.2f} s; {n_req} requests")
     px_req = [w * h for _, _, (w, h) in reqs]
     bufs = {(w, h): np.zeros((h, w, 4), np.uint8) for _, _, (w, h) in reqs}
     for (si, w, h), g in list(plain_sc.items())[:3]:
@@ -207,6 +214,43 @@ def served_leg(hip, configs, r, n_req, n_scenes=32):
     return out
 
 
+def stress_leg(hip, configs, r, frames):
+    """c5: one 3840x2160 frame of a 1000-primitive scene with a 128^3 sampled volume, 32^3 GI
+    probes baked on the device, 2 reflection bounces and fog."""
+    t0 = time.perf_counter()
+    sc, cam = configs.c5_scene()
+    ir = sc.to_ir()
+    build_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g = r.compile(ir, sc.width, sc.height)
+    compile_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    st, msg = g.codegen_status(wait=True)
+    codegen_s = time.perf_counter() - t0
+    info = g.info()
+    log(f"stress: IR {len(ir) / 1e6:.1f} MB built in {build_s:.1f} s; compile (tables, grid, 32^3 GI bake) "
+        f"{compile_s:.1f} s; generated kernel state {st} after {codegen_s:.1f} s; {info}")
+    g.render_only(cam)
+    r.wait_idle()
+    g.timing(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        g.render_only(cam)
+    r.wait_idle()
+    dt = (time.perf_counter() - t0) / frames
+    n, render_ms, post_ms = g.timing(reset=True)
+    out = {"workload": "c5: 3840x2160, 1000 seeded primitives (C3 generator, twist + bend), 128^3 star-shaped "
+                       "sampled volume, 32^3 GI probes baked on device, 2 reflection bounces, fog 0.02, aa=1",
+           "width": sc.width, "height": sc.height, "frames": frames, "ms_per_frame": round(dt * 1e3, 2),
+           "mpix_s": round(sc.width * sc.height / dt / 1e6, 3), "render_kernel": g.kernel_name(),
+           "render_kernel_ms": round(render_ms, 2), "post_ms": round(post_ms, 3), "compile_s": round(compile_s, 2),
+           "codegen_s": round(codegen_s, 1), "objects": info["objects"], "segments": info["segments"],
+           "grid_cells": info["grid_cells"]}
+    log(f"stress: {out['ms_per_frame']} ms/frame ({out['mpix_s']} Mpix/s) with {out['render_kernel']}")
+    g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,7 +258,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--legs", default="plain,batched,served",
-                    help="comma list of plain (always run), batched, served")
+                    help="comma list of plain (always run), batched, served, stress")
+    ap.add_argument("--stress-frames", type=int, default=3)
     ap.add_argument("--batched-reps", type=int, default=3)
     ap.add_argument("--served-requests", type=int, default=48)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -246,6 +291,10 @@ def main():
     g = r.compile(ir, W, H)
     compile_s = time.perf_counter() - t0
     log(f"compiled c2 (tables + device GI bake) in {compile_s:.3f} s")
+    t0 = time.perf_counter()
+    cg_state, cg_msg = g.codegen_status(wait=True)
+    codegen_s = time.perf_counter() - t0
+    log(f"generated kernel: state {cg_state} after {codegen_s:.1f} s {cg_msg[:200]}; rendering with {g.kernel_name()}")
     for _ in range(args.warmup):
         g.render_only(cam)
     r.wait_idle()
@@ -277,7 +326,9 @@ def main():
     prof = os.path.join(ROOT, "profiles", "render_c2_traffic.json")
     if os.path.exists(prof):
         with open(prof) as f:
-            roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+            tr = json.load(f)
+        if tr.get("kernel") == roofline["kernel"]:  # only a profile of the kernel being timed
+            roofline["traffic"] = tr.get("hbm_bytes_per_launch")
 
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
@@ -287,7 +338,8 @@ def main():
                                "probes baked on device (64 requested -> 30 rays/probe), soft shadows, AO, "
                                "1 reflection bounce, bloom+tonemap, aa=2; plain per-frame launch",
                    "width": W, "height": H, "aa": sc.aa, "bounces": sc.bounces, "path": "plain",
-                   "compile_s": round(compile_s, 3), "render_kernel_ms": round(render_ms, 4),
+                   "compile_s": round(compile_s, 3), "codegen_s": round(codegen_s, 2),
+                   "render_kernel": g.kernel_name(), "render_kernel_ms": round(render_ms, 4),
                    "post_ms": round(post_ms, 4)},
         "roofline": roofline,
     }
@@ -304,6 +356,8 @@ def main():
         out["batched"] = batched_leg(hip, configs, r, args.batched_reps)
     if "served" in legs:
         out["served"] = served_leg(hip, configs, r, args.served_requests)
+    if "stress" in legs:
+        out["stress"] = stress_leg(hip, configs, r, args.stress_frames)
     print(json.dumps(out), flush=True)
     g.close()
     r.close()

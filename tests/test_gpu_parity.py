@@ -78,6 +78,51 @@ def test_field_parity_segment_grid(renderer, seed, n):
     assert (mg == mo).mean() >= 0.999
 
 
+def test_field_parity_bent_objects(renderer):
+    """Objects the reference leaves unbounded because of a bend (lib.rs:478-480) carry a skip
+    sphere on the device (exact-SDF primitives, optionally mirrored / elongated / twisted /
+    rounded / hollowed, in unions, smooth unions and tree-walked runs); folding them must give
+    the oracle's field everywhere, near and far."""
+    from mm3e.scene import Material, Object, Prim, Scene, Transform, from_euler, v3
+    rng = configs.SplitMix64(77)
+    sc = Scene(64, 36)
+    m = sc.material(Material.solid((0.6, 0.6, 0.6)))
+    sc.add(Object(Prim.plane((0.0, 1.0, 0.0), 0.0), Transform(), m))
+    kinds = [0, 1, 2, 3, 4, 5, 6, 7, 8]
+    for i in range(48):
+        kind = kinds[i % len(kinds)]
+        prim = configs.random_prim(rng, kind, rng.uniform(0.3, 0.8))
+        xf = Transform(pos=v3((rng.uniform(-5, 5), rng.uniform(0.3, 2.0), rng.uniform(-5, 5))),
+                       rot=from_euler(rng.uniform(-3, 3), rng.uniform(-3, 3), rng.uniform(-3, 3)),
+                       scale=rng.uniform(0.6, 1.5))
+        o = Object(prim, xf, m).bend(rng.uniform(0.2, 0.9))
+        if i % 3 == 1:
+            o.smooth(rng.uniform(0.1, 0.4))
+        if i % 5 == 2:
+            o.twist(rng.uniform(0.3, 1.0))
+        if i % 7 == 3:
+            o.mirror(True, False, True)
+        if i % 4 == 0:
+            o.round(0.05)
+        if i % 6 == 5:
+            o.onion(0.03)
+        if i % 8 == 6:
+            o.elongate((0.2, 0.1, 0.0))
+        sc.add(o)
+    g, o = both(renderer, sc)
+    rng2 = np.random.default_rng(77)
+    centres = rng2.uniform([-7, -0.3, -7], [7, 3.0, 7], (300, 3))
+    pts = np.concatenate([(centres[:, None, :] + rng2.uniform(-0.15, 0.15, (300, 64, 3))).reshape(-1, 3),
+                          rng2.uniform([-9, -1, -9], [9, 5, 9], (8000, 3))]).astype(np.float32)
+    dg, mg = g.field(pts)
+    do, mo = o.field(pts)
+    tol = 5e-5 * np.maximum(np.abs(do), 1.0)
+    bad = np.abs(dg - do) > tol
+    report("field bent", {**g.info(), "max_abs_err": float(np.abs(dg - do).max()), "bad": int(bad.sum()),
+                          "mat_agree": float((mg == mo).mean())})
+    assert bad.mean() <= 1e-3 and (mg == mo).mean() >= 0.999
+
+
 def test_march_parity(renderer):
     sc, cam = configs.c1_scene()
     g, o = both(renderer, sc)
@@ -155,6 +200,16 @@ def test_csg_objects_image_and_field_parity(renderer):
     report("field csg", {"max_abs_err": float(np.abs(dg - do).max()), "bad": int(bad.sum()),
                          "exact": float((dg == do).mean())})
     assert bad.mean() <= 1e-3 and (mg == mo).mean() >= 0.999
+
+
+def test_c5_stress_scene_reduced_parity(renderer):
+    """The stress scene (1000 primitives, twist + bend, 128^3 sampled volume, 2 bounces, fog) at
+    320x180 with 8^3 GI probes so the CPU oracle finishes in seconds; the full 3840x2160 / 32^3
+    configuration is the bench's stress leg."""
+    sc, cam = configs.c5_scene(width=320, height=180, gi_dims=(8, 8, 8))
+    g, o, _, _ = image_parity(renderer, sc, cam, "c5 reduced 320x180")
+    info = g.info()
+    assert info["objects"] == 1002 and info["grid_cells"] > 0
 
 
 def test_reflections_fog_two_bounces(renderer):
