@@ -323,7 +323,7 @@ def main():
         "flops_per_launch": fl["flops_per_frame"], "kernel_ms": round(render_ms, 4),
         "traffic": None,
     }
-    prof = os.path.join(ROOT, "profiles", "render_c2_traffic.json")
+    prof = os.path.join(ROOT, "profiles", "render_c2_gen_traffic.json")
     if os.path.exists(prof):
         with open(prof) as f:
             tr = json.load(f)
