@@ -15,13 +15,13 @@ def _scenes():
     sc1, cam1 = configs.c1_scene()
     sc2, cam2 = configs.c2_scene(width=480, height=270)
     p60 = configs.procedural_scene(41, 60, 320, 180)
-    p180 = configs.procedural_scene(42, 180, 320, 180)
+    p40 = configs.procedural_scene(42, 40, 320, 180)
     csg, camc = configs.csg_scene(320, 180)
     refl = configs.procedural_scene(43, 40, 240, 135)
     refl.bounces, refl.fog_density, refl.aa = 2, 0.03, 2
     return [("c1", sc1, cam1), ("c2", sc2, cam2),
             ("procedural60", p60, configs.orbit_pose(configs.SplitMix64(41))),
-            ("procedural180", p180, configs.orbit_pose(configs.SplitMix64(42))),
+            ("procedural40", p40, configs.orbit_pose(configs.SplitMix64(42))),
             ("csg", csg, camc), ("bounces2", refl, configs.orbit_pose(configs.SplitMix64(43)))]
 
 
@@ -58,10 +58,21 @@ def test_generated_kernels_equal_table_kernels():
 
 
 @pytest.mark.gpu
+def test_long_folds_keep_the_table_kernels():
+    r = hip.Renderer()
+    r.set_codegen(hip.CODEGEN_SYNC)
+    g = r.compile(configs.procedural_scene(42, 180, 64, 36).to_ir(), 64, 36)
+    assert g.codegen_status(wait=True)[0] == hip.CODEGEN_NONE
+    assert g.kernel_name().startswith("mm3e_render_")
+    g.close()
+    r.close()
+
+
+@pytest.mark.gpu
 def test_background_codegen_switches_without_changing_pixels():
     r = hip.Renderer()
     r.set_codegen(hip.CODEGEN_BACKGROUND)
-    sc = configs.procedural_scene(44, 120, 256, 144)
+    sc = configs.procedural_scene(44, 48, 256, 144)
     cam = configs.orbit_pose(configs.SplitMix64(44))
     g = r.compile(sc.to_ir(), 256, 144)
     before = g.render_rgba(cam)          # table kernels while the generated one compiles

@@ -123,6 +123,48 @@ def test_field_parity_bent_objects(renderer):
     assert bad.mean() <= 1e-3 and (mg == mo).mean() >= 0.999
 
 
+def test_field_parity_repeated_objects(renderer):
+    """Repeated objects (repetition folds space; the reference leaves them unbounded and
+    evaluates them exactly everywhere) in unions, smooth unions and tree-walked runs, with
+    twisted / mirrored / elongated members: the device fold must give the oracle's field."""
+    from mm3e.scene import Material, Object, Prim, Scene, Transform, from_euler, v3
+    rng = configs.SplitMix64(78)
+    sc = Scene(64, 36)
+    m = sc.material(Material.solid((0.6, 0.6, 0.6)))
+    sc.add(Object(Prim.plane((0.0, 1.0, 0.0), 0.0), Transform(), m))
+    for i in range(40):
+        prim = configs.random_prim(rng, i % 7, rng.uniform(0.2, 0.6))
+        xf = Transform(pos=v3((rng.uniform(-5, 5), rng.uniform(0.3, 2.0), rng.uniform(-5, 5))),
+                       rot=from_euler(rng.uniform(-3, 3), rng.uniform(-3, 3), rng.uniform(-3, 3)),
+                       scale=rng.uniform(0.6, 1.5))
+        o = Object(prim, xf, m)
+        if i % 2 == 0:
+            o.repeat((rng.uniform(2.0, 4.0), 0.0, rng.uniform(2.0, 4.0)))
+        elif i % 4 == 1:
+            o.repeat((rng.uniform(2.0, 4.0), 0.0, 0.0))
+        if i % 3 == 1:
+            o.smooth(rng.uniform(0.1, 0.4))
+        if i % 5 == 2:
+            o.twist(rng.uniform(0.3, 1.0))
+        if i % 7 == 3:
+            o.mirror(False, True, False)
+        if i % 6 == 4:
+            o.elongate((0.0, 0.15, 0.1))
+        sc.add(o)
+    g, o = both(renderer, sc)
+    rng2 = np.random.default_rng(78)
+    centres = rng2.uniform([-7, -0.3, -7], [7, 3.0, 7], (300, 3))
+    pts = np.concatenate([(centres[:, None, :] + rng2.uniform(-0.15, 0.15, (300, 64, 3))).reshape(-1, 3),
+                          rng2.uniform([-12, -3, -12], [12, 7, 12], (8000, 3))]).astype(np.float32)
+    dg, mg = g.field(pts)
+    do, mo = o.field(pts)
+    tol = 5e-5 * np.maximum(np.abs(do), 1.0)
+    bad = np.abs(dg - do) > tol
+    report("field repeated", {**g.info(), "max_abs_err": float(np.abs(dg - do).max()), "bad": int(bad.sum()),
+                              "mat_agree": float((mg == mo).mean())})
+    assert bad.mean() <= 1e-3 and (mg == mo).mean() >= 0.999
+
+
 def test_march_parity(renderer):
     sc, cam = configs.c1_scene()
     g, o = both(renderer, sc)
