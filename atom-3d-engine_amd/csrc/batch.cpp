@@ -197,6 +197,11 @@ int mm3e_batch_create(mm3e_renderer* r, const mm3e_batch_job* jobs, uint32_t n_j
       set_err(err, errlen, "batch job " + std::to_string(i) + ": scene is missing or compiled on another renderer");
       return MM3E_ERR_INVALID;
     }
+    if (j.scene->host.hdr.mode != 0) {
+      set_err(err, errlen, "batch job " + std::to_string(i) +
+                               ": debug render modes render through mm3e_scene_render_rgba, not in batches");
+      return MM3E_ERR_INVALID;
+    }
     if (j.width == 0 || j.height == 0 || j.width > 16384 || j.height > 16384) {
       set_err(err, errlen, "batch job " + std::to_string(i) + ": " + std::to_string(j.width) + "x" +
                                std::to_string(j.height) + " out of range 1..=16384");

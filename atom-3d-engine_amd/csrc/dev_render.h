@@ -9,6 +9,7 @@
 //   gi_reduce        weighted mean per cube face (gi.rs:327-341).
 //   bloom_h / bloom_v_resolve / resolve_only   post.rs:36-115 + framebuffer.rs:102-113.
 //   primary_aov      centre-pixel primary hit (film.rs:114-118, 164-165).
+//   aov_mode_pixel   debug render modes Normal / Depth / Ao / Steps / Albedo (lib.rs:1178-1218).
 //   field_probe / march_probe   map / raw-march parity probes (wgsl.rs:641-646 twin).
 #pragma once
 #include "dev_math.h"
@@ -546,6 +547,43 @@ MM3E_DEV void resolve_only(const PostParams& pp, const float* __restrict__ hdr, 
 // ---------------------------------------------------------------------------------------
 // parity probes
 // ---------------------------------------------------------------------------------------
+// lib.rs:1401-1407 blue -> green -> yellow -> red heatmap of the march-step count.
+MM3E_DEV V3 heatmap(float t) {
+  t = rclamp(t, 0.0f, 1.0f);
+  return V3{rclamp(t * 3.0f - 1.5f, 0.0f, 1.0f), rclamp(1.0f - rabs(t * 3.0f - 1.3f), 0.0f, 1.0f),
+            rclamp(1.0f - t * 2.2f, 0.0f, 1.0f)};
+}
+
+// lib.rs:1178-1218 aov_pixel: a debug render mode (1 Normal, 2 Depth, 3 Ao, 4 Steps,
+// 5 Albedo), one centre sample per pixel written straight to RGBA8 (no HDR buffer, no post).
+template <class Map>
+MM3E_DEV void aov_mode_pixel(const DevScene* __restrict__ scp, const FrameParams& fp, uchar4* __restrict__ rgba,
+                             uint32_t mode) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= fp.width) return;
+  const DevScene& sc = *scp;
+  Map field{scp, global_tables(scp)};
+  V3 ro, rd;
+  camera_ray(fp, (float)x + 0.5f, (float)y + 0.5f, ro, rd);
+  const Hit h = march(sc, field, ro, rd);
+  V3 v = V3{0.0f, 0.0f, 0.0f};
+  if (mode == 1) {
+    if (h.hit) v = scale(hit_normal(sc, field, h.pos), 0.5f) + V3{0.5f, 0.5f, 0.5f};
+  } else if (mode == 2) {
+    const float d = rclamp(h.t / sc.max_dist, 0.0f, 1.0f);
+    v = V3{1.0f - d, 1.0f - d, 1.0f - d};
+  } else if (mode == 3) {
+    float a = 1.0f;
+    if (h.hit) a = ambient_occlusion(sc, field, h.pos, hit_normal(sc, field, h.pos));
+    v = V3{a, a, a};
+  } else if (mode == 4) {
+    v = heatmap((float)h.steps / (float)sc.max_steps);
+  } else if (mode == 5) {
+    if (h.hit) v = surface_albedo(sc.mat[h.mat % sc.n_mat], h.pos);
+  }
+  rgba[(size_t)y * fp.width + x] = uchar4{to_u8(v.x), to_u8(v.y), to_u8(v.z), (uint8_t)255};
+}
+
 template <class Map>
 MM3E_DEV void primary_aov(const DevScene* __restrict__ scp, const FrameParams& fp, uint8_t* __restrict__ hit,
                           uint32_t* __restrict__ mat, float* __restrict__ depth, uint32_t* __restrict__ steps) {

@@ -88,6 +88,11 @@ extern "C" __global__ void __launch_bounds__(256)
 }
 
 extern "C" __global__ void __launch_bounds__(256)
+    mm3e_aov_mode_table(const DevScene* __restrict__ sc, FrameParams fp, uchar4* __restrict__ rgba, uint32_t mode) {
+  aov_mode_pixel<TableMap>(sc, fp, rgba, mode);
+}
+
+extern "C" __global__ void __launch_bounds__(256)
     mm3e_march_probe_table(const DevScene* __restrict__ sc, const float* __restrict__ rays, uint32_t n,
                            float* __restrict__ t, uint8_t* __restrict__ hit, uint32_t* __restrict__ mat,
                            float* __restrict__ nrm, uint32_t* __restrict__ steps) {
@@ -175,6 +180,13 @@ hipError_t launch_primary_aov_table(hipStream_t st, const DevScene* sc, const Fr
                                     uint32_t* mat, float* depth, uint32_t* steps) {
   dim3 grid((fp.width + 255) / 256, fp.height);
   hipLaunchKernelGGL(mm3e_primary_aov_table, grid, dim3(256), 0, st, sc, fp, hit, mat, depth, steps);
+  return hipGetLastError();
+}
+
+hipError_t launch_aov_mode_table(hipStream_t st, const DevScene* sc, const FrameParams& fp, uchar4* rgba,
+                                 uint32_t mode) {
+  hipLaunchKernelGGL(mm3e_aov_mode_table, dim3((fp.width + 255) / 256, fp.height), dim3(256), 0, st, sc, fp, rgba,
+                     mode);
   return hipGetLastError();
 }
 

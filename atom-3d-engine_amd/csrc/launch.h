@@ -51,6 +51,9 @@ hipError_t launch_post(hipStream_t st, const PostParams& pp, const float* kern, 
                        uchar4* rgba);
 hipError_t launch_primary_aov_table(hipStream_t st, const DevScene* sc, const FrameParams& fp, uint8_t* hit,
                                     uint32_t* mat, float* depth, uint32_t* steps);
+// Debug render mode (lib.rs:1178-1218) straight into RGBA8; mode 1..5.
+hipError_t launch_aov_mode_table(hipStream_t st, const DevScene* sc, const FrameParams& fp, uchar4* rgba,
+                                 uint32_t mode);
 hipError_t launch_field_probe_table(hipStream_t st, const DevScene* sc, const float* pts, uint32_t n, float* dist,
                                     uint32_t* mat);
 hipError_t launch_march_probe_table(hipStream_t st, const DevScene* sc, const float* rays, uint32_t n, float* t,

@@ -230,6 +230,15 @@ bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std
   dim3 block(s->tw * s->th * s->aa * s->aa);
   hipEvent_t* ev = s->ev[s->frames % mm3e_scene::kRing];
   hipEventRecord(ev[0], st);
+  if (s->host.hdr.mode != 0) {  // debug render mode: straight to RGBA8, no post (lib.rs:873-878)
+    if (!hip_ok(launch_aov_mode_table(st, dsc, fp, s->d_rgba, s->host.hdr.mode), "render-mode launch", &msg))
+      return false;
+    hipEventRecord(ev[1], st);
+    hipEventRecord(ev[2], st);
+    s->frames++;
+    s->timing_valid = true;
+    return true;
+  }
   const mm3e::GenJob* gj = s->gen.get();
   if (dsc == s->d_scene && gj && gj->state.load(std::memory_order_acquire) == MM3E_CODEGEN_READY) {
     const DevScene* sc_arg = dsc;

@@ -319,6 +319,10 @@ bool parse_scene_ir(const void* ir, size_t len_, HostScene& sc, std::string& err
     err = "scene IR array lies outside the blob";
     return false;
   }
+  if (h.mode > 5) {
+    err = "unknown render mode in scene IR";
+    return false;
+  }
   if (h.appearance) {  // wgsl.rs:42-44
     err = "UV textures require the checked CPU renderer";
     return false;

@@ -351,6 +351,7 @@ int mm3e_server_create(mm3e_renderer* r, const mm3e_server_config* cfg, mm3e_ser
 
 int mm3e_server_add_scene(mm3e_server* S, mm3e_scene* scene, uint32_t* scene_id) {
   if (!S || !scene || !scene_id || scene->r != S->r) return MM3E_ERR_INVALID;
+  if (scene->host.hdr.mode != 0) return MM3E_ERR_INVALID;  // debug render modes are per-frame only
   std::lock_guard<std::mutex> lk(S->mu);
   S->scenes.push_back(scene);
   *scene_id = (uint32_t)(S->scenes.size() - 1);

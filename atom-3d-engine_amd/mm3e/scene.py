@@ -44,6 +44,8 @@ PRIM_NAMES = ("sphere", "box", "round_box", "torus", "cylinder", "capsule", "con
               "ellipsoid", "octahedron", "hex_prism", "plane", "csg", "volume", "surface")
 
 COMBINE_UNION, COMBINE_SMOOTH, COMBINE_SUBTRACT = 0, 1, 2
+# RenderMode (mm3e-orchestrator/src/lib.rs:49-61), declaration order
+MODE_BEAUTY, MODE_NORMAL, MODE_DEPTH, MODE_AO, MODE_STEPS, MODE_ALBEDO = range(6)
 GI_NONE, GI_BAKED, GI_BAKE = 0, 1, 2
 
 CSG_PRIM, CSG_TRANSFORM, CSG_UNION, CSG_INTERSECT = 0, 1, 2, 3

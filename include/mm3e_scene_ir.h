@@ -71,7 +71,8 @@ typedef struct mm3e_ir_header {
   uint32_t width, height;   /* Scene::width/height (the CPU renderer's image size) */
   uint32_t aa, bounces;     /* Scene::aa (0 acts as 1), Scene::bounces */
   uint32_t shadows, ao;     /* bools */
-  uint32_t mode;            /* RenderMode: 0 = Beauty (the only mode the GPU path renders) */
+  uint32_t mode;            /* RenderMode (lib.rs:49-61): 0 Beauty, 1 Normal, 2 Depth, 3 Ao, 4 Steps,
+                               5 Albedo; 1..5 render one centre sample straight to RGBA8 */
   uint32_t appearance;      /* nonzero = texture bindings present (rejected, wgsl.rs:42-44) */
   float sun_dir[3];
   float fog_density;

@@ -1369,8 +1369,60 @@ int orc_resolve(const orc_scene* s, const float* hdr, uint8_t* rgba) {
   return 0;
 }
 
+// lib.rs:1401-1407 blue -> green -> yellow -> red step heatmap.
+static V3<float> heatmap(float t) {
+  t = rclamp(t, 0.0f, 1.0f);
+  const float r = rclamp(t * 3.0f - 1.5f, 0.0f, 1.0f);
+  const float g = rclamp(1.0f - rabs(t * 3.0f - 1.3f), 0.0f, 1.0f);
+  const float b = rclamp(1.0f - t * 2.2f, 0.0f, 1.0f);
+  return V3<float>(r, g, b);
+}
+
+// lib.rs:1178-1218 aov_pixel: one centre sample per pixel, written straight to display.
+static V3<float> aov_pixel(const Renderer<float>& rd, const Camera& cam, uint32_t x, uint32_t y) {
+  const Scene& sc = *rd.sc;
+  Ray<float> ray = Renderer<float>::cam_ray(cam, (float)x + 0.5f, (float)y + 0.5f, sc.width, sc.height);
+  Hit<float> hit = rd.march(ray);
+  switch (sc.mode) {
+    case 1:  // Normal
+      return hit.hit ? scale(hit.normal, 0.5f) + V3<float>(0.5f, 0.5f, 0.5f) : V3<float>();
+    case 2: {  // Depth
+      const float d = rclamp(hit.t / sc.max_dist, 0.0f, 1.0f);
+      return V3<float>(1.0f - d, 1.0f - d, 1.0f - d);
+    }
+    case 3: {  // Ao
+      if (!hit.hit) return V3<float>(1.0f, 1.0f, 1.0f);
+      const float a = rd.mr.ambient_occlusion(rd.world, hit.pos, hit.normal);
+      return V3<float>(a, a, a);
+    }
+    case 4:  // Steps
+      return heatmap((float)hit.steps / (float)sc.max_steps);
+    case 5:  // Albedo (appearance empty: material_for_hit = surface_albedo, appearance.rs:427-431)
+      return hit.hit ? surface_albedo<float>(sc.materials[hit.mat % sc.materials.size()], hit.pos) : V3<float>();
+    default:
+      return V3<float>();
+  }
+}
+
 int orc_render_rgba(const orc_scene* s, const float* cam13, unsigned threads, uint8_t* rgba) {
-  std::vector<float> hdr((size_t)s->sc.width * s->sc.height * 3);
+  const Scene& sc = s->sc;
+  if (sc.mode != 0) {  // lib.rs:873-878: debug AOV modes skip the HDR buffer and post pass
+    Camera cam = to_cam(cam13);
+    Renderer<float> rd(&sc);
+    for_bands(sc.height, threads, [&](uint32_t y0, uint32_t y1) {
+      for (uint32_t y = y0; y < y1; y++)
+        for (uint32_t x = 0; x < sc.width; x++) {
+          const V3<float> c = clamp01(aov_pixel(rd, cam, x, y));  // Rgba::from_vec3
+          uint8_t* o = rgba + 4 * ((size_t)y * sc.width + x);
+          o[0] = to_u8(rclamp(c.x, 0.0f, 1.0f) * 255.0f + 0.5f);
+          o[1] = to_u8(rclamp(c.y, 0.0f, 1.0f) * 255.0f + 0.5f);
+          o[2] = to_u8(rclamp(c.z, 0.0f, 1.0f) * 255.0f + 0.5f);
+          o[3] = 255;
+        }
+    });
+    return 0;
+  }
+  std::vector<float> hdr((size_t)sc.width * sc.height * 3);
   orc_render_linear(s, cam13, threads, hdr.data());
   return orc_resolve(s, hdr.data(), rgba);
 }

@@ -119,7 +119,11 @@ class OracleScene:
         return out
 
     def render_rgba(self, cam, threads=0):
-        return self.resolve(self.render_linear(cam, threads))
+        """lib.rs:866-879 `render`: beauty through the HDR buffer and post pass, or, in a debug
+        AOV mode, one centre sample per pixel written straight to display."""
+        out = np.zeros((self.height, self.width, 4), np.uint8)
+        lib().orc_render_rgba(self.h, _p(f32(cam)), threads or os.cpu_count(), _p(out))
+        return out
 
     def render_aov(self, cam, threads=0):
         n = self.width * self.height

@@ -254,6 +254,23 @@ def test_c5_stress_scene_reduced_parity(renderer):
     assert info["objects"] == 1002 and info["grid_cells"] > 0
 
 
+@pytest.mark.parametrize("mode", [1, 2, 3, 4, 5])
+def test_debug_render_modes(renderer, mode):
+    """RenderMode Normal / Depth / Ao / Steps / Albedo (lib.rs:1178-1218): one centre sample per
+    pixel straight to RGBA8, on a dual-safe scene (c1) and a tetrahedron-normal one (twist/bend)."""
+    sc, cam = configs.c1_scene()
+    sc.mode = mode
+    p = configs.procedural_scene(46, 50, 320, 180)
+    p.mode = mode
+    for name, scene, c in (("c1", sc, cam), ("procedural50", p, configs.orbit_pose(configs.SplitMix64(46)))):
+        g, o = both(renderer, scene)
+        a, b = g.render_rgba(c), o.render_rgba(c)
+        m = image_metrics(a, b)
+        report(f"mode {mode} {name}", m)
+        assert m["frac_within"] >= 0.999
+        g.close()
+
+
 def test_reflections_fog_two_bounces(renderer):
     sc = configs.procedural_scene(43, 40, 240, 135)
     sc.bounces = 2

@@ -522,3 +522,65 @@ def test_csg_sphere_renders_identically_to_primitive_sphere():  # local_csg.rs:7
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
     assert 0 < a[0].sum() < a[0].size  # the sphere is in view and does not fill it
+
+
+# ---- debug AOV render modes (lib.rs:1178-1218, heatmap lib.rs:1401-1407) ----------------
+
+def _plane_view(mode):
+    from mm3e.scene import Material, Object, Prim, Scene, Transform
+    sc = Scene(32, 24)
+    m = sc.material(Material.solid((0.2, 0.4, 0.6)))
+    sc.add(Object(Prim.plane((0.0, 1.0, 0.0), 0.0), Transform(), m))
+    sc.mode = mode
+    cam = configs.look_at((0.0, 3.0, -6.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), math.radians(60.0))
+    o = po.OracleScene(sc.to_ir())
+    img = o.render_rgba(cam)
+    hit, _, _, _ = o.render_aov(cam)
+    return sc, cam, img, hit.astype(bool)
+
+
+def test_mode_normal_albedo_steps_closed_forms():
+    from mm3e.scene import MODE_ALBEDO, MODE_NORMAL, MODE_STEPS
+    _, _, img, hit = _plane_view(MODE_NORMAL)
+    assert hit.any() and (~hit).any()
+    assert (img[hit][:, :3] == [128, 255, 128]).all()   # n = +y -> 0.5 n + 0.5
+    assert (img[~hit][:, :3] == 0).all() and (img[..., 3] == 255).all()
+    _, _, img, hit = _plane_view(MODE_ALBEDO)
+    assert (img[hit][:, :3] == [51, 102, 153]).all()    # (0.2, 0.4, 0.6) * 255 + 0.5, truncated
+    assert (img[~hit][:, :3] == 0).all()
+    sc, cam, img, hit = _plane_view(MODE_STEPS)
+    assert (img[~hit][:, :3] == [255, 0, 0]).all()      # misses report max_steps: heatmap(1) = red
+    # hits: the heatmap of lib.rs:1401-1407 at steps / max_steps, quantised like every display pixel
+    _, _, _, steps = po.OracleScene(sc.to_ir()).render_aov(cam)
+    t = np.clip(steps.astype(np.float32) / np.float32(sc.marcher.max_steps), 0, 1).astype(np.float32)
+    r = np.clip(t * np.float32(3) - np.float32(1.5), 0, 1)
+    g = np.clip(np.float32(1) - np.abs(t * np.float32(3) - np.float32(1.3)), 0, 1)
+    b = np.clip(np.float32(1) - t * np.float32(2.2), 0, 1)
+    q = lambda c: (c.astype(np.float32) * np.float32(255) + np.float32(0.5)).astype(np.uint8)
+    heat = np.stack([q(r), q(g), q(b)], -1)
+    assert (np.abs(img[hit][:, :3].astype(int) - heat[hit].astype(int)) <= 1).all()
+    assert len(np.unique(steps[hit])) > 3               # grazing rays take more steps: not one colour
+
+
+def test_mode_depth_and_ao_closed_forms():
+    from mm3e.scene import MODE_AO, MODE_DEPTH
+    sc, cam, img, hit = _plane_view(MODE_DEPTH)
+    h, w = hit.shape
+    # analytic t to the plane y = 0 along each pixel-centre ray
+    eye, fwd, right, up, fs = cam[0:3], cam[3:6], cam[6:9], cam[9:12], cam[12]
+    ys, xs = np.nonzero(hit)
+    aspect = w / h
+    for x, y in zip(xs, ys):
+        ndc_x = ((2.0 * (x + 0.5)) / w - 1.0) * aspect * fs
+        ndc_y = (1.0 - (2.0 * (y + 0.5)) / h) * fs
+        d = fwd + right * ndc_x + up * ndc_y
+        d = d / np.linalg.norm(d)
+        t = -eye[1] / d[1]
+        # the march stops once the plane distance (t_hit - t) |d.y| drops below eps (1 + t/2)
+        t_lo = t - sc.marcher.eps * (1.0 + 0.5 * t) / abs(d[1])
+        hi = (1.0 - min(max(t_lo / sc.marcher.max_dist, 0.0), 1.0)) * 255.0 + 0.5
+        lo = (1.0 - min(max(t / sc.marcher.max_dist, 0.0), 1.0)) * 255.0 + 0.5
+        assert int(lo) - 1 <= int(img[y, x, 0]) <= int(hi) + 1
+    assert (img[~hit][:, :3] == 0).all()  # misses end beyond max_dist
+    _, _, img, hit = _plane_view(MODE_AO)
+    assert (img[..., :3] == 255).all()    # an open plane occludes nothing; misses report 1
