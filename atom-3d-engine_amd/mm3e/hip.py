@@ -28,6 +28,7 @@ EXPORTS = (
     "mm3e_server_add_scene", "mm3e_server_submit", "mm3e_server_poll", "mm3e_server_wait", "mm3e_server_stats",
     "mm3e_server_destroy", "mm3e_scene_kernel_name",
     "mm3e_scene_info", "mm3e_renderer_set_codegen", "mm3e_scene_codegen_status", "mm3e_scene_emit_source",
+    "mm3e_scene_parse_text", "mm3e_scene_to_text",
 )
 
 CODEGEN_OFF, CODEGEN_BACKGROUND, CODEGEN_SYNC = 0, 1, 2
@@ -69,6 +70,9 @@ def lib():
         L.mm3e_renderer_set_codegen.argtypes = [P, C.c_int]
         L.mm3e_scene_codegen_status.argtypes = [P, C.c_int, P, C.c_char_p, C.c_size_t]
         L.mm3e_scene_emit_source.argtypes = [P, C.c_size_t, C.c_char_p, C.POINTER(C.c_size_t)]
+        L.mm3e_scene_parse_text.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p, C.POINTER(C.c_size_t), P,
+                                            C.c_char_p, C.c_size_t]
+        L.mm3e_scene_to_text.argtypes = [P, C.c_size_t, P, C.c_char_p, C.POINTER(C.c_size_t), C.c_char_p, C.c_size_t]
         L.mm3e_scene_field.argtypes = [P, P, C.c_uint32, P, P]
         L.mm3e_scene_march.argtypes = [P, P, C.c_uint32, P, P, P, P, P]
         L.mm3e_camera_look_at.argtypes = [P, P, P, C.c_float, P]
@@ -151,6 +155,37 @@ def emit_source(ir: bytes) -> str:
     rc = lib().mm3e_scene_emit_source(ir, len(ir), buf, C.byref(n))
     if rc != 0:
         raise Mm3eError(rc, "scene IR rejected")
+    return buf.value.decode()
+
+
+def parse_text(text: str):
+    """scene_io::parse: `.mm3e` text -> (scene IR bytes, camera[13])."""
+    b = text.encode()
+    n = C.c_size_t(0)
+    err = C.create_string_buffer(1024)
+    cam = np.zeros(13, np.float32)
+    rc = lib().mm3e_scene_parse_text(b, len(b), None, C.byref(n), None, err, 1024)
+    if n.value == 0:
+        raise Mm3eError(rc, err.value.decode())
+    buf = C.create_string_buffer(n.value)
+    rc = lib().mm3e_scene_parse_text(b, len(b), buf, C.byref(n), _p(cam), err, 1024)
+    if rc != 0:
+        raise Mm3eError(rc, err.value.decode())
+    return buf.raw[:n.value], cam
+
+
+def to_text(ir: bytes, cam) -> str:
+    """scene_io::try_serialize: scene IR + camera -> `.mm3e` text."""
+    n = C.c_size_t(0)
+    err = C.create_string_buffer(1024)
+    c = _cam(cam)
+    lib().mm3e_scene_to_text(ir, len(ir), _p(c), None, C.byref(n), err, 1024)
+    if n.value == 0:
+        raise Mm3eError(1, err.value.decode())
+    buf = C.create_string_buffer(n.value)
+    rc = lib().mm3e_scene_to_text(ir, len(ir), _p(c), buf, C.byref(n), err, 1024)
+    if rc != 0:
+        raise Mm3eError(rc, err.value.decode())
     return buf.value.decode()
 
 

@@ -154,6 +154,18 @@ int mm3e_scene_codegen_status(mm3e_scene* s, int wait, int* state, char* err, si
  * to the size needed (including the terminating NUL) and MM3E_ERR_INVALID is returned. */
 int mm3e_scene_emit_source(const void* scene_ir, size_t ir_len, char* src, size_t* src_len);
 
+/* ---- the `.mm3e` text scene format (mm3e-orchestrator/src/scene_io.rs) --------------------
+ * parse (scene_io.rs:200-331): text -> scene IR + camera; errors carry the reference's
+ * messages ("line N: ...").  With ir == NULL or *ir_len too small, *ir_len is set to the size
+ * needed and MM3E_ERR_INVALID is returned.  The format holds no CSG, volumes, GI, shadow/AO
+ * budgets or normal_h; those take Scene::new's values. */
+int mm3e_scene_parse_text(const char* text, size_t text_len, uint8_t* ir, size_t* ir_len, mm3e_camera* cam,
+                          char* err, size_t errlen);
+/* try_serialize (scene_io.rs:32-164): scene IR + camera -> text (numbers as the shortest
+ * decimal that reads back to the same f32), NUL-terminated; same size protocol as above. */
+int mm3e_scene_to_text(const void* scene_ir, size_t ir_len, const mm3e_camera* cam, char* text, size_t* text_len,
+                       char* err, size_t errlen);
+
 /* ---- batched rendering ----------------------------------------------------------------
  * Many (compiled scene, camera, size) jobs rendered by one persistent launch that pulls
  * tiles from a global atomic queue, followed by one batched bloom/tonemap over all jobs;
