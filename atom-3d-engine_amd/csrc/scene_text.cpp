@@ -286,7 +286,7 @@ bool parse_object(Cur& c, mm3e_ir_object& o) {
     else if (kw == "twist") ok = c.f("twist", o.twist);
     else if (kw == "bend") ok = c.f("bend", o.bend);
     else if (kw == "mirror") {
-      uint32_t mx, my, mz;
+      uint32_t mx = 0, my = 0, mz = 0;
       ok = c.b("mx", mx) && c.b("my", my) && c.b("mz", mz);
       o.mirror = mx | (my << 1) | (mz << 2);
     } else if (kw == "repeat") ok = c.v3("repeat", o.repeat);

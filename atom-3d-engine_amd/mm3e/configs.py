@@ -438,3 +438,87 @@ def csg_scene(width: int = 320, height: int = 180, seed: int = 7) -> Tuple[Scene
     sc.light(Light.point((2.0, 3.5, 2.0), (4.0, 3.6, 3.2)))
     cam = look_at((0.0, 3.0, -6.5), (0.0, 0.8, 0.0), (0.0, 1.0, 0.0), math.radians(55.0))
     return sc, cam
+
+
+# ---- triangle meshes (inputs of the mesh -> SDF bake) ------------------------------------------
+
+def uv_sphere_mesh(rings: int, segments: int, radius: float) -> Tuple[np.ndarray, np.ndarray]:
+    """Closed UV sphere as in the reference's mesh tests (mm3e-orchestrator/tests/engine.rs:370-401):
+    a pole vertex, rings-1 rings of `segments` vertices, the other pole; outward winding."""
+    pos = [(0.0, radius, 0.0)]
+    for r in range(1, rings):
+        phi = np.float32(np.pi) * np.float32(r) / np.float32(rings)
+        for s in range(segments):
+            th = np.float32(2 * np.pi) * np.float32(s) / np.float32(segments)
+            pos.append((radius * np.sin(phi) * np.cos(th), radius * np.cos(phi), radius * np.sin(phi) * np.sin(th)))
+    pos.append((0.0, -radius, 0.0))
+
+    def ring(r, s):
+        return 1 + (r - 1) * segments + (s % segments)
+
+    tri = [(0, ring(1, s + 1), ring(1, s)) for s in range(segments)]
+    for r in range(1, rings - 1):
+        for s in range(segments):
+            a, b, c, d = ring(r, s), ring(r, s + 1), ring(r + 1, s + 1), ring(r + 1, s)
+            tri += [(a, b, c), (a, c, d)]
+    south = len(pos) - 1
+    tri += [(south, ring(rings - 1, s), ring(rings - 1, s + 1)) for s in range(segments)]
+    return np.asarray(pos, np.float32), np.asarray(tri, np.uint32)
+
+
+def box_mesh(half=(1.0, 1.0, 1.0), center=(0.0, 0.0, 0.0)) -> Tuple[np.ndarray, np.ndarray]:
+    """Axis-aligned box as 12 outward-facing triangles (two per face, split on a diagonal)."""
+    hx, hy, hz = half
+    cx, cy, cz = center
+    pos = np.array([(cx + sx * hx, cy + sy * hy, cz + sz * hz)
+                    for sz in (-1, 1) for sy in (-1, 1) for sx in (-1, 1)], np.float32)
+    # vertex v = x + 2y + 4z (bits of the + corners)
+    quads = [(0, 2, 3, 1), (4, 5, 7, 6), (0, 1, 5, 4), (2, 6, 7, 3), (0, 4, 6, 2), (1, 3, 7, 5)]
+    tri = []
+    for a, b, c, d in quads:
+        tri += [(a, b, c), (a, c, d)]
+    return pos, np.asarray(tri, np.uint32)
+
+
+def torus_knot_mesh(p: float = 2.0, q: float = 3.0, segments: int = 240, ring_verts: int = 36,
+                    tube_r: float = 0.22) -> Tuple[np.ndarray, np.ndarray]:
+    """Closed (p, q) torus-knot tube, the reference's mesh demo input
+    (mm3e-orchestrator/examples/mesh_demo.rs:12-40): curve r = 1.2 + 0.5 cos(q t), a
+    parallel-transported frame, `ring_verts` vertices per ring, two triangles per quad."""
+    def curve(t):
+        r = 1.2 + 0.5 * np.cos(q * t)
+        return np.array([r * np.cos(p * t), 0.5 * np.sin(q * t), r * np.sin(p * t)])
+
+    def norm(v):
+        n = np.linalg.norm(v)
+        return v / n if n > 1e-12 else v * 0.0
+
+    pos = []
+    normal = np.array([0.0, 1.0, 0.0])
+    for s in range(segments):
+        t = 2 * np.pi * s / segments
+        tangent = norm(curve(t + 1e-3) - curve(t - 1e-3))
+        normal = norm(normal - tangent * normal.dot(tangent))
+        binormal = np.cross(tangent, normal)
+        center = curve(t)
+        for v in range(ring_verts):
+            a = 2 * np.pi * v / ring_verts
+            pos.append(center + normal * (tube_r * np.cos(a)) + binormal * (tube_r * np.sin(a)))
+
+    def idx(s, v):
+        return (s % segments) * ring_verts + (v % ring_verts)
+
+    tri = []
+    for s in range(segments):
+        for v in range(ring_verts):
+            a, b, c, d = idx(s, v), idx(s + 1, v), idx(s + 1, v + 1), idx(s, v + 1)
+            tri += [(a, b, c), (a, c, d)]
+    return np.asarray(pos, np.float32), np.asarray(tri, np.uint32)
+
+
+def mesh_to_obj(pos: np.ndarray, tri: np.ndarray, header: str = "MM3E mesh") -> str:
+    """Wavefront OBJ text (`v` lines with shortest round-trip floats, 1-based `f` lines)."""
+    out = [f"# {header}"]
+    out += ["v {} {} {}".format(*(repr(float(np.float32(c))) for c in p)) for p in pos]
+    out += ["f {} {} {}".format(*(int(i) + 1 for i in t)) for t in tri]
+    return "\n".join(out) + "\n"

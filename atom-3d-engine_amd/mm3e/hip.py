@@ -29,6 +29,9 @@ EXPORTS = (
     "mm3e_server_destroy", "mm3e_scene_kernel_name",
     "mm3e_scene_info", "mm3e_renderer_set_codegen", "mm3e_scene_codegen_status", "mm3e_scene_emit_source",
     "mm3e_scene_parse_text", "mm3e_scene_to_text",
+    "mm3e_mesh_parse_obj", "mm3e_mesh_sdf_layout", "mm3e_mesh_bake_sdf", "mm3e_mesh_baker_create",
+    "mm3e_mesh_baker_layout", "mm3e_mesh_baker_info", "mm3e_mesh_baker_run", "mm3e_mesh_baker_wait",
+    "mm3e_mesh_baker_read", "mm3e_mesh_baker_destroy", "mm3e_sdfv_encode", "mm3e_sdfv_decode",
 )
 
 CODEGEN_OFF, CODEGEN_BACKGROUND, CODEGEN_SYNC = 0, 1, 2
@@ -100,6 +103,20 @@ def lib():
         L.mm3e_server_stats.argtypes = [P, P, P, P, P, P]
         L.mm3e_server_destroy.argtypes = [P]
         L.mm3e_server_destroy.restype = None
+        Z = C.c_size_t
+        L.mm3e_mesh_parse_obj.argtypes = [C.c_char_p, Z, P, Z, P, Z, C.POINTER(Z), C.POINTER(Z), C.c_char_p, Z]
+        L.mm3e_mesh_sdf_layout.argtypes = [P, Z, P, Z, C.c_uint32, C.c_float, P, C.c_char_p, Z]
+        L.mm3e_mesh_bake_sdf.argtypes = [P, P, Z, P, Z, C.c_uint32, C.c_float, P, Z, P, C.c_char_p, Z]
+        L.mm3e_mesh_baker_create.argtypes = [P, P, Z, P, Z, C.c_uint32, C.c_float, C.POINTER(P), C.c_char_p, Z]
+        L.mm3e_mesh_baker_layout.argtypes = [P, P]
+        L.mm3e_mesh_baker_info.argtypes = [P, P, P, P]
+        L.mm3e_mesh_baker_run.argtypes = [P]
+        L.mm3e_mesh_baker_wait.argtypes = [P, P, P]
+        L.mm3e_mesh_baker_read.argtypes = [P, P, Z, C.c_char_p, Z]
+        L.mm3e_mesh_baker_destroy.argtypes = [P]
+        L.mm3e_mesh_baker_destroy.restype = None
+        L.mm3e_sdfv_encode.argtypes = [P, P, Z, P, C.POINTER(Z), C.c_char_p, Z]
+        L.mm3e_sdfv_decode.argtypes = [P, Z, P, P, Z, C.c_char_p, Z]
         _lib = L
     return _lib
 
@@ -189,6 +206,133 @@ def to_text(ir: bytes, cam) -> str:
     return buf.value.decode()
 
 
+class SdfLayout(C.Structure):
+    """mm3e_sdf_layout: SdfVolume's dims / min / cell (mm3e-kit/src/volume.rs:14-23)."""
+    _fields_ = [("dims", C.c_uint32 * 3), ("min", C.c_float * 3), ("cell", C.c_float * 3)]
+
+    def as_tuple(self):
+        return tuple(self.dims), tuple(self.min), tuple(self.cell)
+
+
+def _mesh_arrays(positions, triangles):
+    pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    tri = np.ascontiguousarray(triangles, np.uint32).reshape(-1, 3)
+    return pos, tri
+
+
+def parse_obj(text: str):
+    """mesh::parse_obj: OBJ text -> (positions[n,3] f32, triangles[t,3] u32)."""
+    b = text.encode()
+    n, t = C.c_size_t(0), C.c_size_t(0)
+    err = C.create_string_buffer(1024)
+    rc = lib().mm3e_mesh_parse_obj(b, len(b), None, 0, None, 0, C.byref(n), C.byref(t), err, 1024)
+    if rc != 0:
+        raise Mm3eError(rc, err.value.decode())
+    pos = np.zeros((n.value, 3), np.float32)
+    tri = np.zeros((t.value, 3), np.uint32)
+    rc = lib().mm3e_mesh_parse_obj(b, len(b), _p(pos), n.value, _p(tri), t.value, C.byref(n), C.byref(t), err, 1024)
+    if rc != 0:
+        raise Mm3eError(rc, err.value.decode())
+    return pos, tri
+
+
+def sdf_layout(positions, triangles, resolution: int, padding: float) -> SdfLayout:
+    """The grid mesh::bake_sdf would produce (host only)."""
+    pos, tri = _mesh_arrays(positions, triangles)
+    out = SdfLayout()
+    err = C.create_string_buffer(1024)
+    rc = lib().mm3e_mesh_sdf_layout(_p(pos), len(pos), _p(tri), len(tri), int(resolution), float(padding),
+                                    C.byref(out), err, 1024)
+    if rc != 0:
+        raise Mm3eError(rc, err.value.decode())
+    return out
+
+
+def sdfv_encode(layout: SdfLayout, data) -> bytes:
+    """SdfVolume::to_sdfv_bytes."""
+    d = np.ascontiguousarray(data, np.float32).reshape(-1)
+    n = C.c_size_t(0)
+    err = C.create_string_buffer(1024)
+    lib().mm3e_sdfv_encode(C.byref(layout), _p(d), d.size, None, C.byref(n), err, 1024)
+    if n.value == 0:
+        raise Mm3eError(1, err.value.decode())
+    out = np.zeros(n.value, np.uint8)
+    rc = lib().mm3e_sdfv_encode(C.byref(layout), _p(d), d.size, _p(out), C.byref(n), err, 1024)
+    if rc != 0:
+        raise Mm3eError(rc, err.value.decode())
+    return out.tobytes()
+
+
+def sdfv_decode(blob: bytes):
+    """SdfVolume::from_sdfv_bytes -> (layout, samples[nz, ny, nx])."""
+    b = np.frombuffer(blob, np.uint8)
+    lay = SdfLayout()
+    err = C.create_string_buffer(1024)
+    rc = lib().mm3e_sdfv_decode(_p(b) if b.size else None, b.size, C.byref(lay), None, 0, err, 1024)
+    if rc != 0:
+        raise Mm3eError(rc, err.value.decode())
+    nx, ny, nz = lay.dims
+    data = np.zeros((nz, ny, nx), np.float32)
+    rc = lib().mm3e_sdfv_decode(_p(b), b.size, C.byref(lay), _p(data), data.size, err, 1024)
+    if rc != 0:
+        raise Mm3eError(rc, err.value.decode())
+    return lay, data
+
+
+class MeshBaker:
+    """A resident mesh -> SDF bake (mesh::bake_sdf on the GPU): create once, run / read often."""
+
+    def __init__(self, renderer, positions, triangles, resolution: int, padding: float):
+        pos, tri = _mesh_arrays(positions, triangles)
+        self.h = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        rc = lib().mm3e_mesh_baker_create(renderer.h, _p(pos), len(pos), _p(tri), len(tri), int(resolution),
+                                          float(padding), C.byref(self.h), err, 1024)
+        if rc != 0:
+            self.h = None
+            raise Mm3eError(rc, err.value.decode())
+        self.layout = SdfLayout()
+        lib().mm3e_mesh_baker_layout(self.h, C.byref(self.layout))
+
+    def info(self):
+        t, n, d = C.c_uint32(0), C.c_uint32(0), C.c_uint32(0)
+        lib().mm3e_mesh_baker_info(self.h, C.byref(t), C.byref(n), C.byref(d))
+        return {"triangles": t.value, "nodes": n.value, "depth": d.value}
+
+    def run(self):
+        rc = lib().mm3e_mesh_baker_run(self.h)
+        if rc != 0:
+            raise Mm3eError(rc, "mesh bake launch failed")
+
+    def wait(self):
+        """GPU milliseconds of (parity kernel, nearest kernel) of the last run."""
+        a, b = C.c_float(0), C.c_float(0)
+        rc = lib().mm3e_mesh_baker_wait(self.h, C.byref(a), C.byref(b))
+        if rc != 0:
+            raise Mm3eError(rc, "mesh bake failed")
+        return a.value, b.value
+
+    def read(self) -> np.ndarray:
+        nx, ny, nz = self.layout.dims
+        out = np.zeros((nz, ny, nx), np.float32)
+        err = C.create_string_buffer(1024)
+        rc = lib().mm3e_mesh_baker_read(self.h, _p(out), out.size, err, 1024)
+        if rc != 0:
+            raise Mm3eError(rc, err.value.decode())
+        return out
+
+    def close(self):
+        if self.h:
+            lib().mm3e_mesh_baker_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def list_adapters() -> str:
     buf = C.create_string_buffer(4096)
     lib().mm3e_list_adapters(buf, 4096)
@@ -234,6 +378,19 @@ class Renderer:
             self.close()
         except Exception:
             pass
+
+    def bake_sdf(self, positions, triangles, resolution: int, padding: float):
+        """mesh::bake_sdf on this renderer's GPU -> (layout, samples[nz, ny, nx])."""
+        pos, tri = _mesh_arrays(positions, triangles)
+        lay = sdf_layout(pos, tri, resolution, padding)
+        nx, ny, nz = lay.dims
+        out = np.zeros((nz, ny, nx), np.float32)
+        err = C.create_string_buffer(1024)
+        rc = lib().mm3e_mesh_bake_sdf(self.h, _p(pos), len(pos), _p(tri), len(tri), int(resolution),
+                                      float(padding), _p(out), out.size, C.byref(lay), err, 1024)
+        if rc != 0:
+            raise Mm3eError(rc, err.value.decode())
+        return lay, out
 
 
 class GpuScene:

@@ -26,6 +26,10 @@ Two more legs ride on the same line (``--legs`` picks them):
            plain path's measured capacity, plus a burst; p50/p99 latency (arrival -> RGBA8 in
            host memory) and Mpix/s for the continuous-batching server and for a FIFO server
            that renders each request by a plain per-frame launch.
+  mesh_bake: the reference mesh demo's torus knot (240 x 36 ring vertices, 17,280 triangles)
+           imported from OBJ text and baked to a signed-distance grid with 128 samples along the
+           longest axis (124x63x128) on the device; ms per bake (row-parity + nearest-triangle
+           kernels, HIP events), Msamples/s, BVH tests per sample, and the CPU oracle's rate.
   stress:  (not in the default set) one 3840x2160 frame of the 1000-primitive scene with a 128^3
            sampled volume, 32^3 device-baked GI, 2 bounces and fog; ms/frame.
 """
@@ -251,14 +255,90 @@ def stress_leg(hip, configs, r, frames):
     return out
 
 
+# f32 VALU operations of one BVH box test (clamp: 3 max + 3 min; 3 sub; dot: 3 mul + 2 add) and
+# of one triangle test (the point-triangle projection's longest, interior path plus the squared
+# distance), as the bake kernel computes them
+BOX_OPS = 14
+TRI_OPS = 79
+
+
+def mesh_bake_leg(hip, configs, r, reps, resolution=128, padding=0.35, cpu=True):
+    """mesh::bake_sdf of the mesh demo's torus knot on the device, timed over `reps` bakes."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    pos0, tri0 = configs.torus_knot_mesh()
+    obj = configs.mesh_to_obj(pos0, tri0, "MM3E torus knot")
+    t0 = time.perf_counter()
+    pos, tri = hip.parse_obj(obj)
+    parse_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b = hip.MeshBaker(r, pos, tri, resolution, padding)
+    create_s = time.perf_counter() - t0
+    dims = tuple(b.layout.dims)
+    n = int(np.prod(dims))
+    b.run()
+    b.wait()
+    par, near = [], []
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.run()
+        a, c = b.wait()
+        par.append(a)
+        near.append(c)
+    wall = (time.perf_counter() - t0) / reps
+    vol = b.read()
+    info = b.info()
+    b.close()
+    ms = (sum(par) + sum(near)) / reps
+    out = {"workload": f"torus knot p=2 q=3, {len(tri)} triangles from OBJ text, resolution {resolution}, "
+                       f"padding {padding}",
+           "dims": list(dims), "samples": n, "bvh_depth": info["depth"], "reps": reps,
+           "ms_per_bake": round(ms, 4), "parity_ms": round(sum(par) / reps, 4),
+           "nearest_ms": round(sum(near) / reps, 4), "wall_ms_per_bake": round(wall * 1e3, 3),
+           "msamples_s": round(n / (ms * 1e-3) / 1e6, 1), "parse_obj_ms": round(parse_s * 1e3, 2),
+           "create_ms": round(create_s * 1e3, 2), "inside_fraction": round(float((vol < 0).mean()), 5)}
+    if cpu:
+        import pyoracle as po
+        # the BVH work per sample, counted by the oracle on two slabs of grid points
+        lay = b.layout
+        k = dims[2] // 2
+        jj, ii = np.meshgrid(np.arange(dims[1]), np.arange(dims[0]), indexing="ij")
+        pts = []
+        for kk in (k, k + 1):
+            pts.append(np.stack([np.float32(lay.min[0]) + ii.astype(np.float32) * np.float32(lay.cell[0]),
+                                 np.float32(lay.min[1]) + jj.astype(np.float32) * np.float32(lay.cell[1]),
+                                 np.full(ii.shape, np.float32(lay.min[2]) + np.float32(kk) * np.float32(lay.cell[2]),
+                                         np.float32)], -1).reshape(-1, 3))
+        pts = np.concatenate(pts)
+        _, (boxes, tris) = po.mesh_nearest_sq(pos, tri, pts)
+        per_box, per_tri = boxes / len(pts), tris / len(pts)
+        ops = (per_box * BOX_OPS + per_tri * TRI_OPS) * n
+        out["box_tests_per_sample"] = round(per_box, 1)
+        out["tri_tests_per_sample"] = round(per_tri, 1)
+        out["nearest_valu_tflops"] = round(ops / (sum(near) / reps * 1e-3) / 1e12, 3)
+        out["nearest_valu_frac"] = round(out["nearest_valu_tflops"] / PEAK_FP32_NONPACKED_TFLOPS, 4)
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        slabs = min(32, dims[2])
+        t0 = time.perf_counter()
+        po.mesh_bake(pos, tri, resolution, padding, threads=threads, k0=0, k1=slabs)
+        cdt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(slabs * dims[0] * dims[1] / cdt / 1e6, 3), "unit": "Msamples/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"oracle bake of z-slabs 0..{slabs} ({slabs * dims[0] * dims[1]} samples) "
+                                         f"in {cdt:.2f} s"}
+    log(f"mesh_bake: {dims} in {out['ms_per_bake']} ms ({out['msamples_s']} Msamples/s)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="plain,batched,served",
-                    help="comma list of plain (always run), batched, served, stress")
+    ap.add_argument("--legs", default="plain,batched,served,mesh_bake",
+                    help="comma list of plain (always run), batched, served, mesh_bake, stress")
+    ap.add_argument("--bake-reps", type=int, default=20)
     ap.add_argument("--stress-frames", type=int, default=3)
     ap.add_argument("--batched-reps", type=int, default=3)
     ap.add_argument("--served-requests", type=int, default=48)
@@ -356,6 +436,8 @@ def main():
         out["batched"] = batched_leg(hip, configs, r, args.batched_reps)
     if "served" in legs:
         out["served"] = served_leg(hip, configs, r, args.served_requests)
+    if "mesh_bake" in legs:
+        out["mesh_bake"] = mesh_bake_leg(hip, configs, r, args.bake_reps, cpu=not args.no_cpu)
     if "stress" in legs:
         out["stress"] = stress_leg(hip, configs, r, args.stress_frames)
     print(json.dumps(out), flush=True)

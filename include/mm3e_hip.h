@@ -166,6 +166,61 @@ int mm3e_scene_parse_text(const char* text, size_t text_len, uint8_t* ir, size_t
 int mm3e_scene_to_text(const void* scene_ir, size_t ir_len, const mm3e_camera* cam, char* text, size_t* text_len,
                        char* err, size_t errlen);
 
+/* ---- meshes and sampled SDF volumes --------------------------------------------------------
+ * mm3e-orchestrator/src/mesh.rs (OBJ import, mesh -> SDF bake) and mm3e-kit/src/volume.rs
+ * (SdfVolume, the `.sdfv` codec).  A mesh travels as positions[n_positions][3] (f32) and
+ * triangles[n_triangles][3] (u32 vertex indices); a volume as an mm3e_sdf_layout plus
+ * dims[0]*dims[1]*dims[2] f32 samples at min + (i,j,k)*cell, index (k*ny + j)*nx + i. */
+typedef struct mm3e_sdf_layout {
+  uint32_t dims[3]; /* SdfVolume::dims (volume.rs:16) */
+  float min[3];     /* SdfVolume::min  (volume.rs:18) */
+  float cell[3];    /* SdfVolume::cell (volume.rs:20) */
+} mm3e_sdf_layout;
+
+/* parse_obj (mesh.rs:57-103, then Mesh::validate, mesh.rs:33-52): `v` and `f` lines, faces
+ * fan-triangulated, `v/vt/vn` forms and negative indices accepted, other directives ignored;
+ * errors carry the reference's messages ("line N: ...").  With positions and triangles both
+ * NULL only the counts are returned; otherwise the capacities (in vertices / triangles) must
+ * hold them. */
+int mm3e_mesh_parse_obj(const char* text, size_t text_len, float* positions, size_t pos_cap, uint32_t* triangles,
+                        size_t tri_cap, size_t* n_positions, size_t* n_triangles, char* err, size_t errlen);
+/* The grid bake_sdf would produce (mesh.rs:265-279): bounds padded by `padding`, `resolution`
+ * (clamped to 2..=512) samples along the longest axis.  Host only; same errors as the bake. */
+int mm3e_mesh_sdf_layout(const float* positions, size_t n_positions, const uint32_t* triangles, size_t n_triangles,
+                         uint32_t resolution, float padding, mm3e_sdf_layout* layout, char* err, size_t errlen);
+/* bake_sdf (mesh.rs:265-336) on the renderer's GPU: unsigned distance to the nearest triangle,
+ * negative where a +x ray from the sample crosses the mesh an odd number of times.  The samples
+ * equal the reference's CPU bake bit for bit.  data_cap (floats) must hold the layout's sample
+ * count (query it with mm3e_mesh_sdf_layout). */
+int mm3e_mesh_bake_sdf(mm3e_renderer* r, const float* positions, size_t n_positions, const uint32_t* triangles,
+                       size_t n_triangles, uint32_t resolution, float padding, float* data, size_t data_cap,
+                       mm3e_sdf_layout* layout, char* err, size_t errlen);
+/* The same bake as a resident object (for repeated bakes and timing): create validates, builds
+ * the nearest-triangle BVH (mesh.rs:166-222) on the host and uploads it; run enqueues the two
+ * bake kernels on the renderer's stream; wait returns their GPU times (HIP events); read waits,
+ * copies the samples out and rejects non-finite ones as SdfVolume::new does. */
+typedef struct mm3e_mesh_baker mm3e_mesh_baker;
+int mm3e_mesh_baker_create(mm3e_renderer* r, const float* positions, size_t n_positions, const uint32_t* triangles,
+                           size_t n_triangles, uint32_t resolution, float padding, mm3e_mesh_baker** out, char* err,
+                           size_t errlen);
+int mm3e_mesh_baker_layout(const mm3e_mesh_baker* b, mm3e_sdf_layout* layout);
+int mm3e_mesh_baker_info(const mm3e_mesh_baker* b, uint32_t* n_triangles, uint32_t* n_nodes, uint32_t* depth);
+int mm3e_mesh_baker_run(mm3e_mesh_baker* b);
+int mm3e_mesh_baker_wait(mm3e_mesh_baker* b, float* parity_ms, float* nearest_ms);
+int mm3e_mesh_baker_read(mm3e_mesh_baker* b, float* data, size_t data_cap, char* err, size_t errlen);
+void mm3e_mesh_baker_destroy(mm3e_mesh_baker* b);
+
+/* SdfVolume::to_sdfv_bytes (volume.rs:115-136): `MM3ESDF1`, u32 nx ny nz, f32 min xyz, f32 cell
+ * xyz, then the samples, all little-endian.  The volume is validated first (volume.rs:33-85).
+ * With out == NULL or *out_len too small, *out_len is set to the size needed and
+ * MM3E_ERR_INVALID is returned. */
+int mm3e_sdfv_encode(const mm3e_sdf_layout* layout, const float* data, size_t n_samples, uint8_t* out,
+                     size_t* out_len, char* err, size_t errlen);
+/* SdfVolume::from_sdfv_bytes (volume.rs:138-174), with its messages.  data == NULL: validate
+ * and return the layout only; otherwise data_cap (floats) must hold the samples. */
+int mm3e_sdfv_decode(const uint8_t* bytes, size_t len, mm3e_sdf_layout* layout, float* data, size_t data_cap,
+                     char* err, size_t errlen);
+
 /* ---- batched rendering ----------------------------------------------------------------
  * Many (compiled scene, camera, size) jobs rendered by one persistent launch that pulls
  * tiles from a global atomic queue, followed by one batched bloom/tonemap over all jobs;

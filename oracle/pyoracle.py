@@ -65,6 +65,13 @@ def lib():
         L.orc_phase_count.restype = C.c_int
         L.orc_to_bmp.argtypes = [P, C.c_uint32, C.c_uint32, P]
         L.orc_to_bmp.restype = C.c_size_t
+        Z = C.c_size_t
+        L.orc_mesh_parse_obj.argtypes = [C.c_char_p, Z, P, Z, P, Z, C.POINTER(Z), C.POINTER(Z), C.c_char_p, Z]
+        L.orc_x_crossing.argtypes = [P, P, P, C.c_float, C.c_float, P]
+        L.orc_closest_on_triangle.argtypes = [P, P, P, P, P]
+        L.orc_closest_on_triangle.restype = None
+        L.orc_mesh_nearest_sq.argtypes = [P, Z, P, Z, P, Z, C.c_int, P, P]
+        L.orc_mesh_bake.argtypes = [P, Z, P, Z, Z, C.c_float, C.c_uint, Z, Z, P, P, P, P, Z, C.c_char_p, Z]
         _lib = L
     return _lib
 
@@ -216,3 +223,80 @@ def to_bmp(rgba):
     out = np.zeros(n, np.uint8)
     lib().orc_to_bmp(_p(rgba), w, h, _p(out))
     return out.tobytes()
+
+
+# ---- mesh -> SDF bake (mesh_oracle.cpp) -----------------------------------------------------
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _mesh(pos, tri):
+    return f32(np.asarray(pos).reshape(-1, 3)), np.ascontiguousarray(np.asarray(tri).reshape(-1, 3), np.uint32)
+
+
+def mesh_parse_obj(text: str):
+    b = text.encode()
+    n, t = C.c_size_t(0), C.c_size_t(0)
+    err = C.create_string_buffer(1024)
+    if lib().orc_mesh_parse_obj(b, len(b), None, 0, None, 0, C.byref(n), C.byref(t), err, 1024) != 0:
+        raise OracleError(err.value.decode())
+    pos = np.zeros((n.value, 3), np.float32)
+    tri = np.zeros((t.value, 3), np.uint32)
+    lib().orc_mesh_parse_obj(b, len(b), _p(pos), n.value, _p(tri), t.value, C.byref(n), C.byref(t), err, 1024)
+    return pos, tri
+
+
+def x_crossing(a, b, c, y, z):
+    """mm3e-kit triangle::x_crossing: the x of the +x line at (y, z) through the triangle, or None."""
+    x = np.zeros(1, np.float32)
+    hit = lib().orc_x_crossing(_p(f32(a)), _p(f32(b)), _p(f32(c)), float(y), float(z), _p(x))
+    return float(x[0]) if hit else None
+
+
+def closest_on_triangle(p, a, b, c):
+    out = np.zeros(3, np.float32)
+    lib().orc_closest_on_triangle(_p(f32(p)), _p(f32(a)), _p(f32(b)), _p(f32(c)), _p(out))
+    return out
+
+
+def mesh_nearest_sq(pos, tri, pts, brute=False):
+    """Squared distances to the nearest non-degenerate triangle; with brute=False through the BVH,
+    also returning (box tests, triangle tests)."""
+    pos, tri = _mesh(pos, tri)
+    pts = f32(np.asarray(pts).reshape(-1, 3))
+    out = np.zeros(len(pts), np.float32)
+    counts = np.zeros(2, np.uint64)
+    if lib().orc_mesh_nearest_sq(_p(pos), len(pos), _p(tri), len(tri), _p(pts), len(pts), 1 if brute else 0,
+                                 _p(out), _p(counts)) != 0:
+        raise OracleError("invalid mesh")
+    return out, (int(counts[0]), int(counts[1]))
+
+
+def mesh_bake(pos, tri, resolution, padding, threads=8, k0=0, k1=0):
+    """mesh::bake_sdf (z-slabs k0..k1, all when k1 == 0) -> (dims, lo, cell, samples[k1-k0, ny, nx])."""
+    pos, tri = _mesh(pos, tri)
+    dims = np.zeros(3, np.uint32)
+    lo = np.zeros(3, np.float32)
+    cell = np.zeros(3, np.float32)
+    err = C.create_string_buffer(1024)
+    # layout first (one slab), then the requested slabs
+    if lib().orc_mesh_bake(_p(pos), len(pos), _p(tri), len(tri), int(resolution), float(padding), 1, 0, 1,
+                           _p(dims), _p(lo), _p(cell), None, 0, err, 1024) != 0:
+        raise OracleError(err.value.decode())
+    nx, ny, nz = (int(v) for v in dims)
+    kk1 = nz if k1 == 0 else min(int(k1), nz)
+    out = np.zeros((max(kk1 - int(k0), 0), ny, nx), np.float32)
+    if lib().orc_mesh_bake(_p(pos), len(pos), _p(tri), len(tri), int(resolution), float(padding), int(threads),
+                           int(k0), kk1, _p(dims), _p(lo), _p(cell), _p(out), out.size, err, 1024) != 0:
+        raise OracleError(err.value.decode())
+    return tuple(int(v) for v in dims), lo, cell, out
+
+
+def volume_sample(dims, mn, cell, data, p):
+    """SdfVolume::sample (mm3e-kit/src/volume.rs:194-228) at local point p."""
+    d = np.asarray(dims, np.uint32)
+    a = f32(mn)
+    c = f32(cell)
+    v = f32(np.asarray(data).reshape(-1))
+    return lib().orc_volume_sample(_p(d), _p(a), _p(c), _p(v), v.size, _p(f32(p)))
