@@ -8,7 +8,9 @@
 //   gather_rays      GI bake rays (lib.rs:686-711): one lane per (probe, face, ray).
 //   gi_reduce        weighted mean per cube face (gi.rs:327-341).
 //   bloom_h / bloom_v_resolve / resolve_only   post.rs:36-115 + framebuffer.rs:102-113.
-//   primary_aov      centre-pixel primary hit (film.rs:114-118, 164-165).
+//   film_tile        render_film's samples through a thin lens: beauty, premultiplied
+//                    foreground, coverage alpha (film.rs:63-225).
+//   primary_aov      centre-pixel primary hit: mask, material, depth, normal (film.rs:114-122).
 //   aov_mode_pixel   debug render modes Normal / Depth / Ao / Steps / Albedo (lib.rs:1178-1218).
 //   field_probe / march_probe   map / raw-march parity probes (wgsl.rs:641-646 twin).
 #pragma once
@@ -69,9 +71,18 @@ MM3E_DEV V3 shade_surface(const DevScene& sc, const Map& field, const Hit& h, V3
 // level is a separate inlined copy and no per-level state goes to scratch:
 // out = fog(base*(1-fr) + trace(reflected)*fr + emissive).
 template <class Map, int D, int MAXB>
+MM3E_DEV V3 shade_level(const DevScene& sc, const Map& field, const Hit& h, V3 rd);
+
+template <class Map, int D, int MAXB>
 MM3E_DEV V3 trace_level(const DevScene& sc, const Map& field, V3 ro, V3 rd) {
   Hit h = march(sc, field, ro, rd);
   if (!h.hit) return sky(rd, ld3(sc.sun));
+  return shade_level<Map, D, MAXB>(sc, field, h, rd);
+}
+
+// shade_hit (lib.rs:1286-1360) for a confirmed hit at recursion depth D.
+template <class Map, int D, int MAXB>
+MM3E_DEV V3 shade_level(const DevScene& sc, const Map& field, const Hit& h, V3 rd) {
   V3 n = hit_normal(sc, field, h.pos);
   const DMat& m = sc.mat[h.mat % sc.n_mat];
   V3 radiance = shade_surface(sc, field, h, n, rd, m);
@@ -311,8 +322,9 @@ MM3E_DEV void path_transition(const DevScene& sc, const Tables& tb, PathState& p
 #define MM3E_PATH_GATE 1
 #endif
 
-template <class Map, int MAXB>
-MM3E_DEV V3 trace_sm(const DevScene& sc, const Map& field, V3 ro, V3 rd) {
+// HIT0 also reports whether the primary (depth-0) ray hit a surface, in *hit0.
+template <class Map, int MAXB, bool HIT0 = false>
+MM3E_DEV V3 trace_sm(const DevScene& sc, const Map& field, V3 ro, V3 rd, bool* hit0 = nullptr) {
   PathState ps;
   LevelStack<MAXB> stk;
   ps.level = 0;
@@ -325,6 +337,9 @@ MM3E_DEV V3 trace_sm(const DevScene& sc, const Map& field, V3 ro, V3 rd) {
     const uint64_t search_mask = __builtin_amdgcn_ballot_w64(ps.phase != PH_DONE && !waiting);
     if (wait_mask != 0 && (search_mask == 0 || __builtin_popcountll(wait_mask) >= MM3E_PATH_GATE)) {
       if (waiting) {
+        if constexpr (HIT0) {
+          if (ps.phase == PH_MARCH && ps.level == 0) *hit0 = ps.h.hit;
+        }
         path_transition<MAXB>(sc, field.tb, ps, stk);
         // the only search that can finish before its first evaluation is a march with
         // max_steps == 0; every other search either asks for a point or completes inside
@@ -384,6 +399,145 @@ MM3E_DEV void render_tile(const DevScene* __restrict__ scp, const Tables& tb, co
     acc = scale(acc, inv);
     float* o = hdr + 3 * ((size_t)py * fp.width + px);
     o[0] = acc.x; o[1] = acc.y; o[2] = acc.z;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Film: beauty, premultiplied foreground and coverage alpha through a thin lens
+// (film.rs:63-225, camera.rs:71-102)
+// ---------------------------------------------------------------------------------------
+MM3E_DEV uint32_t film_hash(uint32_t v) {  // film.rs:218-224
+  v ^= v >> 16;
+  v *= 0x7feb352du;
+  v ^= v >> 15;
+  v *= 0x846ca68bu;
+  return v ^ (v >> 16);
+}
+
+// One jittered aperture sample per grid cell, cells permuted per pixel (film.rs:200-216).
+MM3E_DEV void aperture_sample(uint32_t px, uint32_t py, uint32_t sx, uint32_t sy, uint32_t n, float& ax, float& ay) {
+  if (n == 1) {
+    ax = 0.5f;
+    ay = 0.5f;
+    return;
+  }
+  const uint32_t pixel_seed = film_hash(px ^ film_hash(py));
+  const uint64_t n64 = n;
+  const uint32_t cx = (uint32_t)(((uint64_t)sx + (uint64_t)sy + (uint64_t)pixel_seed) % n64);
+  const uint32_t cy = (uint32_t)(((uint64_t)sx + 2ull * (uint64_t)sy + (uint64_t)film_hash(pixel_seed)) % n64);
+  const uint32_t seed = film_hash(pixel_seed ^ film_hash(sx) ^ film_hash(sy + 0xa511e9b3u));
+  const float unit = 1.0f / 16777216.0f;
+  ax = ((float)cx + (float)(seed >> 8) * unit) / (float)n;
+  ay = ((float)cy + (float)(film_hash(seed) >> 8) * unit) / (float)n;
+}
+
+// Equal-area concentric square-to-disk map (camera.rs:105-118).
+MM3E_DEV void concentric_disk(float s0, float s1, float& dx, float& dy) {
+  const float x = 2.0f * s0 - 1.0f, y = 2.0f * s1 - 1.0f;
+  if (x == 0.0f && y == 0.0f) {
+    dx = 0.0f;
+    dy = 0.0f;
+    return;
+  }
+  const float quarter = 0.785398163397448309616f, half = 1.57079632679489661923f;
+  float r, th;
+  if (fabsf(x) > fabsf(y)) {
+    r = x;
+    th = quarter * (y / x);
+  } else {
+    r = y;
+    th = half - quarter * (x / y);
+  }
+  // cos and sin of an f32 angle, computed in f64 and rounded once: the correctly rounded f32
+  // value but for rare double roundings, as the reference's libm cosf / sinf return it
+  dx = r * (float)cos((double)th);
+  dy = r * (float)sin((double)th);
+}
+
+// Camera::ray_with_lens (camera.rs:71-102): the pinhole ray when the radius is zero, else a ray
+// from a point of the lens disk through the pinhole ray's point on the focus plane.
+MM3E_DEV void lens_ray(const FrameParams& fp, const FilmParams& lp, float sx, float sy, float ax, float ay, V3& ro,
+                       V3& rd) {
+  camera_ray(fp, sx, sy, ro, rd);
+  if (lp.radius == 0.0f) return;
+  float dx, dy;
+  concentric_disk(ax, ay, dx, dy);
+  const V3 eye = ld3(fp.eye);
+  const V3 origin = eye + scale(ld3(fp.right), dx * lp.radius) + scale(ld3(fp.up), dy * lp.radius);
+  const V3 focus = eye + scale(rd, lp.focus / dot(rd, ld3(fp.fwd)));
+  ro = origin;
+  rd = normalize(focus - origin);
+}
+
+// trace (lib.rs:1265-1282) that also reports whether the ray hit a surface.
+template <class Map, int MAXB>
+MM3E_DEV V3 trace_hit(const DevScene& sc, const Map& field, V3 ro, V3 rd, bool& hit) {
+  Hit h = march(sc, field, ro, rd);
+  hit = h.hit;
+  if (!h.hit) return sky(rd, ld3(sc.sun));
+  return shade_level<Map, 0, MAXB>(sc, field, h, rd);
+}
+
+// sample_pixel (film.rs:140-198) over one tile: lane -> (pixel, sample) as render_tile; the
+// first lane of each pixel sums its samples in sy-major order: beauty over every sample,
+// foreground over the samples whose primary ray hit, alpha = hits / n^2.
+template <class Map, int MAXB, bool SM>
+MM3E_DEV void film_tile(const DevScene* __restrict__ scp, const Tables& tb, const FrameParams& fp,
+                        const FilmParams& lp, float* __restrict__ beauty, float* __restrict__ fg,
+                        float* __restrict__ alpha, uint32_t tw, uint32_t th, uint32_t tile_x, uint32_t tile_y,
+                        float* smem) {
+  const DevScene& sc = *scp;
+  Map field{scp, tb};
+  const uint32_t aa = sc.aa;
+  const uint32_t ns = aa * aa;
+  const uint32_t t = threadIdx.x;
+  const uint32_t pl = t / ns, s = t - pl * ns;
+  const uint32_t px = tile_x * tw + pl % tw, py = tile_y * th + pl / tw;
+  const bool valid = pl < tw * th && px < fp.width && py < fp.height;
+  V3 rad = V3{0.0f, 0.0f, 0.0f};
+  bool hit = false;
+  if (valid) {
+    V3 ro, rd;
+    if (ns == 1) {
+      camera_ray(fp, (float)px + 0.5f, (float)py + 0.5f, ro, rd);  // the centre ray
+    } else {
+      const uint32_t sy = s / aa, sx = s - sy * aa;
+      const float ix = (float)px + ((float)sx + 0.5f) / (float)aa;
+      const float iy = (float)py + ((float)sy + 0.5f) / (float)aa;
+      float ax, ay;
+      aperture_sample(px, py, sx, sy, aa, ax, ay);
+      lens_ray(fp, lp, ix, iy, ax, ay, ro, rd);
+    }
+    if constexpr (SM) rad = trace_sm<Map, MAXB, true>(sc, field, ro, rd, &hit);
+    else rad = trace_hit<Map, MAXB>(sc, field, ro, rd, hit);
+  }
+  smem[4 * t + 0] = rad.x;
+  smem[4 * t + 1] = rad.y;
+  smem[4 * t + 2] = rad.z;
+  smem[4 * t + 3] = hit ? 1.0f : 0.0f;
+  __syncthreads();
+  if (valid && s == 0) {
+    V3 b = V3{0.0f, 0.0f, 0.0f}, f = V3{0.0f, 0.0f, 0.0f};
+    uint32_t hits = 0;
+    for (uint32_t k = 0; k < ns; k++) {
+      const float* q = smem + 4 * (t + k);
+      const V3 r = V3{q[0], q[1], q[2]};
+      b = b + r;
+      if (q[3] != 0.0f) {
+        f = f + r;
+        hits++;
+      }
+    }
+    const float inv = 1.0f / (float)ns;
+    const size_t i = (size_t)py * fp.width + px;
+    const V3 bo = scale(b, inv), fo = scale(f, inv);
+    beauty[3 * i] = bo.x;
+    beauty[3 * i + 1] = bo.y;
+    beauty[3 * i + 2] = bo.z;
+    fg[3 * i] = fo.x;
+    fg[3 * i + 1] = fo.y;
+    fg[3 * i + 2] = fo.z;
+    alpha[i] = (float)hits * inv;
   }
 }
 
@@ -586,7 +740,8 @@ MM3E_DEV void aov_mode_pixel(const DevScene* __restrict__ scp, const FrameParams
 
 template <class Map>
 MM3E_DEV void primary_aov(const DevScene* __restrict__ scp, const FrameParams& fp, uint8_t* __restrict__ hit,
-                          uint32_t* __restrict__ mat, float* __restrict__ depth, uint32_t* __restrict__ steps) {
+                          uint32_t* __restrict__ mat, float* __restrict__ depth, uint32_t* __restrict__ steps,
+                          float* __restrict__ nrm) {
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
   if (x >= fp.width) return;
   Map field{scp, global_tables(scp)};
@@ -598,6 +753,12 @@ MM3E_DEV void primary_aov(const DevScene* __restrict__ scp, const FrameParams& f
   mat[i] = h.hit ? h.mat : 0xffffffffu;
   depth[i] = h.hit ? dot(h.pos - ld3(fp.eye), ld3(fp.fwd)) : __builtin_inff();
   if (steps) steps[i] = h.steps;
+  if (nrm) {  // the hit's world normal, zero on a miss (film.rs:34-35, 119-122)
+    const V3 n = h.hit ? hit_normal(*scp, field, h.pos) : V3{0.0f, 0.0f, 0.0f};
+    nrm[3 * i] = n.x;
+    nrm[3 * i + 1] = n.y;
+    nrm[3 * i + 2] = n.z;
+  }
 }
 
 template <class Map>

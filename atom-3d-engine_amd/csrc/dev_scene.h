@@ -202,4 +202,10 @@ struct FrameParams {
   float aspect, pad3;
 };
 
+// Thin lens of a film render (Lens, mm3e-kit/src/camera.rs:14-35).
+struct FilmParams {
+  float radius, focus;
+  uint32_t pad0, pad1;
+};
+
 }  // namespace mm3e

@@ -77,8 +77,9 @@ extern "C" __global__ void mm3e_batch_reset(uint32_t* counter) {
 
 extern "C" __global__ void __launch_bounds__(256)
     mm3e_primary_aov_table(const DevScene* __restrict__ sc, FrameParams fp, uint8_t* __restrict__ hit,
-                           uint32_t* __restrict__ mat, float* __restrict__ depth, uint32_t* __restrict__ steps) {
-  primary_aov<TableMap>(sc, fp, hit, mat, depth, steps);
+                           uint32_t* __restrict__ mat, float* __restrict__ depth, uint32_t* __restrict__ steps,
+                           float* __restrict__ nrm) {
+  primary_aov<TableMap>(sc, fp, hit, mat, depth, steps, nrm);
 }
 
 extern "C" __global__ void __launch_bounds__(256)
@@ -113,6 +114,19 @@ hipError_t launch_render_table(uint32_t maxb, dim3 grid, dim3 block, hipStream_t
     case 2: return launch_render_table_b2(grid, block, st, sc, stage_bytes, general, fp, hdr, tw, th);
     case 3: return launch_render_table_b3(grid, block, st, sc, stage_bytes, general, fp, hdr, tw, th);
     case 4: return launch_render_table_b4(grid, block, st, sc, stage_bytes, general, fp, hdr, tw, th);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_film(uint32_t maxb, dim3 grid, dim3 block, hipStream_t st, const DevScene* sc, bool general,
+                       const FrameParams& fp, const FilmParams& lp, float* beauty, float* fg, float* alpha,
+                       uint32_t tw, uint32_t th) {
+  switch (maxb) {
+    case 0: return launch_film_b0(grid, block, st, sc, general, fp, lp, beauty, fg, alpha, tw, th);
+    case 1: return launch_film_b1(grid, block, st, sc, general, fp, lp, beauty, fg, alpha, tw, th);
+    case 2: return launch_film_b2(grid, block, st, sc, general, fp, lp, beauty, fg, alpha, tw, th);
+    case 3: return launch_film_b3(grid, block, st, sc, general, fp, lp, beauty, fg, alpha, tw, th);
+    case 4: return launch_film_b4(grid, block, st, sc, general, fp, lp, beauty, fg, alpha, tw, th);
     default: return hipErrorInvalidValue;
   }
 }
@@ -177,9 +191,9 @@ hipError_t launch_batch_render(uint32_t maxb, uint32_t blocks, hipStream_t st, c
 }
 
 hipError_t launch_primary_aov_table(hipStream_t st, const DevScene* sc, const FrameParams& fp, uint8_t* hit,
-                                    uint32_t* mat, float* depth, uint32_t* steps) {
+                                    uint32_t* mat, float* depth, uint32_t* steps, float* nrm) {
   dim3 grid((fp.width + 255) / 256, fp.height);
-  hipLaunchKernelGGL(mm3e_primary_aov_table, grid, dim3(256), 0, st, sc, fp, hit, mat, depth, steps);
+  hipLaunchKernelGGL(mm3e_primary_aov_table, grid, dim3(256), 0, st, sc, fp, hit, mat, depth, steps, nrm);
   return hipGetLastError();
 }
 

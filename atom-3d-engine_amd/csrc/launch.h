@@ -50,7 +50,22 @@ hipError_t launch_gi_reduce(dim3 grid, dim3 block, hipStream_t st, const GiBakeP
 hipError_t launch_post(hipStream_t st, const PostParams& pp, const float* kern, const float* hdr, float* tmp,
                        uchar4* rgba);
 hipError_t launch_primary_aov_table(hipStream_t st, const DevScene* sc, const FrameParams& fp, uint8_t* hit,
-                                    uint32_t* mat, float* depth, uint32_t* steps);
+                                    uint32_t* mat, float* depth, uint32_t* steps, float* nrm = nullptr);
+
+#define MM3E_DECL_FILM(B)                                                                                       \
+  hipError_t launch_film_b##B(dim3 grid, dim3 block, hipStream_t st, const DevScene* sc, bool general,          \
+                              const FrameParams& fp, const FilmParams& lp, float* beauty, float* fg, float* alpha, \
+                              uint32_t tw, uint32_t th);
+MM3E_DECL_FILM(0)
+MM3E_DECL_FILM(1)
+MM3E_DECL_FILM(2)
+MM3E_DECL_FILM(3)
+MM3E_DECL_FILM(4)
+#undef MM3E_DECL_FILM
+// render_film's sample pass for a scene with `maxb` bounces (0..4).
+hipError_t launch_film(uint32_t maxb, dim3 grid, dim3 block, hipStream_t st, const DevScene* sc, bool general,
+                       const FrameParams& fp, const FilmParams& lp, float* beauty, float* fg, float* alpha,
+                       uint32_t tw, uint32_t th);
 // Debug render mode (lib.rs:1178-1218) straight into RGBA8; mode 1..5.
 hipError_t launch_aov_mode_table(hipStream_t st, const DevScene* sc, const FrameParams& fp, uchar4* rgba,
                                  uint32_t mode);

@@ -663,6 +663,60 @@ int mm3e_scene_render_aov(mm3e_scene* s, const mm3e_camera* cam, uint8_t* hit, u
   return ok ? MM3E_OK : MM3E_ERR_DEVICE;
 }
 
+int mm3e_scene_render_film(mm3e_scene* s, const mm3e_camera* cam, const mm3e_lens* lens, float* beauty,
+                           float* foreground, float* alpha, float* depth, float* normals, uint32_t* material_ids,
+                           char* err, size_t errlen) {
+  if (!s || validate_cam(cam) || !lens) return MM3E_ERR_INVALID;
+  // Lens::validate (camera.rs:25-34)
+  if (!std::isfinite(lens->aperture_radius) || lens->aperture_radius < 0.0f) {
+    set_err(err, errlen, "aperture_radius must be finite and nonnegative");
+    return MM3E_ERR_INVALID;
+  }
+  if (!std::isfinite(lens->focus_distance) || lens->focus_distance <= 0.0f) {
+    set_err(err, errlen, "focus_distance must be finite and positive");
+    return MM3E_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lk(s->r->mu);
+  hipSetDevice(s->r->device);
+  const size_t n = (size_t)s->w * s->h;
+  hipStream_t st = s->r->stream;
+  // one allocation: beauty[3n], foreground[3n], alpha[n], depth[n], normals[3n], ids[n], hit[n]
+  uint8_t* d = nullptr;
+  std::string msg;
+  bool ok = hip_ok(hipMalloc(&d, n * (4 * 12 + 1)), "hipMalloc(film)", &msg);
+  float* d_beauty = (float*)d;
+  float* d_fg = d_beauty + 3 * n;
+  float* d_alpha = d_fg + 3 * n;
+  float* d_depth = d_alpha + n;
+  float* d_nrm = d_depth + n;
+  uint32_t* d_ids = (uint32_t*)(d_nrm + 3 * n);
+  uint8_t* d_hit = (uint8_t*)(d_ids + n);
+  FrameParams fp = make_frame(*cam, s->w, s->h);
+  FilmParams lp{lens->aperture_radius, lens->focus_distance, 0u, 0u};
+  dim3 grid((s->w + s->tw - 1) / s->tw, (s->h + s->th - 1) / s->th);
+  dim3 block(s->tw * s->th * s->aa * s->aa);
+  if (ok)
+    ok = hip_ok(launch_film(s->bounces, grid, block, st, s->d_scene, s->general, fp, lp, d_beauty, d_fg, d_alpha,
+                            s->tw, s->th),
+                "film launch", &msg);
+  if (ok) ok = hip_ok(launch_primary_aov_table(st, s->d_scene, fp, d_hit, d_ids, d_depth, nullptr, d_nrm), "aov", &msg);
+  if (ok) ok = hip_ok(hipStreamSynchronize(st), "film", &msg);
+  struct Out {
+    void* host;
+    const void* dev;
+    size_t bytes;
+  } outs[] = {{beauty, d_beauty, n * 12}, {foreground, d_fg, n * 12}, {alpha, d_alpha, n * 4},
+              {depth, d_depth, n * 4},   {normals, d_nrm, n * 12},   {material_ids, d_ids, n * 4}};
+  for (const Out& o : outs)
+    if (ok && o.host) ok = hip_ok(hipMemcpy(o.host, o.dev, o.bytes, hipMemcpyDeviceToHost), "readback", &msg);
+  hipFree(d);
+  if (!ok) {
+    set_err(err, errlen, msg);
+    return MM3E_ERR_DEVICE;
+  }
+  return MM3E_OK;
+}
+
 int mm3e_scene_gi_raw(mm3e_scene* s, float* min3, float* size3, uint32_t* dims3, float* cubes, uint64_t* n_probes) {
   if (!s) return MM3E_ERR_INVALID;
   if (!s->has_gi) {

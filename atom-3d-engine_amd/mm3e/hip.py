@@ -32,6 +32,7 @@ EXPORTS = (
     "mm3e_mesh_parse_obj", "mm3e_mesh_sdf_layout", "mm3e_mesh_bake_sdf", "mm3e_mesh_baker_create",
     "mm3e_mesh_baker_layout", "mm3e_mesh_baker_info", "mm3e_mesh_baker_run", "mm3e_mesh_baker_wait",
     "mm3e_mesh_baker_read", "mm3e_mesh_baker_destroy", "mm3e_sdfv_encode", "mm3e_sdfv_decode",
+    "mm3e_scene_render_film",
 )
 
 CODEGEN_OFF, CODEGEN_BACKGROUND, CODEGEN_SYNC = 0, 1, 2
@@ -117,6 +118,7 @@ def lib():
         L.mm3e_mesh_baker_destroy.restype = None
         L.mm3e_sdfv_encode.argtypes = [P, P, Z, P, C.POINTER(Z), C.c_char_p, Z]
         L.mm3e_sdfv_decode.argtypes = [P, Z, P, P, Z, C.c_char_p, Z]
+        L.mm3e_scene_render_film.argtypes = [P, P, P, P, P, P, P, P, P, C.c_char_p, Z]
         _lib = L
     return _lib
 
@@ -453,6 +455,22 @@ class GpuScene:
         if rc != 0:
             raise Mm3eError(rc, "render_aov failed")
         return hit, mat, depth, steps
+
+    def render_film(self, cam, aperture_radius=0.0, focus_distance=1.0):
+        """render_film (film.rs:63-198) on the device -> dict of beauty, foreground, alpha, depth,
+        normals, material_ids as [h, w(, 3)] arrays."""
+        h, w = self.height, self.width
+        out = {"beauty": np.zeros((h, w, 3), np.float32), "foreground": np.zeros((h, w, 3), np.float32),
+               "alpha": np.zeros((h, w), np.float32), "depth": np.zeros((h, w), np.float32),
+               "normals": np.zeros((h, w, 3), np.float32), "material_ids": np.zeros((h, w), np.uint32)}
+        lens = (C.c_float * 2)(aperture_radius, focus_distance)
+        err = C.create_string_buffer(1024)
+        rc = lib().mm3e_scene_render_film(self.h, _p(_cam(cam)), C.byref(lens),
+                                          *(_p(out[k]) for k in ("beauty", "foreground", "alpha", "depth", "normals",
+                                                                 "material_ids")), err, 1024)
+        if rc != 0:
+            raise Mm3eError(rc, err.value.decode())
+        return out
 
     def last_timing(self):
         a, b = C.c_float(0), C.c_float(0)

@@ -99,6 +99,26 @@ int mm3e_scene_render_linear(mm3e_scene* s, const mm3e_camera* cam, float* out, 
  * miss), camera-forward depth (+inf on a miss), march steps.  Any pointer may be NULL. */
 int mm3e_scene_render_aov(mm3e_scene* s, const mm3e_camera* cam, uint8_t* hit, uint32_t* mat, float* depth,
                           uint32_t* steps);
+/* Thin lens (Lens, mm3e-kit/src/camera.rs:14-35): the eye is the lens centre; the focus plane
+ * is perpendicular to the camera's forward axis at focus_distance.  Radius 0 = pinhole. */
+typedef struct mm3e_lens {
+  float aperture_radius;
+  float focus_distance;
+} mm3e_lens;
+/* render_film (mm3e-orchestrator/src/film.rs:63-198): scene-linear film of the compiled scene,
+ * before any exposure or post pass, always Beauty whatever the scene's debug mode.  aa^2
+ * samples per pixel integrate the pixel footprint and the aperture (a single sample uses the
+ * lens centre); per pixel:
+ *   beauty[3]      mean of hit radiance and primary-miss sky (equal to render_linear, bit for
+ *                  bit, when the aperture radius is 0);
+ *   foreground[3]  hit radiance / samples (premultiplied by coverage);
+ *   alpha          fraction of samples whose primary ray hit a surface;
+ *   depth, normals[3], material_ids   the centre pinhole ray's hit: camera-forward depth,
+ *                  world normal, material (+inf, 0, u32::MAX on a miss).
+ * Any output may be NULL.  An invalid lens fails with Lens::validate's message. */
+int mm3e_scene_render_film(mm3e_scene* s, const mm3e_camera* cam, const mm3e_lens* lens, float* beauty,
+                           float* foreground, float* alpha, float* depth, float* normals, uint32_t* material_ids,
+                           char* err, size_t errlen);
 /* Raw GI probe volume of the compiled scene (GiVolume::raw, gi.rs:133-135): min, size, dims
  * and, when cubes != NULL, the [probe][6][3] irradiance cubes.  *n_probes = 0 if no GI. */
 int mm3e_scene_gi_raw(mm3e_scene* s, float* min3, float* size3, uint32_t* dims3, float* cubes, uint64_t* n_probes);

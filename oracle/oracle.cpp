@@ -6,7 +6,8 @@
 // Restated reference code (paths relative to the reference tree):
 //   mm3e-kit/src/{vec,sdf,dual,volume,csg,camera,march,shade,color,framebuffer}.rs, lib.rs:63-78
 //   mm3e-orchestrator/src/lib.rs:156-1435, accel.rs, gi.rs, post.rs, appearance.rs:427-431,
-//   film.rs:148-198 (primary-hit AOVs), mm3e-cpu-parallel/src/lib.rs:50-84.
+//   film.rs:63-225 (render_film, primary-hit AOVs), camera.rs:71-118 (thin lens),
+//   mm3e-cpu-parallel/src/lib.rs:50-84.
 #include <atomic>
 #include <cstdio>
 #include <cstring>
@@ -1443,6 +1444,136 @@ int orc_render_aov(const orc_scene* s, const float* cam13, unsigned threads, uin
         mat[i] = hh.hit ? hh.mat : 0xffffffffu;
         depth[i] = hh.hit ? dot(hh.pos - f3<float>(cam.eye), f3<float>(cam.forward)) : INFINITY;
         if (steps) steps[i] = hh.steps;
+      }
+  });
+  return 0;
+}
+
+// ---- film (film.rs:63-225, camera.rs:71-118) -------------------------------------------
+
+}  // extern "C"
+
+static uint32_t film_hash(uint32_t v) {  // film.rs:218-224
+  v ^= v >> 16;
+  v *= 0x7feb352du;
+  v ^= v >> 15;
+  v *= 0x846ca68bu;
+  return v ^ (v >> 16);
+}
+
+static void film_aperture_sample(uint32_t px, uint32_t py, uint32_t sx, uint32_t sy, uint32_t n, float* out) {
+  if (n == 1) {  // film.rs:203-216
+    out[0] = 0.5f;
+    out[1] = 0.5f;
+    return;
+  }
+  const uint32_t pixel_seed = film_hash(px ^ film_hash(py));
+  const uint64_t n64 = n;
+  const uint32_t cell_x = (uint32_t)(((uint64_t)sx + (uint64_t)sy + (uint64_t)pixel_seed) % n64);
+  const uint32_t cell_y = (uint32_t)(((uint64_t)sx + 2 * (uint64_t)sy + (uint64_t)film_hash(pixel_seed)) % n64);
+  const uint32_t seed = film_hash(pixel_seed ^ film_hash(sx) ^ film_hash(sy + 0xa511e9b3u));
+  auto unit = [](uint32_t v) { return (float)(v >> 8) * (1.0f / 16777216.0f); };
+  out[0] = ((float)cell_x + unit(seed)) / (float)n;
+  out[1] = ((float)cell_y + unit(film_hash(seed))) / (float)n;
+}
+
+static Ray<float> film_ray_with_lens(const Camera& cam, float ix, float iy, uint32_t w, uint32_t h, float radius,
+                                     float focus, float ax, float ay) {
+  Ray<float> pin = Renderer<float>::cam_ray(cam, ix, iy, w, h);  // camera.rs:71-102
+  if (radius == 0.0f) return pin;
+  float dx = 0.0f, dy = 0.0f;
+  {  // concentric_disk, camera.rs:105-118
+    const float x = 2.0f * ax - 1.0f, y = 2.0f * ay - 1.0f;
+    if (!(x == 0.0f && y == 0.0f)) {
+      float r, th;
+      if (std::fabs(x) > std::fabs(y)) {
+        r = x;
+        th = 0.785398163397448309616f * (y / x);
+      } else {
+        r = y;
+        th = 1.57079632679489661923f - 0.785398163397448309616f * (x / y);
+      }
+      dx = r * cosf(th);
+      dy = r * sinf(th);
+    }
+  }
+  const V3<float> eye = f3<float>(cam.eye);
+  const V3<float> origin = eye + scale(f3<float>(cam.right), dx * radius) + scale(f3<float>(cam.up), dy * radius);
+  const V3<float> foc = eye + scale(pin.d, focus / dot(pin.d, f3<float>(cam.forward)));
+  return Ray<float>{origin, normalize(foc - origin)};
+}
+
+extern "C" {
+
+void orc_ray_with_lens(const float* cam13, float ix, float iy, uint32_t w, uint32_t h, float radius, float focus,
+                       float ax, float ay, float* origin3, float* dir3) {
+  const Ray<float> r = film_ray_with_lens(to_cam(cam13), ix, iy, w, h, radius, focus, ax, ay);
+  origin3[0] = r.o.x;
+  origin3[1] = r.o.y;
+  origin3[2] = r.o.z;
+  dir3[0] = r.d.x;
+  dir3[1] = r.d.y;
+  dir3[2] = r.d.z;
+}
+
+void orc_aperture_sample(uint32_t px, uint32_t py, uint32_t sx, uint32_t sy, uint32_t n, float* out2) {
+  film_aperture_sample(px, py, sx, sy, n, out2);
+}
+
+// render_film_with_threads (film.rs:74-198).  Any output may be NULL.
+int orc_render_film(const orc_scene* s, const float* cam13, float radius, float focus, unsigned threads, float* beauty,
+                    float* fg, float* alpha, float* depth, float* normals, uint32_t* mat) {
+  const Scene& sc = s->sc;
+  const Camera cam = to_cam(cam13);
+  Renderer<float> rd(&sc);
+  const uint32_t n = std::max(sc.aa, 1u);
+  const float inv = 1.0f / (float)((uint64_t)n * (uint64_t)n);
+  for_bands(sc.height, threads, [&](uint32_t y0, uint32_t y1) {
+    for (uint32_t y = y0; y < y1; y++)
+      for (uint32_t x = 0; x < sc.width; x++) {
+        const Ray<float> cray = Renderer<float>::cam_ray(cam, (float)x + 0.5f, (float)y + 0.5f, sc.width, sc.height);
+        const Hit<float> center = rd.march(cray);
+        V3<float> b, f;
+        uint64_t hits = 0;
+        for (uint32_t sy = 0; sy < n; sy++)
+          for (uint32_t sx = 0; sx < n; sx++) {
+            const float ix = (float)x + ((float)sx + 0.5f) / (float)n;
+            const float iy = (float)y + ((float)sy + 0.5f) / (float)n;
+            float ap[2];
+            film_aperture_sample(x, y, sx, sy, n, ap);
+            const Ray<float> ray = n == 1 ? cray : film_ray_with_lens(cam, ix, iy, sc.width, sc.height, radius, focus,
+                                                                      ap[0], ap[1]);
+            const Hit<float> hit = n == 1 ? center : rd.march(ray);
+            if (hit.hit) {
+              const V3<float> r = rd.shade_hit(hit, ray, 0);
+              b = b + r;
+              f = f + r;
+              hits++;
+            } else {
+              b = b + sky(ray.d, rd.sun());
+            }
+          }
+        const size_t i = (size_t)y * sc.width + x;
+        const V3<float> bo = scale(b, inv), fo = scale(f, inv);
+        if (beauty) {
+          beauty[3 * i] = bo.x;
+          beauty[3 * i + 1] = bo.y;
+          beauty[3 * i + 2] = bo.z;
+        }
+        if (fg) {
+          fg[3 * i] = fo.x;
+          fg[3 * i + 1] = fo.y;
+          fg[3 * i + 2] = fo.z;
+        }
+        if (alpha) alpha[i] = (float)hits * inv;
+        if (depth) depth[i] = center.hit ? dot(center.pos - f3<float>(cam.eye), f3<float>(cam.forward)) : INFINITY;
+        if (normals) {
+          const V3<float> nn = center.hit ? center.normal : V3<float>();
+          normals[3 * i] = nn.x;
+          normals[3 * i + 1] = nn.y;
+          normals[3 * i + 2] = nn.z;
+        }
+        if (mat) mat[i] = center.hit ? center.mat : 0xffffffffu;
       }
   });
   return 0;

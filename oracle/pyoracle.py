@@ -65,6 +65,12 @@ def lib():
         L.orc_phase_count.restype = C.c_int
         L.orc_to_bmp.argtypes = [P, C.c_uint32, C.c_uint32, P]
         L.orc_to_bmp.restype = C.c_size_t
+        L.orc_render_film.argtypes = [P, P, C.c_float, C.c_float, C.c_uint, P, P, P, P, P, P]
+        L.orc_ray_with_lens.argtypes = [P, C.c_float, C.c_float, C.c_uint32, C.c_uint32, C.c_float, C.c_float,
+                                        C.c_float, C.c_float, P, P]
+        L.orc_ray_with_lens.restype = None
+        L.orc_aperture_sample.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P]
+        L.orc_aperture_sample.restype = None
         Z = C.c_size_t
         L.orc_mesh_parse_obj.argtypes = [C.c_char_p, Z, P, Z, P, Z, C.POINTER(Z), C.POINTER(Z), C.c_char_p, Z]
         L.orc_x_crossing.argtypes = [P, P, P, C.c_float, C.c_float, P]
@@ -141,6 +147,19 @@ class OracleScene:
         lib().orc_render_aov(self.h, _p(f32(cam)), threads or os.cpu_count(), _p(hit), _p(mat), _p(depth), _p(steps))
         sh = (self.height, self.width)
         return hit.reshape(sh), mat.reshape(sh), depth.reshape(sh), steps.reshape(sh)
+
+    def render_film(self, cam, aperture_radius=0.0, focus_distance=1.0, threads=0):
+        """render_film_with_threads (film.rs:74-198) -> dict of beauty, foreground, alpha, depth,
+        normals, material_ids as [h, w(, 3)] arrays."""
+        h, w = self.height, self.width
+        out = {"beauty": np.zeros((h, w, 3), np.float32), "foreground": np.zeros((h, w, 3), np.float32),
+               "alpha": np.zeros((h, w), np.float32), "depth": np.zeros((h, w), np.float32),
+               "normals": np.zeros((h, w, 3), np.float32), "material_ids": np.zeros((h, w), np.uint32)}
+        lib().orc_render_film(self.h, _p(f32(cam)), float(aperture_radius), float(focus_distance),
+                              threads or os.cpu_count(), *(_p(out[k]) for k in
+                                                           ("beauty", "foreground", "alpha", "depth", "normals",
+                                                            "material_ids")))
+        return out
 
     def field(self, pts, linear=False):
         pts = f32(pts).reshape(-1, 3)
@@ -300,3 +319,20 @@ def volume_sample(dims, mn, cell, data, p):
     c = f32(cell)
     v = f32(np.asarray(data).reshape(-1))
     return lib().orc_volume_sample(_p(d), _p(a), _p(c), _p(v), v.size, _p(f32(p)))
+
+
+# ---- thin lens (camera.rs:71-118, film.rs:200-224) ---------------------------------------------
+
+def ray_with_lens(cam, image_sample, image_size, aperture_radius, focus_distance, aperture_sample):
+    o = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    lib().orc_ray_with_lens(_p(f32(cam)), float(image_sample[0]), float(image_sample[1]), int(image_size[0]),
+                            int(image_size[1]), float(aperture_radius), float(focus_distance),
+                            float(aperture_sample[0]), float(aperture_sample[1]), _p(o), _p(d))
+    return o, d
+
+
+def aperture_sample(pixel, sample, n):
+    out = np.zeros(2, np.float32)
+    lib().orc_aperture_sample(int(pixel[0]), int(pixel[1]), int(sample[0]), int(sample[1]), int(n), _p(out))
+    return out
