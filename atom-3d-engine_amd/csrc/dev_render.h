@@ -322,9 +322,9 @@ MM3E_DEV void path_transition(const DevScene& sc, const Tables& tb, PathState& p
 #define MM3E_PATH_GATE 1
 #endif
 
-// HIT0 also reports whether the primary (depth-0) ray hit a surface, in *hit0.
+// HIT0 also returns the primary (depth-0) march's hit record in *hit0.
 template <class Map, int MAXB, bool HIT0 = false>
-MM3E_DEV V3 trace_sm(const DevScene& sc, const Map& field, V3 ro, V3 rd, bool* hit0 = nullptr) {
+MM3E_DEV V3 trace_sm(const DevScene& sc, const Map& field, V3 ro, V3 rd, Hit* hit0 = nullptr) {
   PathState ps;
   LevelStack<MAXB> stk;
   ps.level = 0;
@@ -338,7 +338,7 @@ MM3E_DEV V3 trace_sm(const DevScene& sc, const Map& field, V3 ro, V3 rd, bool* h
     if (wait_mask != 0 && (search_mask == 0 || __builtin_popcountll(wait_mask) >= MM3E_PATH_GATE)) {
       if (waiting) {
         if constexpr (HIT0) {
-          if (ps.phase == PH_MARCH && ps.level == 0) *hit0 = ps.h.hit;
+          if (ps.phase == PH_MARCH && ps.level == 0) *hit0 = ps.h;
         }
         path_transition<MAXB>(sc, field.tb, ps, stk);
         // the only search that can finish before its first evaluation is a march with
@@ -469,11 +469,10 @@ MM3E_DEV void lens_ray(const FrameParams& fp, const FilmParams& lp, float sx, fl
   rd = normalize(focus - origin);
 }
 
-// trace (lib.rs:1265-1282) that also reports whether the ray hit a surface.
+// trace (lib.rs:1265-1282) that also returns the primary march's hit record.
 template <class Map, int MAXB>
-MM3E_DEV V3 trace_hit(const DevScene& sc, const Map& field, V3 ro, V3 rd, bool& hit) {
-  Hit h = march(sc, field, ro, rd);
-  hit = h.hit;
+MM3E_DEV V3 trace_hit(const DevScene& sc, const Map& field, V3 ro, V3 rd, Hit& h) {
+  h = march(sc, field, ro, rd);
   if (!h.hit) return sky(rd, ld3(sc.sun));
   return shade_level<Map, 0, MAXB>(sc, field, h, rd);
 }
@@ -508,8 +507,10 @@ MM3E_DEV void film_tile(const DevScene* __restrict__ scp, const Tables& tb, cons
       aperture_sample(px, py, sx, sy, aa, ax, ay);
       lens_ray(fp, lp, ix, iy, ax, ay, ro, rd);
     }
-    if constexpr (SM) rad = trace_sm<Map, MAXB, true>(sc, field, ro, rd, &hit);
-    else rad = trace_hit<Map, MAXB>(sc, field, ro, rd, hit);
+    Hit h0;
+    if constexpr (SM) rad = trace_sm<Map, MAXB, true>(sc, field, ro, rd, &h0);
+    else rad = trace_hit<Map, MAXB>(sc, field, ro, rd, h0);
+    hit = h0.hit;
   }
   smem[4 * t + 0] = rad.x;
   smem[4 * t + 1] = rad.y;
@@ -538,6 +539,74 @@ MM3E_DEV void film_tile(const DevScene* __restrict__ scp, const Tables& tb, cons
     fg[3 * i + 1] = fo.y;
     fg[3 * i + 2] = fo.z;
     alpha[i] = (float)hits * inv;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// G-buffer for reprojection (lib.rs:951-1030) and the re-rendered holes of the hybrid
+// reprojection (lib.rs:1043-1080): one centre sample per pixel straight to display colour
+// (exposure, ACES, gamma, 8 bit; no bloom), the primary hit distance and the moving object the
+// hit belongs to.
+// ---------------------------------------------------------------------------------------
+struct GMover {  // (centre, radius) of a tagged moving object
+  float c[3], r;
+};
+
+MM3E_DEV uint8_t to_u8(float c);
+
+MM3E_DEV uchar4 display_px(V3 c, float exposure) {  // shade::gamma(shade::aces(c * exposure)).clamp01()
+  c = scale(c, exposure);
+  V3 g = V3{gamma1(aces1(c.x)), gamma1(aces1(c.y)), gamma1(aces1(c.z))};
+  g = clamp01(g);
+  return uchar4{to_u8(g.x), to_u8(g.y), to_u8(g.z), (uint8_t)255};
+}
+
+// Lanes map to pixels of a 16x16 tile (list == nullptr) or to list[blockIdx.x*256 + lane]
+// (pixel indices of the frame; *n_list of them, read on the device so the list's producer needs
+// no round trip to the host).  depth / obj may be nullptr (hole re-rendering).
+template <class Map, int MAXB, bool SM>
+MM3E_DEV void gbuffer_tile(const DevScene* __restrict__ scp, const FrameParams& fp, float exposure,
+                           const GMover* __restrict__ movers, uint32_t n_movers, const uint32_t* __restrict__ list,
+                           const uint32_t* __restrict__ n_list, uchar4* __restrict__ color,
+                           float* __restrict__ depth, int32_t* __restrict__ obj) {
+  const DevScene& sc = *scp;
+  Map field{scp, global_tables(scp)};
+  const uint32_t t = threadIdx.x;
+  uint32_t x, y;
+  bool valid;
+  if (list) {
+    const uint32_t k = blockIdx.x * 256u + t;
+    valid = k < *n_list;
+    const uint32_t p = valid ? list[k] : 0u;
+    x = p % fp.width;
+    y = p / fp.width;
+  } else {
+    x = blockIdx.x * 16u + (t & 15u);
+    y = blockIdx.y * 16u + (t >> 4);
+    valid = x < fp.width && y < fp.height;
+  }
+  if (!valid) return;
+  V3 ro, rd;
+  camera_ray(fp, (float)x + 0.5f, (float)y + 0.5f, ro, rd);
+  Hit h;
+  V3 hdr;
+  if constexpr (SM) hdr = trace_sm<Map, MAXB, true>(sc, field, ro, rd, &h);
+  else hdr = trace_hit<Map, MAXB>(sc, field, ro, rd, h);
+  const size_t i = (size_t)y * fp.width + x;
+  color[i] = display_px(hdr, exposure);
+  if (depth) depth[i] = h.hit ? h.t : __builtin_inff();
+  if (obj) {
+    int32_t o = -1;
+    if (h.hit) {
+      for (uint32_t k = 0; k < n_movers; k++) {
+        const GMover& m = movers[k];
+        if (length(h.pos - V3{m.c[0], m.c[1], m.c[2]}) <= m.r + 0.05f) {
+          o = (int32_t)k;
+          break;
+        }
+      }
+    }
+    obj[i] = o;
   }
 }
 

@@ -218,3 +218,150 @@ hipError_t launch_march_probe_table(hipStream_t st, const DevScene* sc, const fl
 }
 
 }  // namespace mm3e
+
+// ---------------------------------------------------------------------------------------
+// Reprojection of a G-buffer frame into a new camera (reproject.rs:57-170)
+// ---------------------------------------------------------------------------------------
+namespace mm3e {
+
+// reproject.rs:57-69: screen position and forward depth of a world point, false behind the camera.
+MM3E_DEV bool project_px(const FrameParams& c, V3 world, float& sx, float& sy, float& fz) {
+  const V3 v = world - ld3(c.eye);
+  fz = dot(v, ld3(c.fwd));
+  if (fz <= 1e-4f) return false;
+  const float aspect = (float)c.width / (float)c.height;
+  const float ndc_x = dot(v, ld3(c.right)) / fz;
+  const float ndc_y = dot(v, ld3(c.up)) / fz;
+  sx = (ndc_x / (aspect * c.fov_scale) + 1.0f) * 0.5f * (float)c.width;
+  sy = (1.0f - ndc_y / c.fov_scale) * 0.5f * (float)c.height;
+  return true;
+}
+
+MM3E_DEV int32_t sat_i32(float f) {  // Rust `f32 as i32`: saturating, NaN -> 0
+  if (f != f) return 0;
+  if (f >= 2147483648.0f) return 2147483647;
+  if (f < -2147483648.0f) return (int32_t)0x80000000u;
+  return (int32_t)f;
+}
+
+}  // namespace mm3e
+
+extern "C" __global__ void __launch_bounds__(256) mm3e_warp_reset(unsigned long long* __restrict__ keys, uint32_t n) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i < n) keys[i] = ~0ull;
+}
+
+// One lane per source pixel: its world point (depth, or 1e6 for sky), advanced by its mover's
+// delta, projected into the new camera; the target keeps the nearest point, ties to the lowest
+// source index (the reference's row-major loop with a strict `<`), as one 64-bit atomic min of
+// (forward-depth bits, source index): positive f32 bit patterns order as the values.
+extern "C" __global__ void __launch_bounds__(256)
+    mm3e_warp_splat(uint32_t w, uint32_t h, FrameParams prev, FrameParams next, const float* __restrict__ depth,
+                    const int32_t* __restrict__ obj, const float* __restrict__ deltas, uint32_t n_deltas,
+                    unsigned long long* __restrict__ keys) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= w * h) return;
+  const uint32_t x = i % w, y = i / w;
+  V3 eye, dir;
+  camera_ray(prev, (float)x + 0.5f, (float)y + 0.5f, eye, dir);
+  const float dp = depth[i];
+  const float d = isfinite(dp) ? dp : 1.0e6f;
+  V3 world = eye + scale(dir, d);
+  const int32_t oi = obj[i];
+  if (oi >= 0 && (uint32_t)oi < n_deltas)
+    world = world + V3{deltas[3 * oi], deltas[3 * oi + 1], deltas[3 * oi + 2]};
+  float sx, sy, fz;
+  if (!project_px(next, world, sx, sy, fz)) return;
+  const int32_t nx = sat_i32(floorf(sx)), ny = sat_i32(floorf(sy));
+  if (nx < 0 || ny < 0 || (uint32_t)nx >= w || (uint32_t)ny >= h) return;
+  if (!(fz < __builtin_inff())) return;  // `fz < zbuf[j]` with zbuf starting at +inf
+  const size_t j = (size_t)ny * w + (uint32_t)nx;
+  atomicMin(&keys[j], ((unsigned long long)__float_as_uint(fz) << 32) | (unsigned long long)i);
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+    mm3e_warp_resolve(uint32_t n, const unsigned long long* __restrict__ keys, const uchar4* __restrict__ color,
+                      uchar4* __restrict__ out, uint8_t* __restrict__ filled) {
+  const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+  if (j >= n) return;
+  const unsigned long long k = keys[j];
+  const bool f = k != ~0ull;
+  out[j] = f ? color[(uint32_t)(k & 0xffffffffull)] : uchar4{0, 0, 0, 0};
+  filled[j] = f ? 1 : 0;
+}
+
+// reproject.rs:141-168: per row, holes take the last filled pixel to their left, then the
+// row's leading holes take the first filled pixel to their right.  After the first sweep every
+// pixel from the row's first filled one on is filled, so the second sweep only copies that
+// pixel into the leading holes; the warp's coverage mask is left as the warp made it.  One lane
+// per row.
+extern "C" __global__ void __launch_bounds__(256)
+    mm3e_row_fill(uint32_t w, uint32_t h, uchar4* __restrict__ out, const uint8_t* __restrict__ filled) {
+  const uint32_t y = blockIdx.x * 256u + threadIdx.x;
+  if (y >= h) return;
+  uchar4* row = out + (size_t)y * w;
+  const uint8_t* fr = filled + (size_t)y * w;
+  uint32_t first = w;
+  uchar4 last = uchar4{0, 0, 0, 0};
+  for (uint32_t x = 0; x < w; x++) {
+    if (fr[x]) {
+      last = row[x];
+      if (first == w) first = x;
+    } else if (first < w) {
+      row[x] = last;
+    }
+  }
+  if (first < w) {
+    const uchar4 lead = row[first];
+    for (uint32_t x = 0; x < first; x++) row[x] = lead;
+  }
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+    mm3e_hole_list(uint32_t n, const uint8_t* __restrict__ filled, uint32_t* __restrict__ list,
+                   uint32_t* __restrict__ count) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n || filled[i]) return;
+  list[atomicAdd(count, 1u)] = i;
+}
+
+namespace mm3e {
+
+hipError_t launch_warp(hipStream_t st, uint32_t w, uint32_t h, const FrameParams& prev, const FrameParams& next,
+                       const float* depth, const int32_t* obj, const float* deltas, uint32_t n_deltas,
+                       const uchar4* color, unsigned long long* keys, uchar4* out, uint8_t* filled) {
+  const uint32_t n = w * h;
+  const dim3 g((n + 255) / 256);
+  hipLaunchKernelGGL(mm3e_warp_reset, g, dim3(256), 0, st, keys, n);
+  hipLaunchKernelGGL(mm3e_warp_splat, g, dim3(256), 0, st, w, h, prev, next, depth, obj, deltas, n_deltas, keys);
+  hipLaunchKernelGGL(mm3e_warp_resolve, g, dim3(256), 0, st, n, keys, color, out, filled);
+  return hipGetLastError();
+}
+
+hipError_t launch_row_fill(hipStream_t st, uint32_t w, uint32_t h, uchar4* out, const uint8_t* filled) {
+  hipLaunchKernelGGL(mm3e_row_fill, dim3((h + 255) / 256), dim3(256), 0, st, w, h, out, filled);
+  return hipGetLastError();
+}
+
+hipError_t launch_hole_list(hipStream_t st, uint32_t n, const uint8_t* filled, uint32_t* list, uint32_t* count) {
+  hipLaunchKernelGGL(mm3e_hole_list, dim3((n + 255) / 256), dim3(256), 0, st, n, filled, list, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_gbuffer(uint32_t maxb, hipStream_t st, const DevScene* sc, bool general, const FrameParams& fp,
+                          float exposure, const GMover* movers, uint32_t n_movers, const uint32_t* list,
+                          const uint32_t* n_list, uint32_t max_list, uchar4* color, float* depth, int32_t* obj) {
+#define MM3E_GB(B) \
+  launch_gbuffer_b##B(st, sc, general, fp, exposure, movers, n_movers, list, n_list, max_list, color, depth, obj)
+  switch (maxb) {
+    case 0: return MM3E_GB(0);
+    case 1: return MM3E_GB(1);
+    case 2: return MM3E_GB(2);
+    case 3: return MM3E_GB(3);
+    case 4: return MM3E_GB(4);
+    default: return hipErrorInvalidValue;
+  }
+#undef MM3E_GB
+}
+
+}  // namespace mm3e

@@ -62,6 +62,30 @@ MM3E_DECL_FILM(2)
 MM3E_DECL_FILM(3)
 MM3E_DECL_FILM(4)
 #undef MM3E_DECL_FILM
+struct GMover;
+#define MM3E_DECL_GBUF(B)                                                                                          \
+  hipError_t launch_gbuffer_b##B(hipStream_t st, const DevScene* sc, bool general, const FrameParams& fp,          \
+                                 float exposure, const GMover* movers, uint32_t n_movers, const uint32_t* list,    \
+                                 const uint32_t* n_list, uint32_t max_list, uchar4* color, float* depth,          \
+                                 int32_t* obj);
+MM3E_DECL_GBUF(0)
+MM3E_DECL_GBUF(1)
+MM3E_DECL_GBUF(2)
+MM3E_DECL_GBUF(3)
+MM3E_DECL_GBUF(4)
+#undef MM3E_DECL_GBUF
+// G-buffer of a frame (list == nullptr) or the display colour of the listed pixels (*n_list of
+// them in device memory, at most max_list).
+hipError_t launch_gbuffer(uint32_t maxb, hipStream_t st, const DevScene* sc, bool general, const FrameParams& fp,
+                          float exposure, const GMover* movers, uint32_t n_movers, const uint32_t* list,
+                          const uint32_t* n_list, uint32_t max_list, uchar4* color, float* depth, int32_t* obj);
+// Reprojection (reproject.rs:57-170): reset the z-buffer keys, splat the previous frame's
+// pixels into the new camera, resolve the winners, optionally fill holes along rows, list holes.
+hipError_t launch_warp(hipStream_t st, uint32_t w, uint32_t h, const FrameParams& prev, const FrameParams& next,
+                       const float* depth, const int32_t* obj, const float* deltas, uint32_t n_deltas,
+                       const uchar4* color, unsigned long long* keys, uchar4* out, uint8_t* filled);
+hipError_t launch_row_fill(hipStream_t st, uint32_t w, uint32_t h, uchar4* out, const uint8_t* filled);
+hipError_t launch_hole_list(hipStream_t st, uint32_t n, const uint8_t* filled, uint32_t* list, uint32_t* count);
 // render_film's sample pass for a scene with `maxb` bounces (0..4).
 hipError_t launch_film(uint32_t maxb, dim3 grid, dim3 block, hipStream_t st, const DevScene* sc, bool general,
                        const FrameParams& fp, const FilmParams& lp, float* beauty, float* fg, float* alpha,

@@ -32,7 +32,8 @@ EXPORTS = (
     "mm3e_mesh_parse_obj", "mm3e_mesh_sdf_layout", "mm3e_mesh_bake_sdf", "mm3e_mesh_baker_create",
     "mm3e_mesh_baker_layout", "mm3e_mesh_baker_info", "mm3e_mesh_baker_run", "mm3e_mesh_baker_wait",
     "mm3e_mesh_baker_read", "mm3e_mesh_baker_destroy", "mm3e_sdfv_encode", "mm3e_sdfv_decode",
-    "mm3e_scene_render_film",
+    "mm3e_scene_render_film", "mm3e_gframe_create", "mm3e_gframe_destroy", "mm3e_gframe_upload",
+    "mm3e_gframe_read", "mm3e_scene_render_gbuffer", "mm3e_gframe_reproject", "mm3e_scene_reproject_hybrid",
 )
 
 CODEGEN_OFF, CODEGEN_BACKGROUND, CODEGEN_SYNC = 0, 1, 2
@@ -119,6 +120,14 @@ def lib():
         L.mm3e_sdfv_encode.argtypes = [P, P, Z, P, C.POINTER(Z), C.c_char_p, Z]
         L.mm3e_sdfv_decode.argtypes = [P, Z, P, P, Z, C.c_char_p, Z]
         L.mm3e_scene_render_film.argtypes = [P, P, P, P, P, P, P, P, P, C.c_char_p, Z]
+        L.mm3e_gframe_create.argtypes = [P, C.c_uint32, C.c_uint32, C.POINTER(P), C.c_char_p, Z]
+        L.mm3e_gframe_destroy.argtypes = [P]
+        L.mm3e_gframe_destroy.restype = None
+        L.mm3e_gframe_upload.argtypes = [P, P, P, P, P]
+        L.mm3e_gframe_read.argtypes = [P, P, P, P, P]
+        L.mm3e_scene_render_gbuffer.argtypes = [P, P, P, C.c_uint32, P, C.c_char_p, Z]
+        L.mm3e_gframe_reproject.argtypes = [P, P, P, C.c_uint32, C.c_int, P, P, C.c_char_p, Z]
+        L.mm3e_scene_reproject_hybrid.argtypes = [P, P, P, P, C.c_uint32, P, C.POINTER(C.c_uint32), C.c_char_p, Z]
         _lib = L
     return _lib
 
@@ -279,6 +288,64 @@ def sdfv_decode(blob: bytes):
     if rc != 0:
         raise Mm3eError(rc, err.value.decode())
     return lay, data
+
+
+def _deltas(deltas):
+    d = np.ascontiguousarray(np.asarray(deltas, np.float32).reshape(-1, 3)) if len(deltas) else None
+    return d, (0 if d is None else len(d))
+
+
+class GFrame:
+    """A device-resident G-buffer frame (reproject::GFrame, reproject.rs:24-36)."""
+
+    def __init__(self, renderer, width: int, height: int):
+        self.h = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        rc = lib().mm3e_gframe_create(renderer.h, int(width), int(height), C.byref(self.h), err, 1024)
+        if rc != 0:
+            self.h = None
+            raise Mm3eError(rc, err.value.decode())
+        self.width, self.height = int(width), int(height)
+
+    def upload(self, g: dict):
+        """Load a host G-frame dict (color, depth, obj, camera)."""
+        rc = lib().mm3e_gframe_upload(self.h, _p(_cam(g["camera"])), _p(np.ascontiguousarray(g["color"], np.uint8)),
+                                      _p(np.ascontiguousarray(g["depth"], np.float32)),
+                                      _p(np.ascontiguousarray(g["obj"], np.int32)))
+        if rc != 0:
+            raise Mm3eError(rc, "gframe upload failed")
+
+    def read(self) -> dict:
+        sh = (self.height, self.width)
+        g = {"color": np.zeros(sh + (4,), np.uint8), "depth": np.zeros(sh, np.float32),
+             "obj": np.zeros(sh, np.int32), "camera": np.zeros(13, np.float32)}
+        rc = lib().mm3e_gframe_read(self.h, _p(g["camera"]), _p(g["color"]), _p(g["depth"]), _p(g["obj"]))
+        if rc != 0:
+            raise Mm3eError(rc, "gframe holds no frame")
+        return g
+
+    def reproject(self, cam, deltas=(), fill=True):
+        """reproject::reproject (fill) / reproject::warp on the device -> (rgba, filled)."""
+        out = np.zeros((self.height, self.width, 4), np.uint8)
+        filled = np.zeros((self.height, self.width), np.uint8)
+        d, n = _deltas(deltas)
+        err = C.create_string_buffer(1024)
+        rc = lib().mm3e_gframe_reproject(self.h, _p(_cam(cam)), _p(d), n, 1 if fill else 0, _p(out), _p(filled), err,
+                                         1024)
+        if rc != 0:
+            raise Mm3eError(rc, err.value.decode())
+        return out, filled
+
+    def close(self):
+        if self.h:
+            lib().mm3e_gframe_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class MeshBaker:
@@ -455,6 +522,27 @@ class GpuScene:
         if rc != 0:
             raise Mm3eError(rc, "render_aov failed")
         return hit, mat, depth, steps
+
+    def render_gbuffer(self, cam, gframe: "GFrame", movers=()):
+        """render_gbuffer (lib.rs:951-1030) into gframe; movers: (cx, cy, cz, radius) each."""
+        mv = np.ascontiguousarray(np.asarray(movers, np.float32).reshape(-1, 4)) if len(movers) else None
+        err = C.create_string_buffer(1024)
+        rc = lib().mm3e_scene_render_gbuffer(self.h, _p(_cam(cam)), _p(mv), 0 if mv is None else len(mv), gframe.h,
+                                             err, 1024)
+        if rc != 0:
+            raise Mm3eError(rc, err.value.decode())
+
+    def reproject_hybrid(self, gframe: "GFrame", cam, deltas=()):
+        """reproject_hybrid (lib.rs:1043-1080) on the device -> (rgba, holes rendered)."""
+        out = np.zeros((gframe.height, gframe.width, 4), np.uint8)
+        d, n = _deltas(deltas)
+        holes = C.c_uint32(0)
+        err = C.create_string_buffer(1024)
+        rc = lib().mm3e_scene_reproject_hybrid(self.h, gframe.h, _p(_cam(cam)), _p(d), n, _p(out), C.byref(holes), err,
+                                               1024)
+        if rc != 0:
+            raise Mm3eError(rc, err.value.decode())
+        return out, holes.value
 
     def render_film(self, cam, aperture_radius=0.0, focus_distance=1.0):
         """render_film (film.rs:63-198) on the device -> dict of beauty, foreground, alpha, depth,

@@ -30,6 +30,9 @@ Two more legs ride on the same line (``--legs`` picks them):
            imported from OBJ text and baked to a signed-distance grid with 128 samples along the
            longest axis (124x63x128) on the device; ms per bake (row-parity + nearest-triangle
            kernels, HIP events), Msamples/s, BVH tests per sample, and the CPU oracle's rate.
+  frame_gen: c2 at 960x540 as a G-buffer keyframe plus reprojected frames for the camera swinging
+           0.01 rad per frame about its target (reproject_hybrid: warp + holes rendered for real); ms per
+           G-buffer frame vs per generated frame (both including the RGBA8 readback), holes.
   stress:  (not in the default set) one 3840x2160 frame of the 1000-primitive scene with a 128^3
            sampled volume, 32^3 device-baked GI, 2 bounces and fog; ms/frame.
 """
@@ -330,14 +333,47 @@ def mesh_bake_leg(hip, configs, r, reps, resolution=128, padding=0.35, cpu=True)
     return out
 
 
+def frame_gen_leg(hip, configs, r, frames=8):
+    """render_gbuffer keyframe + reproject_hybrid frames on c2 (lib.rs:951-1080)."""
+    import math
+    sc, cam = configs.c2_scene()
+    g = r.compile(sc.to_ir(), sc.width, sc.height)
+    gf = hip.GFrame(r, sc.width, sc.height)
+    # the c2 camera (eye (0, 2, 7.5) looking at (0, 1.6, 0)) swung about the vertical through its target
+    cams = [configs.look_at((7.5 * math.sin(0.01 * k), 2.0, 7.5 * math.cos(0.01 * k)), (0.0, 1.6, 0.0),
+                            (0.0, 1.0, 0.0), math.radians(50.0)) for k in range(frames + 1)]
+    g.render_gbuffer(cams[0], gf)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(3):
+        g.render_gbuffer(cams[0], gf)
+    key_ms = (time.perf_counter() - t0) / 3 * 1e3
+    g.reproject_hybrid(gf, cams[1])  # warm-up
+    holes, t = [], 0.0
+    for c in cams[1:]:
+        t0 = time.perf_counter()
+        _, n = g.reproject_hybrid(gf, c)
+        t += time.perf_counter() - t0
+        holes.append(n)
+    gen_ms = t / frames * 1e3
+    gf.close()
+    g.close()
+    px = sc.width * sc.height
+    out = {"workload": f"c2 960x540, the c2 camera as keyframe, {frames} frames swinging 0.01 rad each about "
+                       f"the vertical through its target", "gbuffer_ms": round(key_ms, 3), "hybrid_ms": round(gen_ms, 3),
+           "speedup": round(key_ms / gen_ms, 2), "mean_hole_fraction": round(sum(holes) / len(holes) / px, 5)}
+    log(f"frame_gen: keyframe {out['gbuffer_ms']} ms, generated {out['hybrid_ms']} ms "
+        f"({out['speedup']}x), holes {out['mean_hole_fraction']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="plain,batched,served,mesh_bake",
-                    help="comma list of plain (always run), batched, served, mesh_bake, stress")
+    ap.add_argument("--legs", default="plain,batched,served,mesh_bake,frame_gen",
+                    help="comma list of plain (always run), batched, served, mesh_bake, frame_gen, stress")
     ap.add_argument("--bake-reps", type=int, default=20)
     ap.add_argument("--stress-frames", type=int, default=3)
     ap.add_argument("--batched-reps", type=int, default=3)
@@ -438,6 +474,8 @@ def main():
         out["served"] = served_leg(hip, configs, r, args.served_requests)
     if "mesh_bake" in legs:
         out["mesh_bake"] = mesh_bake_leg(hip, configs, r, args.bake_reps, cpu=not args.no_cpu)
+    if "frame_gen" in legs:
+        out["frame_gen"] = frame_gen_leg(hip, configs, r)
     if "stress" in legs:
         out["stress"] = stress_leg(hip, configs, r, args.stress_frames)
     print(json.dumps(out), flush=True)

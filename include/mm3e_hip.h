@@ -186,6 +186,41 @@ int mm3e_scene_parse_text(const char* text, size_t text_len, uint8_t* ir, size_t
 int mm3e_scene_to_text(const void* scene_ir, size_t ir_len, const mm3e_camera* cam, char* text, size_t* text_len,
                        char* err, size_t errlen);
 
+/* ---- G-buffer frames and reprojection (mm3e-orchestrator/src/reproject.rs, lib.rs:951-1080) ---
+ * A GFrame (reproject.rs:24-36) lives in device memory: display RGBA8, primary-ray hit
+ * distance (+inf for sky), the index of the moving object each pixel belongs to (-1: static or
+ * sky) and the camera it was rendered with.  Frame generation warps it into later cameras
+ * instead of rendering them. */
+typedef struct mm3e_gframe mm3e_gframe;
+/* A moving object for render_gbuffer's tags: a hit within radius + 0.05 of center is tagged. */
+typedef struct mm3e_mover {
+  float center[3];
+  float radius;
+} mm3e_mover;
+int mm3e_gframe_create(mm3e_renderer* r, uint32_t width, uint32_t height, mm3e_gframe** out, char* err,
+                       size_t errlen);
+void mm3e_gframe_destroy(mm3e_gframe* g);
+/* Host <-> device copies of a GFrame: color w*h*4 bytes, depth w*h f32, obj w*h i32. */
+int mm3e_gframe_upload(mm3e_gframe* g, const mm3e_camera* cam, const uint8_t* color, const float* depth,
+                       const int32_t* obj);
+int mm3e_gframe_read(mm3e_gframe* g, mm3e_camera* cam, uint8_t* color, float* depth, int32_t* obj);
+/* render_gbuffer (lib.rs:951-1030) into g (same renderer, the scene's size): one centre sample
+ * per pixel through the full shading (reflections included), displayed with exposure, ACES and
+ * gamma (no bloom); movers (at most 256) tag the pixels whose hit lies within radius + 0.05. */
+int mm3e_scene_render_gbuffer(mm3e_scene* s, const mm3e_camera* cam, const mm3e_mover* movers, uint32_t n_movers,
+                              mm3e_gframe* g, char* err, size_t errlen);
+/* reproject::warp / reproject::reproject (reproject.rs:82-170): forward-warp g into cam with
+ * per-object deltas (mover_deltas[n][3], world units; at most 256), nearest point winning; with
+ * fill_holes each hole takes the last filled pixel to its left on its row, then leading holes
+ * the first to their right (reproject); without, holes stay (0,0,0,0) (warp).  out: w*h*4;
+ * filled (optional, w*h bytes): the warp's coverage before any fill. */
+int mm3e_gframe_reproject(mm3e_gframe* g, const mm3e_camera* cam, const float* mover_deltas, uint32_t n_deltas,
+                          int fill_holes, uint8_t* out, uint8_t* filled, char* err, size_t errlen);
+/* reproject_hybrid (lib.rs:1043-1080): the warp, then only the holes rendered for real through
+ * the scene at the new camera (display colour as render_gbuffer).  *n_holes = pixels rendered. */
+int mm3e_scene_reproject_hybrid(mm3e_scene* s, mm3e_gframe* g, const mm3e_camera* cam, const float* mover_deltas,
+                                uint32_t n_deltas, uint8_t* out, uint32_t* n_holes, char* err, size_t errlen);
+
 /* ---- meshes and sampled SDF volumes --------------------------------------------------------
  * mm3e-orchestrator/src/mesh.rs (OBJ import, mesh -> SDF bake) and mm3e-kit/src/volume.rs
  * (SdfVolume, the `.sdfv` codec).  A mesh travels as positions[n_positions][3] (f32) and

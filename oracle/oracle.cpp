@@ -7,6 +7,7 @@
 //   mm3e-kit/src/{vec,sdf,dual,volume,csg,camera,march,shade,color,framebuffer}.rs, lib.rs:63-78
 //   mm3e-orchestrator/src/lib.rs:156-1435, accel.rs, gi.rs, post.rs, appearance.rs:427-431,
 //   film.rs:63-225 (render_film, primary-hit AOVs), camera.rs:71-118 (thin lens),
+//   lib.rs:951-1080 + reproject.rs:57-170 (G-buffer, warp, reprojection, hybrid),
 //   mm3e-cpu-parallel/src/lib.rs:50-84.
 #include <atomic>
 #include <cstdio>
@@ -1577,6 +1578,162 @@ int orc_render_film(const orc_scene* s, const float* cam13, float radius, float 
       }
   });
   return 0;
+}
+
+// ---- G-buffer frames and reprojection (lib.rs:951-1080, reproject.rs:57-170) -------------
+
+}  // extern "C"
+
+// lib.rs:981-1016: one centre sample, display colour (no bloom), hit distance, mover tag.
+static void gbuffer_pixel(const Renderer<float>& rd, const Camera& cam, uint32_t w, uint32_t h, uint32_t x, uint32_t y,
+                          const float* movers, uint32_t n_movers, uint8_t* color, float* depth, int32_t* obj) {
+  const Scene& sc = *rd.sc;
+  const Ray<float> ray = Renderer<float>::cam_ray(cam, (float)x + 0.5f, (float)y + 0.5f, w, h);
+  const Hit<float> hit = rd.march(ray);
+  V3<float> hdr;
+  float dd = INFINITY;
+  int32_t o = -1;
+  if (hit.hit) {
+    for (uint32_t k = 0; k < n_movers; k++) {
+      const V3<float> c(movers[4 * k], movers[4 * k + 1], movers[4 * k + 2]);
+      if (length(hit.pos - c) <= movers[4 * k + 3] + 0.05f) {
+        o = (int32_t)k;
+        break;
+      }
+    }
+    hdr = rd.shade_hit(hit, ray, 0);
+    dd = hit.t;
+  } else {
+    hdr = sky(ray.d, rd.sun());
+  }
+  const V3<float> d = clamp01(gamma(aces(scale(hdr, sc.exposure))));
+  color[0] = (uint8_t)(d.x * 255.0f + 0.5f);  // `(d * 255.0 + 0.5) as u8` on a clamped value
+  color[1] = (uint8_t)(d.y * 255.0f + 0.5f);
+  color[2] = (uint8_t)(d.z * 255.0f + 0.5f);
+  color[3] = 255;
+  if (depth) *depth = dd;
+  if (obj) *obj = o;
+}
+
+// reproject.rs:57-69
+static bool project_px(const Camera& c, V3<float> world, uint32_t w, uint32_t h, float& sx, float& sy, float& fz) {
+  const V3<float> v = world - f3<float>(c.eye);
+  fz = dot(v, f3<float>(c.forward));
+  if (fz <= 1e-4f) return false;
+  const float aspect = (float)w / (float)h;
+  const float ndc_x = dot(v, f3<float>(c.right)) / fz;
+  const float ndc_y = dot(v, f3<float>(c.up)) / fz;
+  sx = (ndc_x / (aspect * c.fov_scale) + 1.0f) * 0.5f * (float)w;
+  sy = (1.0f - ndc_y / c.fov_scale) * 0.5f * (float)h;
+  return true;
+}
+
+static int32_t as_i32(float f) {  // Rust `f32 as i32`
+  if (std::isnan(f)) return 0;
+  if (f >= 2147483648.0f) return INT32_MAX;
+  if (f < -2147483648.0f) return INT32_MIN;
+  return (int32_t)f;
+}
+
+// reproject.rs:82-130, the reference's sequential loop and z-buffer
+static void warp_frame(uint32_t w, uint32_t h, const Camera& prev, const uint8_t* color, const float* depth,
+                       const int32_t* obj, const Camera& next, const float* deltas, uint32_t n_deltas, uint8_t* out,
+                       uint8_t* filled) {
+  const size_t n = (size_t)w * h;
+  std::vector<float> zbuf(n, INFINITY);
+  std::memset(out, 0, n * 4);
+  std::memset(filled, 0, n);
+  for (uint32_t y = 0; y < h; y++)
+    for (uint32_t x = 0; x < w; x++) {
+      const size_t i = (size_t)y * w + x;
+      const V3<float> dir = Renderer<float>::cam_ray(prev, (float)x + 0.5f, (float)y + 0.5f, w, h).d;
+      const float d = std::isfinite(depth[i]) ? depth[i] : 1.0e6f;
+      V3<float> world = f3<float>(prev.eye) + scale(dir, d);
+      const int32_t oi = obj[i];
+      if (oi >= 0 && (uint32_t)oi < n_deltas)
+        world = world + V3<float>(deltas[3 * oi], deltas[3 * oi + 1], deltas[3 * oi + 2]);
+      float sx, sy, fz;
+      if (!project_px(next, world, w, h, sx, sy, fz)) continue;
+      const int32_t nx = as_i32(std::floor(sx)), ny = as_i32(std::floor(sy));
+      if (nx >= 0 && ny >= 0 && (uint32_t)nx < w && (uint32_t)ny < h) {
+        const size_t j = (size_t)ny * w + (uint32_t)nx;
+        if (fz < zbuf[j]) {
+          zbuf[j] = fz;
+          std::memcpy(out + 4 * j, color + 4 * i, 4);
+          filled[j] = 1;
+        }
+      }
+    }
+}
+
+extern "C" {
+
+// render_gbuffer (lib.rs:951-1030); movers: [n][4] centre xyz, radius.
+int orc_render_gbuffer(const orc_scene* s, const float* cam13, const float* movers, uint32_t n_movers, unsigned threads,
+                       uint8_t* color, float* depth, int32_t* obj) {
+  const Scene& sc = s->sc;
+  const Camera cam = to_cam(cam13);
+  Renderer<float> rd(&sc);
+  for_bands(sc.height, threads, [&](uint32_t y0, uint32_t y1) {
+    for (uint32_t y = y0; y < y1; y++)
+      for (uint32_t x = 0; x < sc.width; x++) {
+        const size_t i = (size_t)y * sc.width + x;
+        gbuffer_pixel(rd, cam, sc.width, sc.height, x, y, movers, n_movers, color + 4 * i, depth + i, obj + i);
+      }
+  });
+  return 0;
+}
+
+// reproject::warp (fill == 0) or reproject::reproject (fill != 0), reproject.rs:82-170.
+void orc_reproject(uint32_t w, uint32_t h, const float* prev_cam13, const uint8_t* color, const float* depth,
+                   const int32_t* obj, const float* cam13, const float* deltas, uint32_t n_deltas, int fill,
+                   uint8_t* out, uint8_t* filled) {
+  warp_frame(w, h, to_cam(prev_cam13), color, depth, obj, to_cam(cam13), deltas, n_deltas, out, filled);
+  if (!fill) return;
+  std::vector<uint8_t> f(filled, filled + (size_t)w * h);
+  for (uint32_t y = 0; y < h; y++) {
+    bool have = false;
+    uint8_t last[4] = {0, 0, 0, 0};
+    for (uint32_t x = 0; x < w; x++) {
+      const size_t i = (size_t)y * w + x;
+      if (f[i]) {
+        std::memcpy(last, out + 4 * i, 4);
+        have = true;
+      } else if (have) {
+        std::memcpy(out + 4 * i, last, 4);
+        f[i] = 1;
+      }
+    }
+    have = false;
+    for (uint32_t x = w; x-- > 0;) {
+      const size_t i = (size_t)y * w + x;
+      if (f[i]) {
+        std::memcpy(last, out + 4 * i, 4);
+        have = true;
+      } else if (have) {
+        std::memcpy(out + 4 * i, last, 4);
+      }
+    }
+  }
+}
+
+// reproject_hybrid (lib.rs:1043-1080): the warp, then the holes rendered through the scene.
+uint32_t orc_reproject_hybrid(const orc_scene* s, uint32_t w, uint32_t h, const float* prev_cam13, const uint8_t* color,
+                              const float* depth, const int32_t* obj, const float* cam13, const float* deltas,
+                              uint32_t n_deltas, uint8_t* out) {
+  std::vector<uint8_t> filled((size_t)w * h);
+  const Camera cam = to_cam(cam13);
+  warp_frame(w, h, to_cam(prev_cam13), color, depth, obj, cam, deltas, n_deltas, out, filled.data());
+  Renderer<float> rd(&s->sc);
+  uint32_t holes = 0;
+  for (uint32_t y = 0; y < h; y++)
+    for (uint32_t x = 0; x < w; x++) {
+      const size_t i = (size_t)y * w + x;
+      if (filled[i]) continue;
+      gbuffer_pixel(rd, cam, w, h, x, y, nullptr, 0, out + 4 * i, nullptr, nullptr);
+      holes++;
+    }
+  return holes;
 }
 
 // World field at points: linear=0 -> accelerated `world()`, 1 -> `world_linear()`.

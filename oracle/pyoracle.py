@@ -66,6 +66,11 @@ def lib():
         L.orc_to_bmp.argtypes = [P, C.c_uint32, C.c_uint32, P]
         L.orc_to_bmp.restype = C.c_size_t
         L.orc_render_film.argtypes = [P, P, C.c_float, C.c_float, C.c_uint, P, P, P, P, P, P]
+        L.orc_render_gbuffer.argtypes = [P, P, P, C.c_uint32, C.c_uint, P, P, P]
+        L.orc_reproject.argtypes = [C.c_uint32, C.c_uint32, P, P, P, P, P, P, C.c_uint32, C.c_int, P, P]
+        L.orc_reproject.restype = None
+        L.orc_reproject_hybrid.argtypes = [P, C.c_uint32, C.c_uint32, P, P, P, P, P, P, C.c_uint32, P]
+        L.orc_reproject_hybrid.restype = C.c_uint32
         L.orc_ray_with_lens.argtypes = [P, C.c_float, C.c_float, C.c_uint32, C.c_uint32, C.c_float, C.c_float,
                                         C.c_float, C.c_float, P, P]
         L.orc_ray_with_lens.restype = None
@@ -160,6 +165,26 @@ class OracleScene:
                                                            ("beauty", "foreground", "alpha", "depth", "normals",
                                                             "material_ids")))
         return out
+
+    def render_gbuffer(self, cam, movers=(), threads=0):
+        """render_gbuffer (lib.rs:951-1030) -> dict(color[h,w,4] u8, depth[h,w], obj[h,w] i32, camera)."""
+        h, w = self.height, self.width
+        mv = f32(np.asarray(movers, np.float32).reshape(-1, 4)) if len(movers) else np.zeros((0, 4), np.float32)
+        g = {"color": np.zeros((h, w, 4), np.uint8), "depth": np.zeros((h, w), np.float32),
+             "obj": np.zeros((h, w), np.int32), "camera": f32(cam).copy()}
+        lib().orc_render_gbuffer(self.h, _p(f32(cam)), _p(mv) if len(mv) else None, len(mv), threads or os.cpu_count(),
+                                 _p(g["color"]), _p(g["depth"]), _p(g["obj"]))
+        return g
+
+    def reproject_hybrid(self, g, cam, deltas=()):
+        """reproject_hybrid (lib.rs:1043-1080) of G-frame g -> (rgba[h,w,4], holes)."""
+        h, w = g["depth"].shape
+        d = f32(np.asarray(deltas, np.float32).reshape(-1, 3)) if len(deltas) else np.zeros((0, 3), np.float32)
+        out = np.zeros((h, w, 4), np.uint8)
+        holes = lib().orc_reproject_hybrid(self.h, w, h, _p(f32(g["camera"])), _p(np.ascontiguousarray(g["color"])),
+                                           _p(f32(g["depth"])), _p(np.ascontiguousarray(g["obj"], np.int32)),
+                                           _p(f32(cam)), _p(d) if len(d) else None, len(d), _p(out))
+        return out, int(holes)
 
     def field(self, pts, linear=False):
         pts = f32(pts).reshape(-1, 3)
@@ -336,3 +361,15 @@ def aperture_sample(pixel, sample, n):
     out = np.zeros(2, np.float32)
     lib().orc_aperture_sample(int(pixel[0]), int(pixel[1]), int(sample[0]), int(sample[1]), int(n), _p(out))
     return out
+
+
+def reproject(g, cam, deltas=(), fill=True):
+    """reproject::reproject (fill) or reproject::warp (no fill) of G-frame g -> (rgba, filled)."""
+    h, w = g["depth"].shape
+    d = f32(np.asarray(deltas, np.float32).reshape(-1, 3)) if len(deltas) else np.zeros((0, 3), np.float32)
+    out = np.zeros((h, w, 4), np.uint8)
+    filled = np.zeros((h, w), np.uint8)
+    lib().orc_reproject(w, h, _p(f32(g["camera"])), _p(np.ascontiguousarray(g["color"])), _p(f32(g["depth"])),
+                        _p(np.ascontiguousarray(g["obj"], np.int32)), _p(f32(cam)), _p(d) if len(d) else None,
+                        len(d), 1 if fill else 0, _p(out), _p(filled))
+    return out, filled
