@@ -403,6 +403,33 @@ MM3E_DEV void render_tile(const DevScene* __restrict__ scp, const Tables& tb, co
 }
 
 // ---------------------------------------------------------------------------------------
+// Ordered tiles: a launch of one block per tile whose block b renders tile order[b], so the
+// tiles the previous frame found most expensive are dispatched first.  A frame whose cost sits
+// in a few long tiles otherwise ends with those tiles running alone on an idle chip; started
+// first, they overlap the cheap ones.  Each block draws a ticket from one global counter when
+// it starts and another when it ends; the difference (how many blocks started or finished
+// meanwhile) is the tile's cost for the next frame's order.  Every tile is rendered exactly
+// as render_tile renders it, whatever the order.
+// ---------------------------------------------------------------------------------------
+struct TileOrder {
+  const uint32_t* order;  // tile indices (ty * tiles_x + tx), most expensive first
+  uint32_t* cost;         // per tile: tickets drawn by other blocks while it ran
+  uint32_t* ticket;       // running ticket counter (wraps; only differences are used)
+  uint32_t tiles_x, pad;
+};
+
+template <class Map, int MAXB, bool SM>
+MM3E_DEV void render_ordered(const DevScene* __restrict__ scp, const Tables& tb, const FrameParams& fp,
+                             float* __restrict__ hdr, uint32_t tw, uint32_t th, const TileOrder& o, float* smem,
+                             uint32_t* start) {
+  const uint32_t tile = o.order[blockIdx.x];
+  if (threadIdx.x == 0) *start = atomicAdd(o.ticket, 1u);
+  render_tile<Map, MAXB, SM>(scp, tb, fp, hdr, tw, th, tile % o.tiles_x, tile / o.tiles_x, smem);
+  __syncthreads();
+  if (threadIdx.x == 0) o.cost[tile] = atomicAdd(o.ticket, 1u) - *start;
+}
+
+// ---------------------------------------------------------------------------------------
 // Film: beauty, premultiplied foreground and coverage alpha through a thin lens
 // (film.rs:63-225, camera.rs:71-102)
 // ---------------------------------------------------------------------------------------

@@ -365,3 +365,44 @@ hipError_t launch_gbuffer(uint32_t maxb, hipStream_t st, const DevScene* sc, boo
 }
 
 }  // namespace mm3e
+
+// ---------------------------------------------------------------------------------------
+// Tile order for the next frame of an ordered-tiles render: tiles sorted into 128 buckets of
+// descending cost (4 per power of two of the ticket count), expensive buckets first; the order
+// inside a bucket is arbitrary (the image does not depend on it).  One block.
+// ---------------------------------------------------------------------------------------
+namespace mm3e {
+MM3E_DEV uint32_t cost_bucket(uint32_t c) {
+  if (c == 0) return 127u;
+  const uint32_t e = 31u - (uint32_t)__builtin_clz(c);
+  const uint32_t m = e >= 2 ? (c >> (e - 2)) & 3u : 0u;
+  return 127u - (e * 4u + m);
+}
+}  // namespace mm3e
+
+extern "C" __global__ void __launch_bounds__(1024)
+    mm3e_tile_order(const uint32_t* __restrict__ cost, uint32_t n, uint32_t* __restrict__ order) {
+  __shared__ uint32_t hist[128];
+  const uint32_t t = threadIdx.x;
+  if (t < 128) hist[t] = 0;
+  __syncthreads();
+  for (uint32_t i = t; i < n; i += 1024) atomicAdd(&hist[cost_bucket(cost[i])], 1u);
+  __syncthreads();
+  if (t == 0) {
+    uint32_t run = 0;
+    for (uint32_t b = 0; b < 128; b++) {
+      const uint32_t c = hist[b];
+      hist[b] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = t; i < n; i += 1024) order[atomicAdd(&hist[cost_bucket(cost[i])], 1u)] = i;
+}
+
+namespace mm3e {
+hipError_t launch_tile_order(hipStream_t st, const uint32_t* cost, uint32_t n, uint32_t* order) {
+  hipLaunchKernelGGL(mm3e_tile_order, dim3(1), dim3(1024), 0, st, cost, n, order);
+  return hipGetLastError();
+}
+}  // namespace mm3e

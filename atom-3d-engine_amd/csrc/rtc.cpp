@@ -20,6 +20,7 @@ struct Entry {
   bool ok = false;
   hipModule_t mod = nullptr;
   hipFunction_t fn = nullptr;
+  hipFunction_t fn_o = nullptr;  // the ordered-tiles variant
   std::string err;
 };
 
@@ -68,7 +69,7 @@ bool compile(const std::string& src, std::vector<char>& code, std::string& err) 
 
 }  // namespace
 
-bool rtc_kernel(const std::string& src, int device, hipFunction_t* fn, std::string& err) {
+bool rtc_kernel(const std::string& src, int device, hipFunction_t* fn, hipFunction_t* fn_o, std::string& err) {
   std::shared_ptr<Entry> e;
   {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -88,6 +89,7 @@ bool rtc_kernel(const std::string& src, int device, hipFunction_t* fn, std::stri
       e->ok = false;
       e->err = "generated module has no mm3e_gen_render";
     }
+    if (e->ok && hipModuleGetFunction(&e->fn_o, e->mod, "mm3e_gen_render_o") != hipSuccess) e->fn_o = nullptr;
     e->done = true;
   }
   if (!e->ok) {
@@ -95,6 +97,7 @@ bool rtc_kernel(const std::string& src, int device, hipFunction_t* fn, std::stri
     return false;
   }
   *fn = e->fn;
+  if (fn_o) *fn_o = e->fn_o;
   return true;
 }
 

@@ -9,7 +9,8 @@ namespace mm3e {
 
 // Compile `src` for gfx950 with the same floating-point contract as the library's own kernels
 // (-O3, no contraction, correctly rounded division/sqrt, no packed FP32), load it on `device`
-// (must be current) and return its `mm3e_gen_render`.  Identical sources compile once.
-bool rtc_kernel(const std::string& src, int device, hipFunction_t* fn, std::string& err);
+// (must be current) and return its `mm3e_gen_render` and, when fn_o is given, its ordered-tiles
+// variant `mm3e_gen_render_o` (nullptr if the source has none).  Identical sources compile once.
+bool rtc_kernel(const std::string& src, int device, hipFunction_t* fn, hipFunction_t* fn_o, std::string& err);
 
 }  // namespace mm3e

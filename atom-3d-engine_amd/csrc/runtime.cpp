@@ -168,10 +168,12 @@ static void gen_run(mm3e::GenJob& j) {
   hipSetDevice(j.device);
   hipFunction_t fn = nullptr;
   std::string err;
-  const bool ok = rtc_kernel(j.src, j.device, &fn, err);
+  hipFunction_t fn_o = nullptr;
+  const bool ok = rtc_kernel(j.src, j.device, &fn, &fn_o, err);
   {
     std::lock_guard<std::mutex> lk(j.mu);
     j.fn = fn;
+    j.fn_o = fn_o;
     j.err = err;
     j.state.store(ok ? MM3E_CODEGEN_READY : MM3E_CODEGEN_FAILED, std::memory_order_release);
   }
@@ -222,6 +224,27 @@ static void gen_schedule(mm3e_scene* s) {
   r->gen_cv.notify_one();
 }
 
+// The tile order of a scene's generated kernel, set up on its first use: the frame's tiles in
+// index order, zero costs, ticket counter at zero.
+static bool tile_order_ready(mm3e_scene* s, uint32_t n, std::string& msg) {
+  if (s->d_to && s->to_tiles == n) return true;
+  if (s->d_to) {
+    hipFree(s->d_to);
+    s->d_to = nullptr;
+  }
+  std::vector<uint32_t> init(2 * (size_t)n + 1, 0u);
+  for (uint32_t i = 0; i < n; i++) init[i] = i;
+  if (!hip_ok(hipMalloc((void**)&s->d_to, init.size() * 4), "hipMalloc(tile order)", &msg) ||
+      !hip_ok(hipMemcpy(s->d_to, init.data(), init.size() * 4, hipMemcpyHostToDevice), "upload tile order", &msg)) {
+    hipFree(s->d_to);
+    s->d_to = nullptr;
+    s->to_on = false;
+    return false;
+  }
+  s->to_tiles = n;
+  return true;
+}
+
 bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std::string& msg) {
   hipStream_t st = s->r->stream;
   const uint32_t stage = dsc == s->d_scene ? s->stage_bytes : s->dyn_stage_bytes;
@@ -240,15 +263,25 @@ bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std
     return true;
   }
   const mm3e::GenJob* gj = s->gen.get();
+  bool ordered = false;
   if (dsc == s->d_scene && gj && gj->state.load(std::memory_order_acquire) == MM3E_CODEGEN_READY) {
     const DevScene* sc_arg = dsc;
     FrameParams fp_arg = fp;
     float* hdr_arg = s->d_hdr;
     uint32_t tw = s->tw, th = s->th, gstage = s->gen_stage;
-    void* args[] = {&sc_arg, &fp_arg, &hdr_arg, &tw, &th, &gstage};
-    if (!hip_ok(hipModuleLaunchKernel(gj->fn, grid.x, grid.y, 1, block.x, 1, 1, gstage, st, args, nullptr),
-                "generated render launch", &msg))
-      return false;
+    if (gj->fn_o && s->to_on && tile_order_ready(s, grid.x * grid.y, msg)) {
+      TileOrder o{s->d_to, s->d_to + s->to_tiles, s->d_to + 2 * s->to_tiles, grid.x, 0u};
+      void* args[] = {&sc_arg, &fp_arg, &hdr_arg, &tw, &th, &gstage, &o};
+      if (!hip_ok(hipModuleLaunchKernel(gj->fn_o, s->to_tiles, 1, 1, block.x, 1, 1, gstage, st, args, nullptr),
+                  "generated render launch", &msg))
+        return false;
+      ordered = true;
+    } else {
+      void* args[] = {&sc_arg, &fp_arg, &hdr_arg, &tw, &th, &gstage};
+      if (!hip_ok(hipModuleLaunchKernel(gj->fn, grid.x, grid.y, 1, block.x, 1, 1, gstage, st, args, nullptr),
+                  "generated render launch", &msg))
+        return false;
+    }
   } else if (!hip_ok(launch_render_table(s->bounces, grid, block, st, dsc, stage, general, fp, s->d_hdr, s->tw, s->th),
                      "render launch", &msg)) {
     return false;
@@ -256,6 +289,8 @@ bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std
   hipEventRecord(ev[1], st);
   if (!hip_ok(launch_post(st, s->pp, s->d_kern, s->d_hdr, s->d_tmp, s->d_rgba), "post launch", &msg)) return false;
   hipEventRecord(ev[2], st);
+  if (ordered && !hip_ok(launch_tile_order(st, s->d_to + s->to_tiles, s->to_tiles, s->d_to), "tile order launch", &msg))
+    return false;
   s->frames++;
   s->timing_valid = true;
   return true;
@@ -321,6 +356,7 @@ int mm3e_renderer_create(const char* adapter_filter, mm3e_renderer** out, char* 
     else if (m == "sync" || m == "2") r->codegen = MM3E_CODEGEN_SYNC;
     else r->codegen = MM3E_CODEGEN_BACKGROUND;
   }
+  if (const char* to = std::getenv("MM3E_TILE_ORDER")) r->tile_order = !(lower(to) == "off" || lower(to) == "0");
   r->name = describe(dev);
   // Tables staged in LDS share the block's LDS with the 3 KiB sample-reduction buffer; keep
   // the copy to 56 KiB so three 256-lane blocks still fit one CU.
@@ -379,6 +415,7 @@ int mm3e_scene_compile(mm3e_renderer* r, const void* ir, size_t ir_len, uint32_t
   s->w = w;
   s->h = h;
   s->aa = std::max(s->host.hdr.aa, 1u);  // lib.rs:282
+  s->to_on = r->tile_order;
   s->bounces = s->host.hdr.bounces;
   tile_shape(s->aa, &s->tw, &s->th);
   bool ok = true;
@@ -476,6 +513,7 @@ void mm3e_scene_destroy(mm3e_scene* s) {
   if (s->d_hdr) hipFree(s->d_hdr);
   if (s->d_tmp) hipFree(s->d_tmp);
   if (s->d_rgba) hipFree(s->d_rgba);
+  if (s->d_to) hipFree(s->d_to);
   if (s->h_rgba) hipHostFree(s->h_rgba);
   for (int f = 0; f < mm3e_scene::kRing; f++)
     for (int i = 0; i < 3; i++)
@@ -590,7 +628,7 @@ int mm3e_scene_info(const mm3e_scene* s, uint32_t* n_objects, uint32_t* n_segmen
 int mm3e_scene_kernel_name(const mm3e_scene* s, char* buf, size_t buflen) {
   if (!s || !buf || buflen == 0) return MM3E_ERR_INVALID;
   if (s->gen && s->gen->state.load(std::memory_order_acquire) == MM3E_CODEGEN_READY) {
-    std::snprintf(buf, buflen, "mm3e_gen_render");
+    std::snprintf(buf, buflen, "%s", s->gen->fn_o && s->to_on ? "mm3e_gen_render_o" : "mm3e_gen_render");
     return MM3E_OK;
   }
   const char* base = s->general ? (s->stage_bytes ? "mm3e_render_lds_x_b" : "mm3e_render_table_x_b")
@@ -886,6 +924,13 @@ int mm3e_write_bmp(const uint8_t* rgba, uint32_t w, uint32_t h, uint8_t* out, si
     }
     for (size_t i = 0; i < pad; i++) *q++ = 0;
   }
+  return MM3E_OK;
+}
+
+int mm3e_renderer_set_tile_order(mm3e_renderer* r, int on) {
+  if (!r) return MM3E_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(r->mu);
+  r->tile_order = on != 0;
   return MM3E_OK;
 }
 

@@ -22,6 +22,7 @@ struct GenJob {
   int device = 0;
   std::atomic<int> state{MM3E_CODEGEN_QUEUED};
   hipFunction_t fn = nullptr;
+  hipFunction_t fn_o = nullptr;  // ordered-tiles variant
   std::string err;
   std::mutex mu;
   std::condition_variable cv;
@@ -36,6 +37,7 @@ struct mm3e_renderer {
   std::mutex mu;
   // scene code generation: mode, queue and worker threads (started on first use)
   int codegen = MM3E_CODEGEN_BACKGROUND;
+  bool tile_order = true;        // generated kernels dispatch tiles most expensive first
   std::mutex gen_mu;
   std::condition_variable gen_cv;
   std::deque<std::shared_ptr<mm3e::GenJob>> gen_queue;
@@ -73,6 +75,11 @@ struct mm3e_scene {
   bool general = false;                // the scene needs the general kernels (segment grid / CSG)
   bool dyn_general = false;            // the same for the tables with dynamic spheres appended
   std::shared_ptr<mm3e::GenJob> gen;   // generated kernel (null when code generation is off)
+  // tile order of the generated kernel: the frame's tiles most expensive first (costs from the
+  // previous frame), their costs, the ticket counter
+  uint32_t* d_to = nullptr;            // [order n_tiles][cost n_tiles][ticket]
+  uint32_t to_tiles = 0;
+  bool to_on = false;
   uint32_t gen_stage = 0;              // LDS bytes the generated kernel stages (0: none needed)
 };
 

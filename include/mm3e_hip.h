@@ -166,6 +166,12 @@ enum { MM3E_CODEGEN_OFF = 0, MM3E_CODEGEN_BACKGROUND = 1, MM3E_CODEGEN_SYNC = 2 
 enum { MM3E_CODEGEN_NONE = 0, MM3E_CODEGEN_QUEUED = 1, MM3E_CODEGEN_READY = 2, MM3E_CODEGEN_FAILED = 3 };
 /* Mode for scenes compiled afterwards on this renderer. */
 int mm3e_renderer_set_codegen(mm3e_renderer* r, int mode);
+/* Tile order for generated kernels of scenes compiled afterwards (default on; the environment
+ * variable MM3E_TILE_ORDER=off sets the default off): the frame's tiles are dispatched most
+ * expensive first, ordered by the previous frame's per-tile costs, so a frame whose cost sits
+ * in a few long tiles does not end with them running alone.  The image is the same bit for
+ * bit either way. */
+int mm3e_renderer_set_tile_order(mm3e_renderer* r, int on);
 /* State of a scene's generated kernel (MM3E_CODEGEN_NONE/QUEUED/READY/FAILED); wait != 0
  * blocks until it is READY or FAILED.  On FAILED the compiler's message goes to err. */
 int mm3e_scene_codegen_status(mm3e_scene* s, int wait, int* state, char* err, size_t errlen);

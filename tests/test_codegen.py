@@ -36,25 +36,39 @@ def test_emitted_source_names_every_segment():
 
 @pytest.mark.gpu
 def test_generated_kernels_equal_table_kernels():
+    """Generated kernels, dispatched in index order and most-expensive-tile first (the order is
+    rebuilt from each frame's tile costs, so three frames see three orders), equal the table
+    kernels bit for bit."""
     table = hip.Renderer()
     table.set_codegen(hip.CODEGEN_OFF)
     gen = hip.Renderer()
     gen.set_codegen(hip.CODEGEN_SYNC)
+    gen.set_tile_order(False)
+    ordered = hip.Renderer()
+    ordered.set_codegen(hip.CODEGEN_SYNC)
     for name, sc, cam in _scenes():
         ir = sc.to_ir()
         gt = table.compile(ir, sc.width, sc.height)
         gg = gen.compile(ir, sc.width, sc.height)
-        state, msg = gg.codegen_status(wait=True)
-        assert state == hip.CODEGEN_READY, msg
+        go = ordered.compile(ir, sc.width, sc.height)
+        for g in (gg, go):
+            state, msg = g.codegen_status(wait=True)
+            assert state == hip.CODEGEN_READY, msg
         assert gg.kernel_name() == "mm3e_gen_render"
+        assert go.kernel_name() == "mm3e_gen_render_o"
         a, b = gt.render_rgba(cam), gg.render_rgba(cam)
-        ha, ma, da, _ = gt.render_aov(cam)
-        print(f"\n[codegen {name}] kernel={gt.kernel_name()} equal={bool((a == b).all())}", flush=True)
+        frames = [go.render_rgba(cam) for _ in range(3)]
+        print(f"\n[codegen {name}] kernel={gt.kernel_name()} equal={bool((a == b).all())} "
+              f"ordered_equal={[bool((a == f).all()) for f in frames]}", flush=True)
         assert (a == b).all(), name
+        for f in frames:
+            assert (a == f).all(), name
         gt.close()
         gg.close()
+        go.close()
     table.close()
     gen.close()
+    ordered.close()
 
 
 @pytest.mark.gpu
@@ -79,7 +93,7 @@ def test_background_codegen_switches_without_changing_pixels():
     state, msg = g.codegen_status(wait=True)
     assert state == hip.CODEGEN_READY, msg
     after = g.render_rgba(cam)
-    assert g.kernel_name() == "mm3e_gen_render"
+    assert g.kernel_name() == "mm3e_gen_render_o"
     assert (before == after).all()
     g.close()
     r.close()
