@@ -14,6 +14,7 @@
 #include "codegen.h"
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 namespace mm3e {
@@ -73,6 +74,16 @@ std::string emit_scene_source(const Blob& b, uint32_t maxb, bool sm) {
   o.reserve(4096 + b.objs.size() * 900);
   o += "// generated: world fold of one compiled scene (objects: " + std::to_string(b.objs.size()) +
        ", segments: " + std::to_string(b.segs.size()) + ")\n";
+  // Waves per SIMD the generated kernel is register-allocated for: 6 (at most 80 VGPRs).  The
+  // kernel is bound by dependency latency more than by issue; measured on MI355X against 3, 4,
+  // 5 and 8: c2 960x540 1.62 / 1.62 / 1.43 / 1.35 (6) / 1.74 ms, a 40-object 1280x720 scene
+  // 15.4 / 12.6 / 11.8 / 11.9 / 16.2 ms.  MM3E_GEN_WAVES overrides it.
+  int waves = 6;
+  if (const char* wv = std::getenv("MM3E_GEN_WAVES")) {
+    const int n = std::atoi(wv);
+    if (n >= 1 && n <= 8) waves = n;
+  }
+  o += "#define MM3E_RENDER_WAVES " + std::to_string(waves) + "\n";
   o += "#ifndef __HIPCC_RTC__\n#include <hip/hip_runtime.h>\n#endif\n#include \"dev_render.h\"\nnamespace mm3e {\n";
   o += "struct GenMap {\n  const DevScene* sc;\n  Tables tb;\n  MM3E_DEV Field operator()(V3 p) const;\n};\n";
   // The fold is the loop of world_eval with the per-segment body specialised: a uniform switch

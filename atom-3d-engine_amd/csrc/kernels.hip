@@ -406,3 +406,27 @@ hipError_t launch_tile_order(hipStream_t st, const uint32_t* cost, uint32_t n, u
   return hipGetLastError();
 }
 }  // namespace mm3e
+
+// ---------------------------------------------------------------------------------------
+// Self-test of crsqrt against the compiler's correctly rounded sqrtf over every 32-bit pattern
+// (NaNs compare equal to NaNs).  Each lane checks a contiguous run of 256 patterns.
+// ---------------------------------------------------------------------------------------
+extern "C" __global__ void __launch_bounds__(256) mm3e_sqrt_selftest(unsigned long long* __restrict__ mismatches) {
+  const uint64_t lo = ((uint64_t)blockIdx.x * 256u + threadIdx.x) * 256u;
+  uint32_t bad = 0;
+  for (uint32_t k = 0; k < 256; k++) {
+    const uint64_t u = lo + k;
+    if (u > 0xffffffffull) break;
+    const float x = __uint_as_float((uint32_t)u);
+    const float a = __builtin_sqrtf(x), b = crsqrt(x);
+    if (__float_as_uint(a) != __float_as_uint(b) && !(a != a && b != b)) bad++;
+  }
+  if (bad) atomicAdd(mismatches, (unsigned long long)bad);
+}
+
+namespace mm3e {
+hipError_t launch_sqrt_selftest(hipStream_t st, unsigned long long* mismatches) {
+  hipLaunchKernelGGL(mm3e_sqrt_selftest, dim3(65536), dim3(256), 0, st, mismatches);
+  return hipGetLastError();
+}
+}  // namespace mm3e

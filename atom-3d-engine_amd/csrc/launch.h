@@ -86,6 +86,8 @@ hipError_t launch_warp(hipStream_t st, uint32_t w, uint32_t h, const FrameParams
                        const uchar4* color, unsigned long long* keys, uchar4* out, uint8_t* filled);
 hipError_t launch_row_fill(hipStream_t st, uint32_t w, uint32_t h, uchar4* out, const uint8_t* filled);
 hipError_t launch_hole_list(hipStream_t st, uint32_t n, const uint8_t* filled, uint32_t* list, uint32_t* count);
+// crsqrt against sqrtf over all 2^32 inputs; *mismatches must start at zero.
+hipError_t launch_sqrt_selftest(hipStream_t st, unsigned long long* mismatches);
 // Next frame's tile order from this frame's per-tile costs.
 hipError_t launch_tile_order(hipStream_t st, const uint32_t* cost, uint32_t n, uint32_t* order);
 // render_film's sample pass for a scene with `maxb` bounces (0..4).

@@ -927,6 +927,23 @@ int mm3e_write_bmp(const uint8_t* rgba, uint32_t w, uint32_t h, uint8_t* out, si
   return MM3E_OK;
 }
 
+int mm3e_selftest_sqrt(mm3e_renderer* r, uint64_t* mismatches) {
+  if (!r || !mismatches) return MM3E_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(r->mu);
+  hipSetDevice(r->device);
+  unsigned long long* d = nullptr;
+  std::string msg;
+  unsigned long long h = 0;
+  bool ok = hip_ok(hipMalloc((void**)&d, 8), "hipMalloc", &msg) &&
+            hip_ok(hipMemcpy(d, &h, 8, hipMemcpyHostToDevice), "upload", &msg) &&
+            hip_ok(launch_sqrt_selftest(r->stream, d), "self-test launch", &msg) &&
+            hip_ok(hipStreamSynchronize(r->stream), "self-test", &msg) &&
+            hip_ok(hipMemcpy(&h, d, 8, hipMemcpyDeviceToHost), "readback", &msg);
+  hipFree(d);
+  *mismatches = h;
+  return ok ? MM3E_OK : MM3E_ERR_DEVICE;
+}
+
 int mm3e_renderer_set_tile_order(mm3e_renderer* r, int on) {
   if (!r) return MM3E_ERR_INVALID;
   std::lock_guard<std::mutex> lk(r->mu);

@@ -27,7 +27,7 @@ EXPORTS = (
     "mm3e_batch_destroy", "mm3e_render_batch", "mm3e_server_default_config", "mm3e_server_create",
     "mm3e_server_add_scene", "mm3e_server_submit", "mm3e_server_poll", "mm3e_server_wait", "mm3e_server_stats",
     "mm3e_server_destroy", "mm3e_scene_kernel_name",
-    "mm3e_scene_info", "mm3e_renderer_set_codegen", "mm3e_renderer_set_tile_order", "mm3e_scene_codegen_status", "mm3e_scene_emit_source",
+    "mm3e_scene_info", "mm3e_renderer_set_codegen", "mm3e_renderer_set_tile_order", "mm3e_selftest_sqrt", "mm3e_scene_codegen_status", "mm3e_scene_emit_source",
     "mm3e_scene_parse_text", "mm3e_scene_to_text",
     "mm3e_mesh_parse_obj", "mm3e_mesh_sdf_layout", "mm3e_mesh_bake_sdf", "mm3e_mesh_baker_create",
     "mm3e_mesh_baker_layout", "mm3e_mesh_baker_info", "mm3e_mesh_baker_run", "mm3e_mesh_baker_wait",
@@ -74,6 +74,7 @@ def lib():
         L.mm3e_scene_info.argtypes = [P, P, P, P, P, P]
         L.mm3e_renderer_set_codegen.argtypes = [P, C.c_int]
         L.mm3e_renderer_set_tile_order.argtypes = [P, C.c_int]
+        L.mm3e_selftest_sqrt.argtypes = [P, C.POINTER(C.c_uint64)]
         L.mm3e_scene_codegen_status.argtypes = [P, C.c_int, P, C.c_char_p, C.c_size_t]
         L.mm3e_scene_emit_source.argtypes = [P, C.c_size_t, C.c_char_p, C.POINTER(C.c_size_t)]
         L.mm3e_scene_parse_text.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p, C.POINTER(C.c_size_t), P,
@@ -440,6 +441,14 @@ class Renderer:
         rc = lib().mm3e_renderer_set_tile_order(self.h, 1 if on else 0)
         if rc != 0:
             raise Mm3eError(rc, "set_tile_order failed")
+
+    def selftest_sqrt(self) -> int:
+        """Inputs (of all 2^32) where the kernels' square root differs from IEEE sqrtf."""
+        n = C.c_uint64(0)
+        rc = lib().mm3e_selftest_sqrt(self.h, C.byref(n))
+        if rc != 0:
+            raise Mm3eError(rc, "sqrt self-test failed to run")
+        return n.value
 
     def compile(self, ir: bytes, width: int, height: int) -> "GpuScene":
         return GpuScene(self, ir, width, height)

@@ -172,6 +172,10 @@ int mm3e_renderer_set_codegen(mm3e_renderer* r, int mode);
  * in a few long tiles does not end with them running alone.  The image is the same bit for
  * bit either way. */
 int mm3e_renderer_set_tile_order(mm3e_renderer* r, int on);
+/* Device self-test (no reference counterpart): the render kernels' square root (a shortened
+ * correctly rounded sequence) against the compiler's IEEE sqrtf over all 2^32 f32 inputs;
+ * *mismatches counts the inputs whose results differ (0 expected). */
+int mm3e_selftest_sqrt(mm3e_renderer* r, uint64_t* mismatches);
 /* State of a scene's generated kernel (MM3E_CODEGEN_NONE/QUEUED/READY/FAILED); wait != 0
  * blocks until it is READY or FAILED.  On FAILED the compiler's message goes to err. */
 int mm3e_scene_codegen_status(mm3e_scene* s, int wait, int* state, char* err, size_t errlen);

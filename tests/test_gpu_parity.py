@@ -415,3 +415,9 @@ def test_server_results_equal_alone_and_in_mix(renderer):
     with pytest.raises(hip.Mm3eError):
         srv.wait(rid)
     srv.close()
+
+
+def test_kernel_square_root_is_ieee_for_every_input(renderer):
+    """The render kernels take square roots through a shortened correctly rounded sequence;
+    over all 2^32 f32 bit patterns it must equal the compiler's IEEE sqrtf (NaN == NaN)."""
+    assert renderer.selftest_sqrt() == 0
