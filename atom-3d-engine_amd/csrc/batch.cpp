@@ -1,4 +1,7 @@
 // batch.cpp — batched rendering of many (scene, camera, size) jobs in one persistent launch.
+// When some jobs' scenes have generated kernels, the autotuner also times a mixed mode: those
+// jobs as their own launches spread over four streams, beside the persistent launch for the
+// rest, and keeps whichever is faster.
 //
 // A batch owns a job table in HBM, per-job framebuffers (HDR, bloom scratch, RGBA8) carved
 // from one arena, and a queue counter.  A frame of the batch is three kernels — queue reset,
@@ -40,6 +43,15 @@ struct mm3e_batch {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool launched = false;
   uint64_t total_pixels = 0;
+  // Mixed mode: jobs whose scene has a generated kernel run as their own launches, spread over
+  // kStreams streams, while the other jobs' tiles go through the persistent table kernel on
+  // one of them; the batched post waits for all.  `gen` holds those jobs' table positions.
+  bool mixed = false;
+  std::vector<uint32_t> gen;                 // positions (in posts) of generated-kernel jobs
+  std::vector<mm3e::BatchJob> gen_jobs;      // their launch records (tile shape, camera, hdr)
+  static constexpr int kStreams = 4;
+  hipStream_t aux[kStreams - 1] = {};
+  hipEvent_t fork = nullptr, join[kStreams - 1] = {};
 };
 
 namespace mm3e {
@@ -55,9 +67,17 @@ void job_tiles(const mm3e_batch* b, uint32_t aa, uint32_t* tw, uint32_t* th) {
 }
 
 // (Re)build the job and post tables for the current tiling parameters.
+// A job's scene has a generated kernel ready (read once per table build).
+bool job_generated(const mm3e_batch_job& in) {
+  const GenJob* g = in.scene->gen.get();
+  return g && g->state.load(std::memory_order_acquire) == MM3E_CODEGEN_READY && g->fn;
+}
+
 void build_tables(mm3e_batch* b) {
   b->jobs.clear();
   b->posts.clear();
+  b->gen.clear();
+  b->gen_jobs.clear();
   uint32_t tile = 0, row = 0;
   b->max_w = 0;
   b->max_radius = 1;
@@ -72,16 +92,27 @@ void build_tables(mm3e_batch* b) {
     BatchJob J{};
     J.sc = s->d_scene;
     J.fp = make_frame(in.cam, w, h);
-    job_tiles(b, s->aa, &J.tw, &J.th);
-    J.tiles_x = (w + J.tw - 1) / J.tw;
-    const uint32_t tiles_y = (h + J.th - 1) / J.th;
-    J.tile_base = tile;
-    tile += J.tiles_x * tiles_y;
-    J.stage_bytes = s->hdr.stage_bytes;
-    stage_max = std::max(stage_max, J.stage_bytes);
-    all_fit = all_fit && J.stage_bytes <= b->r->lds_budget;
     const size_t px = (size_t)w * h;
     J.hdr = (float*)(b->d_arena + off);
+    const bool generated = b->mixed && job_generated(in);
+    if (generated) {
+      // its own launch: the scene's tile shape and table staging, as a plain frame
+      J.tw = s->tw;
+      J.th = s->th;
+      J.tiles_x = (w + J.tw - 1) / J.tw;
+      J.stage_bytes = s->gen_stage;
+      b->gen.push_back((uint32_t)b->posts.size());
+      b->gen_jobs.push_back(J);
+    } else {
+      job_tiles(b, s->aa, &J.tw, &J.th);
+      J.tiles_x = (w + J.tw - 1) / J.tw;
+      const uint32_t tiles_y = (h + J.th - 1) / J.th;
+      J.tile_base = tile;
+      tile += J.tiles_x * tiles_y;
+      J.stage_bytes = s->hdr.stage_bytes;
+      stage_max = std::max(stage_max, J.stage_bytes);
+      all_fit = all_fit && J.stage_bytes <= b->r->lds_budget;
+    }
     BatchPost P{};
     P.pp = s->pp;
     P.pp.width = w;
@@ -96,8 +127,10 @@ void build_tables(mm3e_batch* b) {
     off = (off + 255) & ~size_t(255);
     b->max_w = std::max(b->max_w, w);
     b->max_radius = std::max(b->max_radius, P.pp.radius);
-    b->maxb = std::max(b->maxb, s->bounces);
-    b->jobs.push_back(J);
+    if (!generated) {
+      b->maxb = std::max(b->maxb, s->bounces);
+      b->jobs.push_back(J);
+    }
     b->posts.push_back(P);
   }
   b->total_tiles = tile;
@@ -116,18 +149,56 @@ size_t arena_size(const std::vector<mm3e_batch_job>& in) {
 
 bool enqueue(mm3e_batch* b, std::string& msg) {
   hipStream_t st = b->r->stream;
-  if (!hip_ok(launch_batch_reset(st, b->d_counter), "batch reset", &msg)) return false;
-  if (!hip_ok(launch_batch_render(b->maxb, b->cus * b->blocks_per_cu, st, b->d_jobs, (uint32_t)b->jobs.size(),
-                                  b->total_tiles, b->d_counter, b->lds_bytes),
-              "batch render", &msg))
-    return false;
+  if (b->mixed) {
+    // fork: the aux streams start after everything already on the renderer's stream
+    if (!hip_ok(hipEventRecord(b->fork, st), "fork", &msg)) return false;
+    for (int k = 0; k < mm3e_batch::kStreams - 1; k++)
+      if (!hip_ok(hipStreamWaitEvent(b->aux[k], b->fork, 0), "fork wait", &msg)) return false;
+    if (!b->jobs.empty()) {
+      hipStream_t ts = b->aux[0];
+      if (!hip_ok(launch_batch_reset(ts, b->d_counter), "batch reset", &msg)) return false;
+      if (!hip_ok(launch_batch_render(b->maxb, b->cus * b->blocks_per_cu, ts, b->d_jobs, (uint32_t)b->jobs.size(),
+                                      b->total_tiles, b->d_counter, b->lds_bytes),
+                  "batch render", &msg))
+        return false;
+    }
+    // generated jobs round-robin over the renderer's stream and the other aux streams
+    for (size_t i = 0; i < b->gen_jobs.size(); i++) {
+      const BatchJob& J = b->gen_jobs[i];
+      const mm3e_scene* s = b->in[b->order[b->gen[i]]].scene;
+      const int lane = (int)(i % (mm3e_batch::kStreams - 1));
+      hipStream_t gs = lane == 0 ? st : b->aux[lane];
+      const DevScene* sc_arg = J.sc;
+      FrameParams fp_arg = J.fp;
+      float* hdr_arg = J.hdr;
+      uint32_t tw = J.tw, th = J.th, gstage = J.stage_bytes;
+      void* args[] = {&sc_arg, &fp_arg, &hdr_arg, &tw, &th, &gstage};
+      const uint32_t gx = J.tiles_x, gy = (J.fp.height + J.th - 1) / J.th;
+      if (!hip_ok(hipModuleLaunchKernel(s->gen->fn, gx, gy, 1, tw * th * s->aa * s->aa, 1, 1, gstage, gs, args,
+                                        nullptr),
+                  "generated batch job", &msg))
+        return false;
+    }
+    // join
+    for (int k = 0; k < mm3e_batch::kStreams - 1; k++) {
+      if (!hip_ok(hipEventRecord(b->join[k], b->aux[k]), "join", &msg)) return false;
+      if (!hip_ok(hipStreamWaitEvent(st, b->join[k], 0), "join wait", &msg)) return false;
+    }
+  } else {
+    if (!hip_ok(launch_batch_reset(st, b->d_counter), "batch reset", &msg)) return false;
+    if (!hip_ok(launch_batch_render(b->maxb, b->cus * b->blocks_per_cu, st, b->d_jobs, (uint32_t)b->jobs.size(),
+                                    b->total_tiles, b->d_counter, b->lds_bytes),
+                "batch render", &msg))
+      return false;
+  }
   return hip_ok(launch_batch_post(st, b->d_posts, (uint32_t)b->posts.size(), b->total_rows, b->max_w, b->max_radius),
                 "batch post", &msg);
 }
 
 bool upload(mm3e_batch* b, std::string& msg) {
   hipStream_t st = b->r->stream;
-  if (!hip_ok(hipMemcpyAsync(b->d_jobs, b->jobs.data(), b->jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice, st),
+  if (!b->jobs.empty() &&
+      !hip_ok(hipMemcpyAsync(b->d_jobs, b->jobs.data(), b->jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice, st),
               "upload jobs", &msg))
     return false;
   if (!hip_ok(hipMemcpyAsync(b->d_posts, b->posts.data(), b->posts.size() * sizeof(BatchPost), hipMemcpyHostToDevice,
@@ -143,6 +214,7 @@ bool capture(mm3e_batch* b, std::string& msg) {
   if (b->graph) hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
+  if (b->mixed) return true;  // mixed batches are enqueued directly on several streams
   hipStream_t st = b->r->stream;
   if (!hip_ok(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture", &msg)) return false;
   std::string m2;
@@ -180,6 +252,14 @@ void mm3e_batch_destroy(mm3e_batch* b) {
   if (b->d_arena) hipFree(b->d_arena);
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);
+  if (b->fork) hipEventDestroy(b->fork);
+  for (int k = 0; k < mm3e_batch::kStreams - 1; k++) {
+    if (b->join[k]) hipEventDestroy(b->join[k]);
+    if (b->aux[k]) {
+      hipStreamSynchronize(b->aux[k]);
+      hipStreamDestroy(b->aux[k]);
+    }
+  }
   delete b;
 }
 
@@ -229,7 +309,11 @@ int mm3e_batch_create(mm3e_renderer* r, const mm3e_batch_job* jobs, uint32_t n_j
             hip_ok(hipMalloc(&b->d_jobs, n_jobs * sizeof(BatchJob)), "hipMalloc(jobs)", &msg) &&
             hip_ok(hipMalloc(&b->d_posts, n_jobs * sizeof(BatchPost)), "hipMalloc(posts)", &msg) &&
             hip_ok(hipMalloc(&b->d_counter, 256), "hipMalloc(counter)", &msg) &&
-            hip_ok(hipEventCreate(&b->ev0), "hipEventCreate", &msg) && hip_ok(hipEventCreate(&b->ev1), "hipEventCreate", &msg);
+            hip_ok(hipEventCreate(&b->ev0), "hipEventCreate", &msg) && hip_ok(hipEventCreate(&b->ev1), "hipEventCreate", &msg) &&
+            hip_ok(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate", &msg);
+  for (int k = 0; ok && k < mm3e_batch::kStreams - 1; k++)
+    ok = hip_ok(hipStreamCreate(&b->aux[k]), "hipStreamCreate", &msg) &&
+         hip_ok(hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming), "hipEventCreate", &msg);
   if (ok) {
     build_tables(b);
     ok = upload(b, msg) && capture(b, msg);
@@ -246,9 +330,7 @@ int mm3e_batch_create(mm3e_renderer* r, const mm3e_batch_job* jobs, uint32_t n_j
 int mm3e_batch_set_camera(mm3e_batch* b, uint32_t job, const mm3e_camera* cam) {
   if (!b || !cam || job >= b->in.size()) return MM3E_ERR_INVALID;
   b->in[job].cam = *cam;
-  const uint32_t k = b->pos_of[job];
-  b->jobs[k].fp = make_frame(*cam, b->in[job].width, b->in[job].height);
-  b->jobs_dirty = true;
+  b->jobs_dirty = true;  // the job records are rebuilt from the cameras at the next launch
   return MM3E_OK;
 }
 
@@ -257,10 +339,17 @@ int mm3e_batch_launch(mm3e_batch* b) {
   std::lock_guard<std::mutex> lk(b->r->mu);
   hipSetDevice(b->r->device);
   std::string msg;
-  if (b->jobs_dirty && !upload(b, msg)) return MM3E_ERR_DEVICE;
+  if (b->jobs_dirty) {
+    build_tables(b);
+    if (!upload(b, msg)) return MM3E_ERR_DEVICE;
+  }
   hipStream_t st = b->r->stream;
   hipEventRecord(b->ev0, st);
-  if (!hip_ok(hipGraphLaunch(b->exec, st), "graph launch", &msg)) return MM3E_ERR_DEVICE;
+  if (b->mixed) {
+    if (!enqueue(b, msg)) return MM3E_ERR_DEVICE;
+  } else if (!hip_ok(hipGraphLaunch(b->exec, st), "graph launch", &msg)) {
+    return MM3E_ERR_DEVICE;
+  }
   hipEventRecord(b->ev1, st);
   b->launched = true;
   return MM3E_OK;
@@ -309,6 +398,7 @@ int mm3e_batch_autotune(mm3e_batch* b, uint32_t reps, char* report, size_t repor
   std::lock_guard<std::mutex> lk(b->r->mu);
   hipSetDevice(b->r->device);
   reps = std::max(1u, reps);
+  b->mixed = false;
   static const uint32_t bpcs[] = {2, 3, 4, 6, 8};
   static const uint32_t tiles[][2] = {{16, 16}, {32, 8}, {64, 4}, {8, 32}};
   bool any_aa1 = false;
@@ -350,10 +440,36 @@ int mm3e_batch_autotune(mm3e_batch* b, uint32_t reps, char* report, size_t repor
   b->blocks_per_cu = best_bpc;
   b->tile_aa1[0] = best_t[0];
   b->tile_aa1[1] = best_t[1];
+  // mixed mode (generated kernels as their own launches on several streams) when any job's
+  // scene has one; kept if faster than the best persistent-only configuration
+  bool any_gen = false;
+  for (const auto& j : b->in) any_gen = any_gen || job_generated(j);
+  float mixed_ms = -1.0f;
+  if (any_gen) {
+    b->mixed = true;
+    build_tables(b);
+    if (!upload(b, msg)) return MM3E_ERR_DEVICE;
+    if (!enqueue(b, msg)) return MM3E_ERR_DEVICE;  // warm-up
+    hipEventRecord(b->ev0, st);
+    for (uint32_t k = 0; k < reps; k++)
+      if (!enqueue(b, msg)) return MM3E_ERR_DEVICE;
+    hipEventRecord(b->ev1, st);
+    if (hipEventSynchronize(b->ev1) != hipSuccess) return MM3E_ERR_DEVICE;
+    hipEventElapsedTime(&mixed_ms, b->ev0, b->ev1);
+    mixed_ms /= (float)reps;
+    char line[128];
+    std::snprintf(line, sizeof line, "mixed: %zu generated-kernel jobs on %d streams, rest tile %ux%u blocks/CU %u: %.3f ms\n",
+                  b->gen_jobs.size(), mm3e_batch::kStreams, best_t[0], best_t[1], best_bpc, mixed_ms);
+    rep += line;
+    b->mixed = mixed_ms < best;
+  }
   build_tables(b);
   if (!upload(b, msg) || !capture(b, msg)) return MM3E_ERR_DEVICE;
-  char line[96];
-  std::snprintf(line, sizeof line, "chosen: tile %ux%u blocks/CU %u (%.3f ms)\n", best_t[0], best_t[1], best_bpc, best);
+  char line[128];
+  if (b->mixed)
+    std::snprintf(line, sizeof line, "chosen: mixed (%.3f ms)\n", mixed_ms);
+  else
+    std::snprintf(line, sizeof line, "chosen: tile %ux%u blocks/CU %u (%.3f ms)\n", best_t[0], best_t[1], best_bpc, best);
   rep += line;
   set_err(report, report_len, rep);
   return MM3E_OK;

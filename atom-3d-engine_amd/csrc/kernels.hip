@@ -118,6 +118,21 @@ hipError_t launch_render_table(uint32_t maxb, dim3 grid, dim3 block, hipStream_t
   }
 }
 
+hipError_t launch_render_ordered(uint32_t maxb, uint32_t n_tiles, dim3 block, hipStream_t st, const DevScene* sc,
+                                 uint32_t stage_bytes, bool general, const FrameParams& fp, float* hdr, uint32_t tw,
+                                 uint32_t th, const TileOrder& o) {
+#define MM3E_ORD(B) launch_render_ordered_b##B(n_tiles, block, st, sc, stage_bytes, general, fp, hdr, tw, th, o)
+  switch (maxb) {
+    case 0: return MM3E_ORD(0);
+    case 1: return MM3E_ORD(1);
+    case 2: return MM3E_ORD(2);
+    case 3: return MM3E_ORD(3);
+    case 4: return MM3E_ORD(4);
+    default: return hipErrorInvalidValue;
+  }
+#undef MM3E_ORD
+}
+
 hipError_t launch_film(uint32_t maxb, dim3 grid, dim3 block, hipStream_t st, const DevScene* sc, bool general,
                        const FrameParams& fp, const FilmParams& lp, float* beauty, float* fg, float* alpha,
                        uint32_t tw, uint32_t th) {

@@ -2,7 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include "dev_scene.h"
+#include "dev_render.h"
 
 namespace mm3e {
 
@@ -36,6 +36,20 @@ MM3E_DECL_RENDER(3)
 MM3E_DECL_RENDER(4)
 #undef MM3E_DECL_RENDER
 
+#define MM3E_DECL_ORD(B)                                                                                    \
+  hipError_t launch_render_ordered_b##B(uint32_t n_tiles, dim3 block, hipStream_t st, const DevScene* sc,     \
+                                        uint32_t stage_bytes, bool general, const FrameParams& fp, float* hdr, \
+                                        uint32_t tw, uint32_t th, const TileOrder& o);
+MM3E_DECL_ORD(0)
+MM3E_DECL_ORD(1)
+MM3E_DECL_ORD(2)
+MM3E_DECL_ORD(3)
+MM3E_DECL_ORD(4)
+#undef MM3E_DECL_ORD
+// The table kernels with ordered tiles (one block per tile, block b renders o.order[b]).
+hipError_t launch_render_ordered(uint32_t maxb, uint32_t n_tiles, dim3 block, hipStream_t st, const DevScene* sc,
+                                 uint32_t stage_bytes, bool general, const FrameParams& fp, float* hdr, uint32_t tw,
+                                 uint32_t th, const TileOrder& o);
 // stage_bytes > 0 selects the variant that copies the scene tables into LDS first; general
 // selects the kernels with the segment grid and CSG (scene_is_general).
 hipError_t launch_render_table(uint32_t maxb, dim3 grid, dim3 block, hipStream_t st, const DevScene* sc,

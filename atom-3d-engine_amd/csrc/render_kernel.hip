@@ -21,6 +21,9 @@
 #define KNAME_X MM3E_CAT(mm3e_render_table_x_b, MM3E_MAXB)
 #define KNAME_LDS_X MM3E_CAT(mm3e_render_lds_x_b, MM3E_MAXB)
 #define LNAME MM3E_CAT(launch_render_table_b, MM3E_MAXB)
+#define ONAME MM3E_CAT(mm3e_render_ord_b, MM3E_MAXB)
+#define ONAME_X MM3E_CAT(mm3e_render_ord_x_b, MM3E_MAXB)
+#define OLNAME MM3E_CAT(launch_render_ordered_b, MM3E_MAXB)
 
 using namespace mm3e;
 
@@ -56,7 +59,36 @@ extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
   render_tile<TableMap, MM3E_MAXB, true>(sc, tb, fp, hdr, tw, th, blockIdx.x, blockIdx.y, smem);
 }
 
+// Ordered tiles (block b renders tile o.order[b]; see render_ordered), tables staged in LDS
+// when stage_bytes > 0, else read from HBM.
+extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
+    ONAME(const DevScene* __restrict__ sc, FrameParams fp, float* __restrict__ hdr, uint32_t tw, uint32_t th,
+          uint32_t stage_bytes, TileOrder o) {
+  __shared__ float smem[256 * 3];
+  __shared__ uint32_t start;
+  extern __shared__ uint4 tables_lds[];
+  const Tables tb = stage_bytes ? stage_tables(sc, tables_lds, stage_bytes) : global_tables(sc);
+  render_ordered<SimpleMap, MM3E_MAXB, false>(sc, tb, fp, hdr, tw, th, o, smem, &start);
+}
+
+extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
+    ONAME_X(const DevScene* __restrict__ sc, FrameParams fp, float* __restrict__ hdr, uint32_t tw, uint32_t th,
+            uint32_t stage_bytes, TileOrder o) {
+  __shared__ float smem[256 * 3];
+  __shared__ uint32_t start;
+  extern __shared__ uint4 tables_lds[];
+  const Tables tb = stage_bytes ? stage_tables(sc, tables_lds, stage_bytes) : global_tables(sc);
+  render_ordered<TableMap, MM3E_MAXB, true>(sc, tb, fp, hdr, tw, th, o, smem, &start);
+}
+
 namespace mm3e {
+hipError_t OLNAME(uint32_t n_tiles, dim3 block, hipStream_t st, const DevScene* sc, uint32_t stage_bytes, bool general,
+                  const FrameParams& fp, float* hdr, uint32_t tw, uint32_t th, const TileOrder& o) {
+  if (general) hipLaunchKernelGGL(ONAME_X, dim3(n_tiles), block, stage_bytes, st, sc, fp, hdr, tw, th, stage_bytes, o);
+  else hipLaunchKernelGGL(ONAME, dim3(n_tiles), block, stage_bytes, st, sc, fp, hdr, tw, th, stage_bytes, o);
+  return hipGetLastError();
+}
+
 hipError_t LNAME(dim3 grid, dim3 block, hipStream_t st, const DevScene* sc, uint32_t stage_bytes, bool general,
                  const FrameParams& fp, float* hdr, uint32_t tw, uint32_t th) {
   if (general) {

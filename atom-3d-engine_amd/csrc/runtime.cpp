@@ -282,6 +282,13 @@ bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std
                   "generated render launch", &msg))
         return false;
     }
+  } else if (dsc == s->d_scene && s->to_on && tile_order_ready(s, grid.x * grid.y, msg)) {
+    TileOrder o{s->d_to, s->d_to + s->to_tiles, s->d_to + 2 * s->to_tiles, grid.x, 0u};
+    if (!hip_ok(launch_render_ordered(s->bounces, s->to_tiles, block, st, dsc, stage, general, fp, s->d_hdr, s->tw,
+                                      s->th, o),
+                "render launch", &msg))
+      return false;
+    ordered = true;
   } else if (!hip_ok(launch_render_table(s->bounces, grid, block, st, dsc, stage, general, fp, s->d_hdr, s->tw, s->th),
                      "render launch", &msg)) {
     return false;
@@ -631,8 +638,9 @@ int mm3e_scene_kernel_name(const mm3e_scene* s, char* buf, size_t buflen) {
     std::snprintf(buf, buflen, "%s", s->gen->fn_o && s->to_on ? "mm3e_gen_render_o" : "mm3e_gen_render");
     return MM3E_OK;
   }
-  const char* base = s->general ? (s->stage_bytes ? "mm3e_render_lds_x_b" : "mm3e_render_table_x_b")
-                                 : (s->stage_bytes ? "mm3e_render_lds_b" : "mm3e_render_table_b");
+  const char* base = s->to_on ? (s->general ? "mm3e_render_ord_x_b" : "mm3e_render_ord_b")
+                   : s->general ? (s->stage_bytes ? "mm3e_render_lds_x_b" : "mm3e_render_table_x_b")
+                                : (s->stage_bytes ? "mm3e_render_lds_b" : "mm3e_render_table_b");
   std::snprintf(buf, buflen, "%s%u", base, s->bounces);
   return MM3E_OK;
 }

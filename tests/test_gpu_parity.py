@@ -421,3 +421,41 @@ def test_kernel_square_root_is_ieee_for_every_input(renderer):
     """The render kernels take square roots through a shortened correctly rounded sequence;
     over all 2^32 f32 bit patterns it must equal the compiler's IEEE sqrtf (NaN == NaN)."""
     assert renderer.selftest_sqrt() == 0
+
+
+def test_mixed_batch_with_generated_kernels_equals_per_frame_renders():
+    """With generated kernels ready for some scenes (40-object scenes; the 90-object one keeps
+    the table kernels), the autotuner may switch the batch to the mixed mode: generated jobs as
+    their own launches on several streams beside the persistent launch.  Whatever it picks,
+    every job equals the per-frame render bit for bit, across camera changes."""
+    r = hip.Renderer()
+    r.set_codegen(hip.CODEGEN_SYNC)
+    scenes = [configs.procedural_scene(61, 40, 320, 180), configs.procedural_scene(62, 90, 320, 180),
+              configs.c2_scene(width=160, height=90, gi_dims=(4, 4, 4), gi_rays=5)[0]]
+    compiled = [r.compile(sc.to_ir(), sc.width, sc.height) for sc in scenes]
+    rng = configs.SplitMix64(78)
+    sizes = [(320, 180), (200, 120), (96, 54), (320, 180), (160, 90), (130, 77), (256, 144)]
+    jobs = [(compiled[i % 3], configs.orbit_pose(rng), wh) for i, wh in enumerate(sizes)]
+    b = hip.Batch(r, jobs)
+    report = b.autotune(reps=1)
+    print("\n" + report, flush=True)
+    assert "mixed:" in report
+    plain = hip.Renderer()
+    plain.set_codegen(hip.CODEGEN_OFF)
+    for rep in range(2):
+        if rep:
+            for i in range(len(jobs)):
+                cam = configs.orbit_pose(rng)
+                jobs[i] = (jobs[i][0], cam, jobs[i][2])
+                b.set_camera(i, cam)
+        b.launch()
+        b.wait()
+        for i, (g, cam, (w, h)) in enumerate(jobs):
+            ref = plain.compile(scenes[i % 3].to_ir(), w, h)
+            assert (b.read(i) == ref.render_rgba(cam)).all(), f"replay {rep} job {i}"
+            ref.close()
+    b.close()
+    for g in compiled:
+        g.close()
+    plain.close()
+    r.close()
