@@ -52,6 +52,10 @@ struct mm3e_batch {
   static constexpr int kStreams = 4;
   hipStream_t aux[kStreams - 1] = {};
   hipEvent_t fork = nullptr, join[kStreams - 1] = {};
+  // tile order of the persistent launch: [order n][cost n][ticket], n = total_tiles; each replay
+  // records tile costs and re-sorts the order for the next one (expensive tiles first)
+  uint32_t* d_tq = nullptr;
+  uint32_t tq_tiles = 0;
 };
 
 namespace mm3e {
@@ -147,6 +151,39 @@ size_t arena_size(const std::vector<mm3e_batch_job>& in) {
   return off;
 }
 
+// The tile order for the current table, reset to index order when the tile count changed.
+bool ensure_order(mm3e_batch* b, std::string& msg) {
+  if (b->d_tq && b->tq_tiles == b->total_tiles) return true;
+  hipFree(b->d_tq);
+  b->d_tq = nullptr;
+  b->tq_tiles = 0;
+  if (b->total_tiles == 0) return true;
+  std::vector<uint32_t> init(2 * (size_t)b->total_tiles + 1, 0u);
+  for (uint32_t i = 0; i < b->total_tiles; i++) init[i] = i;
+  if (!hip_ok(hipMalloc((void**)&b->d_tq, init.size() * 4), "hipMalloc(batch tile order)", &msg) ||
+      !hip_ok(hipMemcpy(b->d_tq, init.data(), init.size() * 4, hipMemcpyHostToDevice), "upload tile order", &msg))
+    return false;
+  b->tq_tiles = b->total_tiles;
+  return true;
+}
+
+TileOrder batch_order(const mm3e_batch* b) {
+  if (!b->d_tq) return TileOrder{};
+  return TileOrder{b->d_tq, b->d_tq + b->tq_tiles, b->d_tq + 2 * b->tq_tiles, 0u, 0u};
+}
+
+// Persistent render of the table jobs on `ts`, then the next replay's order.
+bool enqueue_persistent(mm3e_batch* b, hipStream_t ts, std::string& msg) {
+  const TileOrder q = batch_order(b);
+  if (!hip_ok(launch_batch_reset(ts, b->d_counter), "batch reset", &msg)) return false;
+  if (!hip_ok(launch_batch_render(b->maxb, b->cus * b->blocks_per_cu, ts, b->d_jobs, (uint32_t)b->jobs.size(),
+                                  b->total_tiles, b->d_counter, b->lds_bytes, q),
+              "batch render", &msg))
+    return false;
+  if (q.order && !hip_ok(launch_tile_order(ts, q.cost, b->tq_tiles, b->d_tq), "batch tile order", &msg)) return false;
+  return true;
+}
+
 bool enqueue(mm3e_batch* b, std::string& msg) {
   hipStream_t st = b->r->stream;
   if (b->mixed) {
@@ -154,14 +191,7 @@ bool enqueue(mm3e_batch* b, std::string& msg) {
     if (!hip_ok(hipEventRecord(b->fork, st), "fork", &msg)) return false;
     for (int k = 0; k < mm3e_batch::kStreams - 1; k++)
       if (!hip_ok(hipStreamWaitEvent(b->aux[k], b->fork, 0), "fork wait", &msg)) return false;
-    if (!b->jobs.empty()) {
-      hipStream_t ts = b->aux[0];
-      if (!hip_ok(launch_batch_reset(ts, b->d_counter), "batch reset", &msg)) return false;
-      if (!hip_ok(launch_batch_render(b->maxb, b->cus * b->blocks_per_cu, ts, b->d_jobs, (uint32_t)b->jobs.size(),
-                                      b->total_tiles, b->d_counter, b->lds_bytes),
-                  "batch render", &msg))
-        return false;
-    }
+    if (!b->jobs.empty() && !enqueue_persistent(b, b->aux[0], msg)) return false;
     // generated jobs round-robin over the renderer's stream and the other aux streams
     for (size_t i = 0; i < b->gen_jobs.size(); i++) {
       const BatchJob& J = b->gen_jobs[i];
@@ -184,12 +214,8 @@ bool enqueue(mm3e_batch* b, std::string& msg) {
       if (!hip_ok(hipEventRecord(b->join[k], b->aux[k]), "join", &msg)) return false;
       if (!hip_ok(hipStreamWaitEvent(st, b->join[k], 0), "join wait", &msg)) return false;
     }
-  } else {
-    if (!hip_ok(launch_batch_reset(st, b->d_counter), "batch reset", &msg)) return false;
-    if (!hip_ok(launch_batch_render(b->maxb, b->cus * b->blocks_per_cu, st, b->d_jobs, (uint32_t)b->jobs.size(),
-                                    b->total_tiles, b->d_counter, b->lds_bytes),
-                "batch render", &msg))
-      return false;
+  } else if (!enqueue_persistent(b, st, msg)) {
+    return false;
   }
   return hip_ok(launch_batch_post(st, b->d_posts, (uint32_t)b->posts.size(), b->total_rows, b->max_w, b->max_radius),
                 "batch post", &msg);
@@ -197,6 +223,7 @@ bool enqueue(mm3e_batch* b, std::string& msg) {
 
 bool upload(mm3e_batch* b, std::string& msg) {
   hipStream_t st = b->r->stream;
+  if (!ensure_order(b, msg)) return false;
   if (!b->jobs.empty() &&
       !hip_ok(hipMemcpyAsync(b->d_jobs, b->jobs.data(), b->jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice, st),
               "upload jobs", &msg))
@@ -249,6 +276,7 @@ void mm3e_batch_destroy(mm3e_batch* b) {
   if (b->d_jobs) hipFree(b->d_jobs);
   if (b->d_posts) hipFree(b->d_posts);
   if (b->d_counter) hipFree(b->d_counter);
+  if (b->d_tq) hipFree(b->d_tq);
   if (b->d_arena) hipFree(b->d_arena);
   if (b->ev0) hipEventDestroy(b->ev0);
   if (b->ev1) hipEventDestroy(b->ev1);

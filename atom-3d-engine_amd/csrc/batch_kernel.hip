@@ -15,15 +15,22 @@
 
 using namespace mm3e;
 
+// With q.order, queue position k holds tile q.order[k] (most expensive first, from the previous
+// replay's costs) and each tile's cost (tickets drawn meanwhile from q.ticket) goes to q.cost;
+// without, tiles are taken in index order.
 template <bool STAGED>
 __device__ __forceinline__ void batch_render_body(const BatchJob* __restrict__ jobs, uint32_t n_jobs,
                                                   uint32_t total_tiles, uint32_t* __restrict__ counter,
-                                                  uint4* tables_lds) {
+                                                  const TileOrder& q, uint4* tables_lds) {
   __shared__ float smem[256 * 3];
-  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_tile, s_start;
   const DevScene* staged = nullptr;
   for (;;) {
-    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+    if (threadIdx.x == 0) {
+      const uint32_t k = atomicAdd(counter, 1u);
+      s_tile = (k < total_tiles && q.order) ? q.order[k] : k;
+      if (q.order) s_start = atomicAdd(q.ticket, 1u);
+    }
     __syncthreads();
     const uint32_t tile = s_tile;
     __syncthreads();  // every lane has read s_tile before lane 0 may overwrite it
@@ -51,32 +58,33 @@ __device__ __forceinline__ void batch_render_body(const BatchJob* __restrict__ j
       render_tile<TableMap, MM3E_MAXB, true>(sc, global_tables(sc), J.fp, J.hdr, J.tw, J.th, tx, ty, smem);
     }
     __syncthreads();  // smem is reused by the next tile
+    if (q.order && threadIdx.x == 0) q.cost[tile] = atomicAdd(q.ticket, 1u) - s_start;
   }
 }
 
 extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
     MM3E_CAT(mm3e_batch_render_lds_b, MM3E_MAXB)(const BatchJob* __restrict__ jobs, uint32_t n_jobs,
-                                                 uint32_t total_tiles, uint32_t* __restrict__ counter) {
+                                                 uint32_t total_tiles, uint32_t* __restrict__ counter, TileOrder q) {
   extern __shared__ uint4 tables_lds[];
-  batch_render_body<true>(jobs, n_jobs, total_tiles, counter, tables_lds);
+  batch_render_body<true>(jobs, n_jobs, total_tiles, counter, q, tables_lds);
 }
 
 extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
     MM3E_CAT(mm3e_batch_render_table_b, MM3E_MAXB)(const BatchJob* __restrict__ jobs, uint32_t n_jobs,
-                                                   uint32_t total_tiles, uint32_t* __restrict__ counter) {
-  batch_render_body<false>(jobs, n_jobs, total_tiles, counter, nullptr);
+                                                   uint32_t total_tiles, uint32_t* __restrict__ counter, TileOrder q) {
+  batch_render_body<false>(jobs, n_jobs, total_tiles, counter, q, nullptr);
 }
 
 namespace mm3e {
 hipError_t MM3E_CAT(launch_batch_render_b, MM3E_MAXB)(uint32_t blocks, hipStream_t st, const BatchJob* jobs,
                                                       uint32_t n_jobs, uint32_t total_tiles, uint32_t* counter,
-                                                      uint32_t lds_bytes) {
+                                                      uint32_t lds_bytes, const TileOrder& q) {
   if (lds_bytes)
     hipLaunchKernelGGL(MM3E_CAT(mm3e_batch_render_lds_b, MM3E_MAXB), dim3(blocks), dim3(256), lds_bytes, st, jobs,
-                       n_jobs, total_tiles, counter);
+                       n_jobs, total_tiles, counter, q);
   else
     hipLaunchKernelGGL(MM3E_CAT(mm3e_batch_render_table_b, MM3E_MAXB), dim3(blocks), dim3(256), 0, st, jobs, n_jobs,
-                       total_tiles, counter);
+                       total_tiles, counter, q);
   return hipGetLastError();
 }
 }  // namespace mm3e

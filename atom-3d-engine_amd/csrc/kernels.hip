@@ -194,13 +194,13 @@ hipError_t launch_batch_reset(hipStream_t st, uint32_t* counter) {
 }
 
 hipError_t launch_batch_render(uint32_t maxb, uint32_t blocks, hipStream_t st, const BatchJob* jobs, uint32_t n_jobs,
-                               uint32_t total_tiles, uint32_t* counter, uint32_t lds_bytes) {
+                               uint32_t total_tiles, uint32_t* counter, uint32_t lds_bytes, const TileOrder& q) {
   switch (maxb) {
     case 0:
-    case 1: return launch_batch_render_b1(blocks, st, jobs, n_jobs, total_tiles, counter, lds_bytes);
-    case 2: return launch_batch_render_b2(blocks, st, jobs, n_jobs, total_tiles, counter, lds_bytes);
+    case 1: return launch_batch_render_b1(blocks, st, jobs, n_jobs, total_tiles, counter, lds_bytes, q);
+    case 2: return launch_batch_render_b2(blocks, st, jobs, n_jobs, total_tiles, counter, lds_bytes, q);
     case 3:
-    case 4: return launch_batch_render_b4(blocks, st, jobs, n_jobs, total_tiles, counter, lds_bytes);
+    case 4: return launch_batch_render_b4(blocks, st, jobs, n_jobs, total_tiles, counter, lds_bytes, q);
     default: return hipErrorInvalidValue;
   }
 }

@@ -13,14 +13,17 @@ struct BatchPost;
 
 #define MM3E_DECL_BATCH(B)                                                                                   \
   hipError_t launch_batch_render_b##B(uint32_t blocks, hipStream_t st, const BatchJob* jobs, uint32_t n_jobs, \
-                                      uint32_t total_tiles, uint32_t* counter, uint32_t lds_bytes);
+                                      uint32_t total_tiles, uint32_t* counter, uint32_t lds_bytes,            \
+                                      const TileOrder& q);
 MM3E_DECL_BATCH(1)
 MM3E_DECL_BATCH(2)
 MM3E_DECL_BATCH(4)
 #undef MM3E_DECL_BATCH
 // Persistent batched render; maxb is the batch's largest `bounces` (variants 1, 2, 4 cover 0..4).
+// q.order == nullptr: tiles in index order, no cost record.
 hipError_t launch_batch_render(uint32_t maxb, uint32_t blocks, hipStream_t st, const BatchJob* jobs, uint32_t n_jobs,
-                               uint32_t total_tiles, uint32_t* counter, uint32_t lds_bytes);
+                               uint32_t total_tiles, uint32_t* counter, uint32_t lds_bytes,
+                               const TileOrder& q = TileOrder{});
 hipError_t launch_batch_post(hipStream_t st, const BatchPost* posts, uint32_t n, uint32_t total_rows, uint32_t max_w,
                              uint32_t max_radius);
 hipError_t launch_batch_reset(hipStream_t st, uint32_t* counter);
