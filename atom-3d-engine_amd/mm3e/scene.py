@@ -342,6 +342,60 @@ class Marcher:
 
 
 @dataclass
+class Quality:
+    """Quality presets (mm3e-orchestrator/src/lib.rs:67-149): resolution and the marcher / AA /
+    reflection budgets, written into a scene with `apply`."""
+    width: int
+    height: int
+    aa: int
+    bounces: int
+    max_steps: int
+    shadow_steps: int
+    ao_samples: int
+    lod_footprint: float
+
+    @staticmethod
+    def fast(width: int, height: int) -> "Quality":
+        """Low resolution preview: no AA, no reflections, lean shadows, AO off, LOD on."""
+        return Quality(width, height, 1, 0, 64, 12, 0, float(F(0.006)))
+
+    @staticmethod
+    def balanced(width: int, height: int) -> "Quality":
+        return Quality(width, height, 1, 1, 110, 28, 3, float(F(0.002)))
+
+    @staticmethod
+    def full(width: int, height: int) -> "Quality":
+        """Converged stills: LOD off (exact)."""
+        return Quality(width, height, 2, 2, 160, 64, 5, 0.0)
+
+    @staticmethod
+    def lerp(a: "Quality", b: "Quality", t: float) -> "Quality":
+        """Budgets from `a` toward `b` by t in [0, 1] in f32, rounded half away from zero;
+        the resolution is b's (lib.rs:117-136)."""
+        t = F(min(max(F(t), F(0.0)), F(1.0)))
+
+        def mix(x, y):
+            v = F(F(x) + F(F(F(y) - F(x)) * t))
+            r = np.floor(np.abs(v) + F(0.5)) * np.sign(v)  # f32::round: half away from zero
+            return max(int(r), 0)  # `as u32` saturates at 0
+
+        def mixf(x, y):
+            return float(F(F(x) + F(F(F(y) - F(x)) * t)))
+
+        return Quality(b.width, b.height, mix(a.aa, b.aa), mix(a.bounces, b.bounces),
+                       mix(a.max_steps, b.max_steps), mix(a.shadow_steps, b.shadow_steps),
+                       mix(a.ao_samples, b.ao_samples), mixf(a.lod_footprint, b.lod_footprint))
+
+    def apply(self, scene: "Scene") -> None:
+        scene.width, scene.height = self.width, self.height
+        scene.aa, scene.bounces = self.aa, self.bounces
+        scene.marcher.max_steps = self.max_steps
+        scene.marcher.shadow_steps = self.shadow_steps
+        scene.marcher.ao_samples = self.ao_samples
+        scene.marcher.lod_footprint = self.lod_footprint
+
+
+@dataclass
 class Post:
     """post.rs:25-29."""
     exposure: float = 1.0

@@ -459,3 +459,14 @@ def test_mixed_batch_with_generated_kernels_equals_per_frame_renders():
         g.close()
     plain.close()
     r.close()
+
+
+@pytest.mark.parametrize("preset", ["fast", "balanced"])
+def test_quality_presets_parity(renderer, preset):
+    """Quality::fast / balanced (lib.rs:82-107) written into a procedural scene: LOD footprint
+    on (eps grows with distance), lean shadow budgets, AO off or 3 taps."""
+    from mm3e.scene import Quality
+    sc = configs.procedural_scene(47, 60, 320, 180)
+    getattr(Quality, preset)(320, 180).apply(sc)
+    assert sc.marcher.lod_footprint > 0.0
+    image_parity(renderer, sc, configs.orbit_pose(configs.SplitMix64(47)), f"quality {preset}")

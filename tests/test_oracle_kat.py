@@ -584,3 +584,24 @@ def test_mode_depth_and_ao_closed_forms():
     assert (img[~hit][:, :3] == 0).all()  # misses end beyond max_dist
     _, _, img, hit = _plane_view(MODE_AO)
     assert (img[..., :3] == 255).all()    # an open plane occludes nothing; misses report 1
+
+
+# ---- Quality presets (lib.rs:67-149; engine.rs:255-271) ----------------------------------------
+
+def test_quality_presets_and_lerp():
+    from mm3e.scene import Quality
+    fast = Quality.fast(320, 180)
+    full = Quality.full(960, 540)
+    assert fast.aa <= full.aa and fast.bounces <= full.bounces and fast.shadow_steps < full.shadow_steps
+    mid = Quality.lerp(fast, full, 0.5)
+    assert (mid.width, mid.height) == (960, 540)
+    assert fast.shadow_steps < mid.shadow_steps < full.shadow_steps
+    s = Scene(10, 10)
+    full.apply(s)
+    assert (s.width, s.height, s.aa, s.bounces) == (960, 540, 2, 2)
+    assert (s.marcher.shadow_steps, s.marcher.ao_samples) == (64, 5)
+    # closed form of the f32 mix with half-away rounding: 64 + (160 - 64) * 0.5 = 112,
+    # 12 + 52 * 0.5 = 38, aa 1 + 0.5 = 1.5 -> 2, bounces 0 + 2 * 0.5 = 1, ao 0 + 2.5 -> 3
+    assert (mid.max_steps, mid.shadow_steps, mid.aa, mid.bounces, mid.ao_samples) == (112, 38, 2, 1, 3)
+    assert mid.lod_footprint == float(np.float32(np.float32(0.006) + np.float32(-0.006) * np.float32(0.5)))
+    assert Quality.lerp(fast, full, -1.0).max_steps == 64 and Quality.lerp(fast, full, 7.0).max_steps == 160
