@@ -1,6 +1,6 @@
 // batch.cpp — batched rendering of many (scene, camera, size) jobs in one persistent launch.
 // When some jobs' scenes have generated kernels, the autotuner also times a mixed mode: those
-// jobs as their own launches spread over four streams, beside the persistent launch for the
+// jobs as their own launches spread over four streams, then the persistent launch for the
 // rest, and keeps whichever is faster.
 //
 // A batch owns a job table in HBM, per-job framebuffers (HDR, bloom scratch, RGBA8) carved
@@ -44,8 +44,8 @@ struct mm3e_batch {
   bool launched = false;
   uint64_t total_pixels = 0;
   // Mixed mode: jobs whose scene has a generated kernel run as their own launches, spread over
-  // kStreams streams, while the other jobs' tiles go through the persistent table kernel on
-  // one of them; the batched post waits for all.  `gen` holds those jobs' table positions.
+  // kStreams streams; then the other jobs' tiles go through the persistent table kernel and the
+  // batched post.  `gen` holds those jobs' table positions.
   bool mixed = false;
   std::vector<uint32_t> gen;                 // positions (in posts) of generated-kernel jobs
   std::vector<mm3e::BatchJob> gen_jobs;      // their launch records (tile shape, camera, hdr)
@@ -191,13 +191,12 @@ bool enqueue(mm3e_batch* b, std::string& msg) {
     if (!hip_ok(hipEventRecord(b->fork, st), "fork", &msg)) return false;
     for (int k = 0; k < mm3e_batch::kStreams - 1; k++)
       if (!hip_ok(hipStreamWaitEvent(b->aux[k], b->fork, 0), "fork wait", &msg)) return false;
-    if (!b->jobs.empty() && !enqueue_persistent(b, b->aux[0], msg)) return false;
-    // generated jobs round-robin over the renderer's stream and the other aux streams
+    // generated jobs round-robin over the renderer's stream and the aux streams
     for (size_t i = 0; i < b->gen_jobs.size(); i++) {
       const BatchJob& J = b->gen_jobs[i];
       const mm3e_scene* s = b->in[b->order[b->gen[i]]].scene;
-      const int lane = (int)(i % (mm3e_batch::kStreams - 1));
-      hipStream_t gs = lane == 0 ? st : b->aux[lane];
+      const int lane = (int)(i % mm3e_batch::kStreams);
+      hipStream_t gs = lane == 0 ? st : b->aux[lane - 1];
       const DevScene* sc_arg = J.sc;
       FrameParams fp_arg = J.fp;
       float* hdr_arg = J.hdr;
@@ -209,11 +208,12 @@ bool enqueue(mm3e_batch* b, std::string& msg) {
                   "generated batch job", &msg))
         return false;
     }
-    // join
+    // join, then the table jobs through the persistent launch on the whole chip
     for (int k = 0; k < mm3e_batch::kStreams - 1; k++) {
       if (!hip_ok(hipEventRecord(b->join[k], b->aux[k]), "join", &msg)) return false;
       if (!hip_ok(hipStreamWaitEvent(st, b->join[k], 0), "join wait", &msg)) return false;
     }
+    if (!b->jobs.empty() && !enqueue_persistent(b, st, msg)) return false;
   } else if (!enqueue_persistent(b, st, msg)) {
     return false;
   }
