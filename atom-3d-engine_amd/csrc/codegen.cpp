@@ -68,7 +68,7 @@ bool generated_needs_tables(const Blob& b) {
 
 bool generated_prefers_sm(const Blob& b) { return b.objs.size() > 24 || b.header.grid != nullptr; }
 
-std::string emit_scene_source(const Blob& b, uint32_t maxb, bool sm) {
+std::string emit_scene_source(const Blob& b, uint32_t maxb, bool sm, bool queued) {
   const char* csg = has_csg(b) ? "true" : "false";
   std::string o;
   o.reserve(4096 + b.objs.size() * 900);
@@ -84,7 +84,9 @@ std::string emit_scene_source(const Blob& b, uint32_t maxb, bool sm) {
     if (n >= 1 && n <= 8) waves = n;
   }
   o += "#define MM3E_RENDER_WAVES " + std::to_string(waves) + "\n";
-  o += "#ifndef __HIPCC_RTC__\n#include <hip/hip_runtime.h>\n#endif\n#include \"dev_render.h\"\nnamespace mm3e {\n";
+  o += "#ifndef __HIPCC_RTC__\n#include <hip/hip_runtime.h>\n#endif\n#include \"dev_render.h\"\n";
+  if (queued) o += "#include \"dev_wavefront.h\"\n";
+  o += "namespace mm3e {\n";
   o += "struct GenMap {\n  const DevScene* sc;\n  Tables tb;\n  MM3E_DEV Field operator()(V3 p) const;\n};\n";
   // The fold is the loop of world_eval with the per-segment body specialised: a uniform switch
   // on the segment index holds each One / linear-run segment's code once; tree runs share the
@@ -158,6 +160,19 @@ std::string emit_scene_source(const Blob& b, uint32_t maxb, bool sm) {
        "  const Tables tb = stage_bytes ? stage_tables(sc, tables_lds, stage_bytes) : global_tables(sc);\n";
   o += "  render_ordered<GenMap, " + std::to_string(maxb) + ", " + (sm ? "true" : "false") +
        ">(sc, tb, fp, hdr, tw, th, o, smem, &start);\n}\n";
+  if (queued) {
+    const std::string tbl = "  extern __shared__ uint4 tables_lds[];\n"
+                            "  const Tables tb = stage_bytes ? stage_tables(sc, tables_lds, stage_bytes) : global_tables(sc);\n";
+    o += "extern \"C\" __global__ void __launch_bounds__(256)\n"
+         "    mm3e_gen_wf_first(const DevScene* __restrict__ sc, FrameParams fp, WfFrame w, uint32_t budget,\n"
+         "                      uint32_t stage_bytes) {\n" + tbl + "  wf_march_first<GenMap>(sc, tb, fp, w, budget);\n}\n";
+    o += "extern \"C\" __global__ void __launch_bounds__(256)\n"
+         "    mm3e_gen_wf_cont(const DevScene* __restrict__ sc, FrameParams fp, WfFrame w, uint32_t k, uint32_t budget,\n"
+         "                     uint32_t stage_bytes) {\n" + tbl + "  wf_march_cont<GenMap>(sc, tb, fp, w, k, budget);\n}\n";
+    o += "extern \"C\" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY\n"
+         "    mm3e_gen_wf_shade(const DevScene* __restrict__ sc, FrameParams fp, WfFrame w, uint32_t stage_bytes) {\n" +
+         tbl + "  wf_shade<GenMap, " + std::to_string(maxb) + ", " + (sm ? "true" : "false") + ">(sc, tb, fp, w);\n}\n";
+  }
   return o;
 }
 

@@ -322,14 +322,11 @@ MM3E_DEV void path_transition(const DevScene& sc, const Tables& tb, PathState& p
 #define MM3E_PATH_GATE 1
 #endif
 
-// HIT0 also returns the primary (depth-0) march's hit record in *hit0.
-template <class Map, int MAXB, bool HIT0 = false>
-MM3E_DEV V3 trace_sm(const DevScene& sc, const Map& field, V3 ro, V3 rd, Hit* hit0 = nullptr) {
-  PathState ps;
-  LevelStack<MAXB> stk;
-  ps.level = 0;
-  path_march<MAXB>(sc, ps, ro, rd);
-  bool waiting = ps.mst == MARCH_DONE;
+// Runs a lane's path to PH_DONE from its current state (`waiting`: its search has finished and
+// its transition is due).  HIT0 also returns the primary (depth-0) march's hit record in *hit0.
+template <class Map, int MAXB, bool HIT0>
+MM3E_DEV void path_run(const DevScene& sc, const Map& field, PathState& ps, LevelStack<MAXB>& stk, bool waiting,
+                       Hit* hit0) {
   for (;;) {
     const bool searching = ps.phase != PH_DONE && !waiting;
     if (searching) waiting = path_search_step<MAXB>(sc, ps, field(ps.q));
@@ -349,6 +346,15 @@ MM3E_DEV V3 trace_sm(const DevScene& sc, const Map& field, V3 ro, V3 rd, Hit* hi
     }
     if (__builtin_amdgcn_ballot_w64(ps.phase != PH_DONE) == 0) break;
   }
+}
+
+template <class Map, int MAXB, bool HIT0 = false>
+MM3E_DEV V3 trace_sm(const DevScene& sc, const Map& field, V3 ro, V3 rd, Hit* hit0 = nullptr) {
+  PathState ps;
+  LevelStack<MAXB> stk;
+  ps.level = 0;
+  path_march<MAXB>(sc, ps, ro, rd);
+  path_run<Map, MAXB, HIT0>(sc, field, ps, stk, ps.mst == MARCH_DONE, hit0);
   return ps.radiance;
 }
 

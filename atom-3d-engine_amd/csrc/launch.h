@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "dev_render.h"
+#include "dev_wavefront.h"
 
 namespace mm3e {
 
@@ -53,6 +54,24 @@ MM3E_DECL_ORD(4)
 hipError_t launch_render_ordered(uint32_t maxb, uint32_t n_tiles, dim3 block, hipStream_t st, const DevScene* sc,
                                  uint32_t stage_bytes, bool general, const FrameParams& fp, float* hdr, uint32_t tw,
                                  uint32_t th, const TileOrder& o);
+#define MM3E_DECL_WF(B)                                                                                       \
+  hipError_t launch_wf_shade_b##B(dim3 grid, hipStream_t st, const DevScene* sc, uint32_t stage_bytes, bool general, \
+                                  const FrameParams& fp, const WfFrame& w);
+MM3E_DECL_WF(0)
+MM3E_DECL_WF(1)
+MM3E_DECL_WF(2)
+MM3E_DECL_WF(3)
+MM3E_DECL_WF(4)
+#undef MM3E_DECL_WF
+// The queued frame (dev_wavefront.h): counter reset; march pass k (0: one block per tile of
+// tw*th*aa*aa lanes; k >= 1: grid-stride over queue k-1); shading of the hit queue
+// (grid-stride, 256 lanes a block); per-pixel resolve into hdr.
+hipError_t launch_wf_reset(hipStream_t st, uint32_t* cnt);
+hipError_t launch_wf_march(hipStream_t st, bool general, uint32_t k, dim3 grid, dim3 block, const DevScene* sc,
+                           uint32_t stage_bytes, const FrameParams& fp, const WfFrame& w, uint32_t budget);
+hipError_t launch_wf_shade(uint32_t maxb, dim3 grid, hipStream_t st, const DevScene* sc, uint32_t stage_bytes,
+                           bool general, const FrameParams& fp, const WfFrame& w);
+hipError_t launch_wf_resolve(hipStream_t st, const FrameParams& fp, const WfFrame& w, uint32_t aa, float* hdr);
 // stage_bytes > 0 selects the variant that copies the scene tables into LDS first; general
 // selects the kernels with the segment grid and CSG (scene_is_general).
 hipError_t launch_render_table(uint32_t maxb, dim3 grid, dim3 block, hipStream_t st, const DevScene* sc,

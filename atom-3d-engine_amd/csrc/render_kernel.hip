@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include "dev_render.h"
+#include "dev_wavefront.h"
 #include "launch.h"
 
 #ifndef MM3E_MAXB
@@ -24,6 +25,9 @@
 #define ONAME MM3E_CAT(mm3e_render_ord_b, MM3E_MAXB)
 #define ONAME_X MM3E_CAT(mm3e_render_ord_x_b, MM3E_MAXB)
 #define OLNAME MM3E_CAT(launch_render_ordered_b, MM3E_MAXB)
+#define WNAME MM3E_CAT(mm3e_wf_shade_b, MM3E_MAXB)
+#define WNAME_X MM3E_CAT(mm3e_wf_shade_x_b, MM3E_MAXB)
+#define WLNAME MM3E_CAT(launch_wf_shade_b, MM3E_MAXB)
 
 using namespace mm3e;
 
@@ -81,7 +85,30 @@ extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
   render_ordered<TableMap, MM3E_MAXB, true>(sc, tb, fp, hdr, tw, th, o, smem, &start);
 }
 
+// Shading stage of the queued frame (dev_wavefront.h): the hit queue, nested loops or the
+// per-lane state machine.
+extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
+    WNAME(const DevScene* __restrict__ sc, FrameParams fp, WfFrame w, uint32_t stage_bytes) {
+  extern __shared__ uint4 tables_lds[];
+  const Tables tb = stage_bytes ? stage_tables(sc, tables_lds, stage_bytes) : global_tables(sc);
+  wf_shade<SimpleMap, MM3E_MAXB, false>(sc, tb, fp, w);
+}
+
+extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
+    WNAME_X(const DevScene* __restrict__ sc, FrameParams fp, WfFrame w, uint32_t stage_bytes) {
+  extern __shared__ uint4 tables_lds[];
+  const Tables tb = stage_bytes ? stage_tables(sc, tables_lds, stage_bytes) : global_tables(sc);
+  wf_shade<TableMap, MM3E_MAXB, true>(sc, tb, fp, w);
+}
+
 namespace mm3e {
+hipError_t WLNAME(dim3 grid, hipStream_t st, const DevScene* sc, uint32_t stage_bytes, bool general,
+                  const FrameParams& fp, const WfFrame& w) {
+  if (general) hipLaunchKernelGGL(WNAME_X, grid, dim3(256), stage_bytes, st, sc, fp, w, stage_bytes);
+  else hipLaunchKernelGGL(WNAME, grid, dim3(256), stage_bytes, st, sc, fp, w, stage_bytes);
+  return hipGetLastError();
+}
+
 hipError_t OLNAME(uint32_t n_tiles, dim3 block, hipStream_t st, const DevScene* sc, uint32_t stage_bytes, bool general,
                   const FrameParams& fp, float* hdr, uint32_t tw, uint32_t th, const TileOrder& o) {
   if (general) hipLaunchKernelGGL(ONAME_X, dim3(n_tiles), block, stage_bytes, st, sc, fp, hdr, tw, th, stage_bytes, o);

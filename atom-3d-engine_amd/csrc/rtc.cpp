@@ -21,6 +21,7 @@ struct Entry {
   hipModule_t mod = nullptr;
   hipFunction_t fn = nullptr;
   hipFunction_t fn_o = nullptr;  // the ordered-tiles variant
+  hipFunction_t fn_wf[3] = {nullptr, nullptr, nullptr};
   std::string err;
 };
 
@@ -69,7 +70,8 @@ bool compile(const std::string& src, std::vector<char>& code, std::string& err) 
 
 }  // namespace
 
-bool rtc_kernel(const std::string& src, int device, hipFunction_t* fn, hipFunction_t* fn_o, std::string& err) {
+bool rtc_kernel(const std::string& src, int device, hipFunction_t* fn, hipFunction_t* fn_o, std::string& err,
+                hipFunction_t* fn_wf) {
   std::shared_ptr<Entry> e;
   {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -90,6 +92,9 @@ bool rtc_kernel(const std::string& src, int device, hipFunction_t* fn, hipFuncti
       e->err = "generated module has no mm3e_gen_render";
     }
     if (e->ok && hipModuleGetFunction(&e->fn_o, e->mod, "mm3e_gen_render_o") != hipSuccess) e->fn_o = nullptr;
+    static const char* wf_names[3] = {"mm3e_gen_wf_first", "mm3e_gen_wf_cont", "mm3e_gen_wf_shade"};
+    for (int k = 0; k < 3 && e->ok && src.find(wf_names[k]) != std::string::npos; k++)
+      if (hipModuleGetFunction(&e->fn_wf[k], e->mod, wf_names[k]) != hipSuccess) e->fn_wf[k] = nullptr;
     e->done = true;
   }
   if (!e->ok) {
@@ -98,6 +103,8 @@ bool rtc_kernel(const std::string& src, int device, hipFunction_t* fn, hipFuncti
   }
   *fn = e->fn;
   if (fn_o) *fn_o = e->fn_o;
+  if (fn_wf)
+    for (int k = 0; k < 3; k++) fn_wf[k] = e->fn_wf[k];
   return true;
 }
 

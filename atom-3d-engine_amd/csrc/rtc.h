@@ -10,7 +10,10 @@ namespace mm3e {
 // Compile `src` for gfx950 with the same floating-point contract as the library's own kernels
 // (-O3, no contraction, correctly rounded division/sqrt, no packed FP32), load it on `device`
 // (must be current) and return its `mm3e_gen_render` and, when fn_o is given, its ordered-tiles
-// variant `mm3e_gen_render_o` (nullptr if the source has none).  Identical sources compile once.
-bool rtc_kernel(const std::string& src, int device, hipFunction_t* fn, hipFunction_t* fn_o, std::string& err);
+// variant `mm3e_gen_render_o` (nullptr if the source has none) and, when fn_wf is given, the
+// queued-frame stages mm3e_gen_wf_first / _cont / _shade (nullptr each if absent).  Identical
+// sources compile once.
+bool rtc_kernel(const std::string& src, int device, hipFunction_t* fn, hipFunction_t* fn_o, std::string& err,
+                hipFunction_t* fn_wf = nullptr);
 
 }  // namespace mm3e

@@ -169,11 +169,13 @@ static void gen_run(mm3e::GenJob& j) {
   hipFunction_t fn = nullptr;
   std::string err;
   hipFunction_t fn_o = nullptr;
-  const bool ok = rtc_kernel(j.src, j.device, &fn, &fn_o, err);
+  hipFunction_t fn_wf[3] = {nullptr, nullptr, nullptr};
+  const bool ok = rtc_kernel(j.src, j.device, &fn, &fn_o, err, fn_wf);
   {
     std::lock_guard<std::mutex> lk(j.mu);
     j.fn = fn;
     j.fn_o = fn_o;
+    for (int k = 0; k < 3; k++) j.fn_wf[k] = fn_wf[k];
     j.err = err;
     j.state.store(ok ? MM3E_CODEGEN_READY : MM3E_CODEGEN_FAILED, std::memory_order_release);
   }
@@ -206,7 +208,7 @@ static void gen_schedule(mm3e_scene* s) {
   Blob b;
   build_blob(s->host, {}, {}, s->aa, s->bounces, (uint64_t)(uintptr_t)s->d_tables, s->d_gi, false, b);
   auto j = std::make_shared<GenJob>();
-  j->src = emit_scene_source(b, s->bounces, generated_prefers_sm(b));
+  j->src = emit_scene_source(b, s->bounces, generated_prefers_sm(b), r->wf_budget > 0);
   j->device = r->device;
   s->gen_stage = generated_needs_tables(b) ? s->stage_bytes : 0u;
   s->gen = j;
@@ -245,6 +247,101 @@ static bool tile_order_ready(mm3e_scene* s, uint32_t n, std::string& msg) {
   return true;
 }
 
+// The queued-frame setting in force for a scene: its own, else the renderer's.
+static void wf_settings(const mm3e_scene* s, uint32_t* budget, uint32_t* passes, uint32_t* bpc) {
+  *budget = s->wf_own ? s->wf_set[0] : s->r->wf_budget;
+  *passes = s->wf_own ? s->wf_set[1] : s->r->wf_passes;
+  *bpc = s->wf_own ? s->wf_set[2] : s->r->wf_bpc;
+}
+
+// Queued-frame buffers for `cap` sample slots, one allocation:
+// [mq0][mq1] WF_STATE_WORDS*cap words each, [mslot0][mslot1][hq 4*cap][hslot][hmat][rad 3*cap][cnt].
+static bool wf_ready(mm3e_scene* s, uint32_t cap, std::string& msg) {
+  if (s->d_wf && s->wf_cap == cap) return true;
+  if (s->d_wf) hipFree(s->d_wf);
+  s->d_wf = nullptr;
+  s->wf_cap = 0;
+  const size_t words = (size_t)cap * (2 * WF_STATE_WORDS + 2 + WF_HIT_WORDS + 2 + 3) + 16;
+  if (!hip_ok(hipMalloc((void**)&s->d_wf, words * 4), "hipMalloc(queued frame)", &msg)) {
+    s->d_wf = nullptr;
+    return false;
+  }
+  s->wf_cap = cap;
+  return true;
+}
+
+static WfFrame wf_frame(mm3e_scene* s, uint32_t tiles_x) {
+  WfFrame w{};
+  const size_t c = s->wf_cap;
+  float* f = (float*)s->d_wf;
+  w.mq[0] = f;
+  w.mq[1] = f + WF_STATE_WORDS * c;
+  uint32_t* u = (uint32_t*)(f + 2 * WF_STATE_WORDS * c);
+  w.mslot[0] = u;
+  w.mslot[1] = u + c;
+  w.hq = (float*)(u + 2 * c);
+  w.hslot = (uint32_t*)(w.hq + WF_HIT_WORDS * c);
+  w.hmat = w.hslot + c;
+  w.rad = (float*)(w.hmat + c);
+  w.cnt = (uint32_t*)(w.rad + 3 * c);
+  w.cap = s->wf_cap;
+  w.tw = s->tw;
+  w.th = s->th;
+  w.tiles_x = tiles_x;
+  return w;
+}
+
+// A frame as queued stages (dev_wavefront.h) into s->d_hdr: budgeted march passes over
+// compacted continuation queues, shading of the compacted hit queue, per-pixel resolve.
+static bool launch_queued(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, dim3 grid, dim3 block,
+                          uint32_t stage, bool general, std::string& msg) {
+  mm3e_renderer* r = s->r;
+  hipStream_t st = r->stream;
+  const uint32_t cap = grid.x * grid.y * block.x;
+  if (!wf_ready(s, cap, msg)) return false;
+  WfFrame w = wf_frame(s, grid.x);
+  uint32_t budget, passes, bpc;
+  wf_settings(s, &budget, &passes, &bpc);
+  // grid-stride stages: cus * bpc blocks of 256, or enough for one lane per slot
+  const uint32_t full = (cap + 255) / 256;
+  const uint32_t gs = bpc ? std::min(full, r->cus * bpc) : full;
+  const mm3e::GenJob* gj = s->gen.get();
+  const bool gen = dsc == s->d_scene && gj && gj->state.load(std::memory_order_acquire) == MM3E_CODEGEN_READY &&
+                   gj->fn_wf[0] && gj->fn_wf[1] && gj->fn_wf[2];
+  uint32_t gstage = gen ? s->gen_stage : stage;
+  const DevScene* sc_arg = dsc;
+  FrameParams fp_arg = fp;
+  if (!hip_ok(launch_wf_reset(st, w.cnt), "queued frame reset", &msg)) return false;
+  for (uint32_t k = 0; k <= passes; k++) {
+    uint32_t b = k == passes ? 0u : budget;  // the last pass runs every march to its end
+    uint32_t kk = k;
+    const dim3 g = k == 0 ? dim3(grid.x * grid.y) : dim3(gs);
+    const dim3 bl = k == 0 ? block : dim3(256);
+    hipError_t e;
+    if (gen) {
+      if (k == 0) {
+        void* args[] = {&sc_arg, &fp_arg, &w, &b, &gstage};
+        e = hipModuleLaunchKernel(gj->fn_wf[0], g.x, 1, 1, bl.x, 1, 1, gstage, st, args, nullptr);
+      } else {
+        void* args[] = {&sc_arg, &fp_arg, &w, &kk, &b, &gstage};
+        e = hipModuleLaunchKernel(gj->fn_wf[1], g.x, 1, 1, bl.x, 1, 1, gstage, st, args, nullptr);
+      }
+    } else {
+      e = launch_wf_march(st, general, k, g, bl, dsc, stage, fp, w, b);
+    }
+    if (!hip_ok(e, "queued march launch", &msg)) return false;
+  }
+  hipError_t e;
+  if (gen) {
+    void* args[] = {&sc_arg, &fp_arg, &w, &gstage};
+    e = hipModuleLaunchKernel(gj->fn_wf[2], gs, 1, 1, 256, 1, 1, gstage, st, args, nullptr);
+  } else {
+    e = launch_wf_shade(s->bounces, dim3(gs), st, dsc, stage, general, fp, w);
+  }
+  if (!hip_ok(e, "queued shade launch", &msg)) return false;
+  return hip_ok(launch_wf_resolve(st, fp, w, s->aa, s->d_hdr), "queued resolve launch", &msg);
+}
+
 bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std::string& msg) {
   hipStream_t st = s->r->stream;
   const uint32_t stage = dsc == s->d_scene ? s->stage_bytes : s->dyn_stage_bytes;
@@ -264,7 +361,11 @@ bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std
   }
   const mm3e::GenJob* gj = s->gen.get();
   bool ordered = false;
-  if (dsc == s->d_scene && gj && gj->state.load(std::memory_order_acquire) == MM3E_CODEGEN_READY) {
+  uint32_t wf_b, wf_n, wf_c;
+  wf_settings(s, &wf_b, &wf_n, &wf_c);
+  if (wf_b > 0) {
+    if (!launch_queued(s, dsc, fp, grid, block, stage, general, msg)) return false;
+  } else if (dsc == s->d_scene && gj && gj->state.load(std::memory_order_acquire) == MM3E_CODEGEN_READY) {
     const DevScene* sc_arg = dsc;
     FrameParams fp_arg = fp;
     float* hdr_arg = s->d_hdr;
@@ -308,6 +409,20 @@ bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std
 using namespace mm3e;
 
 extern "C" {
+
+// Queued-frame default from MM3E_WAVEFRONT="budget[,passes[,blocks_per_cu]]"; out of range or
+// unset leaves the arguments unchanged.
+static void env_wavefront(uint32_t* budget, uint32_t* passes, uint32_t* bpc) {
+  const char* wf = std::getenv("MM3E_WAVEFRONT");
+  if (!wf) return;
+  unsigned b = 0, n = *passes, c = *bpc;
+  const int got = std::sscanf(wf, "%u,%u,%u", &b, &n, &c);
+  if (got >= 1 && n + 1 <= WF_MAX_PASSES && c <= 64) {
+    *budget = b;
+    *passes = n;
+    *bpc = c;
+  }
+}
 
 int mm3e_list_adapters(char* buf, size_t buflen) {
   int n = 0;
@@ -364,6 +479,8 @@ int mm3e_renderer_create(const char* adapter_filter, mm3e_renderer** out, char* 
     else r->codegen = MM3E_CODEGEN_BACKGROUND;
   }
   if (const char* to = std::getenv("MM3E_TILE_ORDER")) r->tile_order = !(lower(to) == "off" || lower(to) == "0");
+  env_wavefront(&r->wf_budget, &r->wf_passes, &r->wf_bpc);
+  r->cus = (uint32_t)std::max(1, p.multiProcessorCount);
   r->name = describe(dev);
   // Tables staged in LDS share the block's LDS with the 3 KiB sample-reduction buffer; keep
   // the copy to 56 KiB so three 256-lane blocks still fit one CU.
@@ -521,6 +638,7 @@ void mm3e_scene_destroy(mm3e_scene* s) {
   if (s->d_tmp) hipFree(s->d_tmp);
   if (s->d_rgba) hipFree(s->d_rgba);
   if (s->d_to) hipFree(s->d_to);
+  if (s->d_wf) hipFree(s->d_wf);
   if (s->h_rgba) hipHostFree(s->h_rgba);
   for (int f = 0; f < mm3e_scene::kRing; f++)
     for (int i = 0; i < 3; i++)
@@ -634,7 +752,15 @@ int mm3e_scene_info(const mm3e_scene* s, uint32_t* n_objects, uint32_t* n_segmen
 
 int mm3e_scene_kernel_name(const mm3e_scene* s, char* buf, size_t buflen) {
   if (!s || !buf || buflen == 0) return MM3E_ERR_INVALID;
-  if (s->gen && s->gen->state.load(std::memory_order_acquire) == MM3E_CODEGEN_READY) {
+  const bool gen_ready = s->gen && s->gen->state.load(std::memory_order_acquire) == MM3E_CODEGEN_READY;
+  uint32_t wf_b, wf_n, wf_c;
+  wf_settings(s, &wf_b, &wf_n, &wf_c);
+  if (wf_b > 0) {  // queued frame: name its shading stage
+    if (gen_ready && s->gen->fn_wf[2]) std::snprintf(buf, buflen, "mm3e_gen_wf_shade");
+    else std::snprintf(buf, buflen, "%s%u", s->general ? "mm3e_wf_shade_x_b" : "mm3e_wf_shade_b", s->bounces);
+    return MM3E_OK;
+  }
+  if (gen_ready) {
     std::snprintf(buf, buflen, "%s", s->gen->fn_o && s->to_on ? "mm3e_gen_render_o" : "mm3e_gen_render");
     return MM3E_OK;
   }
@@ -959,6 +1085,82 @@ int mm3e_renderer_set_tile_order(mm3e_renderer* r, int on) {
   return MM3E_OK;
 }
 
+int mm3e_renderer_set_wavefront(mm3e_renderer* r, uint32_t budget, uint32_t passes, uint32_t blocks_per_cu) {
+  if (!r || passes + 1 > WF_MAX_PASSES || blocks_per_cu > 64) return MM3E_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(r->mu);
+  r->wf_budget = budget;
+  r->wf_passes = passes;
+  r->wf_bpc = blocks_per_cu;
+  return MM3E_OK;
+}
+
+int mm3e_scene_set_wavefront(mm3e_scene* s, int own, uint32_t budget, uint32_t passes, uint32_t blocks_per_cu) {
+  if (!s || passes + 1 > WF_MAX_PASSES || blocks_per_cu > 64) return MM3E_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->r->mu);
+  s->wf_own = own != 0;
+  s->wf_set[0] = budget;
+  s->wf_set[1] = passes;
+  s->wf_set[2] = blocks_per_cu;
+  return MM3E_OK;
+}
+
+// Times the scene's frame (render stage, kernel events) under each candidate: one kernel, then
+// queued frames over a grid of budgets / passes / grid sizes; keeps the fastest as the scene's
+// own setting.
+int mm3e_scene_autotune(mm3e_scene* s, const mm3e_camera* cam, uint32_t reps, char* report, size_t report_len) {
+  if (!s || validate_cam(cam)) return MM3E_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(s->r->mu);
+  hipSetDevice(s->r->device);
+  reps = std::max(1u, std::min(reps, (uint32_t)mm3e_scene::kRing - 1));
+  static const uint32_t cand[][3] = {{0, 0, 0},   {1000, 0, 0}, {64, 1, 0}, {32, 2, 0},
+                                     {16, 3, 0},  {32, 2, 8},   {64, 1, 4}};
+  const FrameParams fp = make_frame(*cam, s->w, s->h);
+  const bool own0 = s->wf_own;
+  uint32_t set0[3] = {s->wf_set[0], s->wf_set[1], s->wf_set[2]};
+  std::string msg, rep;
+  float best = 1e30f;
+  size_t best_i = 0;
+  for (size_t c = 0; c < sizeof cand / sizeof cand[0]; c++) {
+    s->wf_own = true;
+    for (int k = 0; k < 3; k++) s->wf_set[k] = cand[c][k];
+    bool ok = launch_frame(s, s->d_scene, fp, msg);  // warm-up (queued buffers, caches)
+    const uint64_t first = s->frames;
+    for (uint32_t i = 0; ok && i < reps; i++) ok = launch_frame(s, s->d_scene, fp, msg);
+    ok = ok && hip_ok(hipStreamSynchronize(s->r->stream), "autotune frames", &msg);
+    if (!ok) {
+      s->wf_own = own0;
+      for (int k = 0; k < 3; k++) s->wf_set[k] = set0[k];
+      set_err(report, report_len, "autotune failed: " + msg);
+      return MM3E_ERR_DEVICE;
+    }
+    float sum = 0.0f;
+    for (uint64_t f = first; f < s->frames; f++) {
+      float a = 0.0f;
+      hipEvent_t* ev = s->ev[f % mm3e_scene::kRing];
+      hipEventElapsedTime(&a, ev[0], ev[1]);
+      sum += a;
+    }
+    const float ms = sum / (float)reps;
+    char line[128];
+    if (cand[c][0] == 0) std::snprintf(line, sizeof line, "one kernel: %.3f ms\n", ms);
+    else
+      std::snprintf(line, sizeof line, "queued budget %u passes %u blocks/CU %u: %.3f ms\n", cand[c][0], cand[c][1],
+                    cand[c][2], ms);
+    rep += line;
+    if (ms < best) {
+      best = ms;
+      best_i = c;
+    }
+  }
+  s->wf_own = true;
+  for (int k = 0; k < 3; k++) s->wf_set[k] = cand[best_i][k];
+  char line[96];
+  std::snprintf(line, sizeof line, "chosen: %s (%.3f ms)\n", cand[best_i][0] ? "queued" : "one kernel", best);
+  rep += line;
+  set_err(report, report_len, rep);
+  return MM3E_OK;
+}
+
 int mm3e_renderer_set_codegen(mm3e_renderer* r, int mode) {
   if (!r || mode < MM3E_CODEGEN_OFF || mode > MM3E_CODEGEN_SYNC) return MM3E_ERR_INVALID;
   r->codegen = mode;
@@ -990,7 +1192,9 @@ int mm3e_scene_emit_source(const void* ir, size_t ir_len, char* src, size_t* src
   if (!parse_scene_ir(ir, ir_len, hs, msg)) return MM3E_ERR_INVALID;
   Blob b;
   build_blob(hs, {}, {}, hs.hdr.aa == 0 ? 1u : hs.hdr.aa, hs.hdr.bounces, 0, nullptr, false, b);
-  const std::string text = emit_scene_source(b, hs.hdr.bounces, generated_prefers_sm(b));
+  uint32_t wb = 0, wn = 2, wc = 0;
+  env_wavefront(&wb, &wn, &wc);
+  const std::string text = emit_scene_source(b, hs.hdr.bounces, generated_prefers_sm(b), wb > 0);
   const size_t need = text.size() + 1;
   if (!src || *src_len < need) {
     *src_len = need;

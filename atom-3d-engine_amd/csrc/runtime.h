@@ -23,6 +23,9 @@ struct GenJob {
   std::atomic<int> state{MM3E_CODEGEN_QUEUED};
   hipFunction_t fn = nullptr;
   hipFunction_t fn_o = nullptr;  // ordered-tiles variant
+  // stages of the queued frame (first march pass, continuation pass, shading); null unless
+  // the source was generated with them
+  hipFunction_t fn_wf[3] = {nullptr, nullptr, nullptr};
   std::string err;
   std::mutex mu;
   std::condition_variable cv;
@@ -38,6 +41,10 @@ struct mm3e_renderer {
   // scene code generation: mode, queue and worker threads (started on first use)
   int codegen = MM3E_CODEGEN_BACKGROUND;
   bool tile_order = true;        // generated kernels dispatch tiles most expensive first
+  // queued frames (dev_wavefront.h): evaluations per march pass (0: one kernel per frame),
+  // continuation passes, blocks per CU of the grid-stride stages (0: one lane per entry)
+  uint32_t wf_budget = 0, wf_passes = 2, wf_bpc = 0;
+  uint32_t cus = 0;              // compute units of the device
   std::mutex gen_mu;
   std::condition_variable gen_cv;
   std::deque<std::shared_ptr<mm3e::GenJob>> gen_queue;
@@ -81,6 +88,13 @@ struct mm3e_scene {
   uint32_t to_tiles = 0;
   bool to_on = false;
   uint32_t gen_stage = 0;              // LDS bytes the generated kernel stages (0: none needed)
+  // queued-frame buffers (allocated on the first queued frame, resized with the sample count)
+  uint8_t* d_wf = nullptr;
+  uint32_t wf_cap = 0;
+  // the scene's own queued-frame setting (budget, passes, blocks per CU; from the autotuner or
+  // mm3e_scene_set_wavefront) when wf_own, else the renderer's
+  bool wf_own = false;
+  uint32_t wf_set[3] = {0, 0, 0};
 };
 
 namespace mm3e {

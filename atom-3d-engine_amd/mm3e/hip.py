@@ -27,7 +27,7 @@ EXPORTS = (
     "mm3e_batch_destroy", "mm3e_render_batch", "mm3e_server_default_config", "mm3e_server_create",
     "mm3e_server_add_scene", "mm3e_server_submit", "mm3e_server_poll", "mm3e_server_wait", "mm3e_server_stats",
     "mm3e_server_destroy", "mm3e_scene_kernel_name",
-    "mm3e_scene_info", "mm3e_renderer_set_codegen", "mm3e_renderer_set_tile_order", "mm3e_selftest_sqrt", "mm3e_scene_codegen_status", "mm3e_scene_emit_source",
+    "mm3e_scene_info", "mm3e_renderer_set_codegen", "mm3e_renderer_set_tile_order", "mm3e_renderer_set_wavefront", "mm3e_scene_set_wavefront", "mm3e_scene_autotune", "mm3e_selftest_sqrt", "mm3e_scene_codegen_status", "mm3e_scene_emit_source",
     "mm3e_scene_parse_text", "mm3e_scene_to_text",
     "mm3e_mesh_parse_obj", "mm3e_mesh_sdf_layout", "mm3e_mesh_bake_sdf", "mm3e_mesh_baker_create",
     "mm3e_mesh_baker_layout", "mm3e_mesh_baker_info", "mm3e_mesh_baker_run", "mm3e_mesh_baker_wait",
@@ -74,6 +74,9 @@ def lib():
         L.mm3e_scene_info.argtypes = [P, P, P, P, P, P]
         L.mm3e_renderer_set_codegen.argtypes = [P, C.c_int]
         L.mm3e_renderer_set_tile_order.argtypes = [P, C.c_int]
+        L.mm3e_renderer_set_wavefront.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.mm3e_scene_set_wavefront.argtypes = [P, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.mm3e_scene_autotune.argtypes = [P, P, C.c_uint32, C.c_char_p, C.c_size_t]
         L.mm3e_selftest_sqrt.argtypes = [P, C.POINTER(C.c_uint64)]
         L.mm3e_scene_codegen_status.argtypes = [P, C.c_int, P, C.c_char_p, C.c_size_t]
         L.mm3e_scene_emit_source.argtypes = [P, C.c_size_t, C.c_char_p, C.POINTER(C.c_size_t)]
@@ -442,6 +445,14 @@ class Renderer:
         if rc != 0:
             raise Mm3eError(rc, "set_tile_order failed")
 
+    def set_wavefront(self, budget: int, passes: int = 2, blocks_per_cu: int = 0):
+        """Queued frames: march passes of at most `budget` evaluations over compacted queues of
+        unfinished rays, then shading of the compacted hit queue (budget 0: one kernel per frame).
+        Generated kernels of scenes compiled while this is on carry the queued stages."""
+        rc = lib().mm3e_renderer_set_wavefront(self.h, budget, passes, blocks_per_cu)
+        if rc != 0:
+            raise Mm3eError(rc, "set_wavefront: passes 0..7, blocks_per_cu 0..64")
+
     def selftest_sqrt(self) -> int:
         """Inputs (of all 2^32) where the kernels' square root differs from IEEE sqrtf."""
         n = C.c_uint64(0)
@@ -515,6 +526,21 @@ class GpuScene:
         if rc != 0:
             raise Mm3eError(rc, "render_rgba failed")
         return out
+
+    def set_wavefront(self, budget=None, passes: int = 2, blocks_per_cu: int = 0):
+        """The scene's own queued-frame setting; budget None returns it to the renderer's."""
+        own = budget is not None
+        rc = lib().mm3e_scene_set_wavefront(self.h, 1 if own else 0, budget or 0, passes, blocks_per_cu)
+        if rc != 0:
+            raise Mm3eError(rc, "set_wavefront: passes 0..7, blocks_per_cu 0..64")
+
+    def autotune(self, cam, reps: int = 5) -> str:
+        """Times one-kernel and queued frames of `cam`; keeps the fastest; returns the report."""
+        rep = C.create_string_buffer(2048)
+        rc = lib().mm3e_scene_autotune(self.h, _p(_cam(cam)), reps, rep, 2048)
+        if rc != 0:
+            raise Mm3eError(rc, "scene autotune failed: " + rep.value.decode(errors="replace"))
+        return rep.value.decode()
 
     def render_only(self, cam):
         rc = lib().mm3e_scene_render_only(self.h, _p(_cam(cam)))

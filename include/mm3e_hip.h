@@ -172,6 +172,25 @@ int mm3e_renderer_set_codegen(mm3e_renderer* r, int mode);
  * in a few long tiles does not end with them running alone.  The image is the same bit for
  * bit either way. */
 int mm3e_renderer_set_tile_order(mm3e_renderer* r, int on);
+/* Queued frames (no reference counterpart; the reference's frame is one kernel, wgsl.rs:562-581,
+ * which this replaces for the renderer's subsequent frames): budget > 0 splits each frame into
+ * a first march pass of at most `budget` field evaluations per ray, `passes` continuation
+ * passes over the compacted queue of unfinished rays (the last runs them to their end; 0..7),
+ * shading of the compacted hit queue and a per-pixel resolve.  blocks_per_cu sizes the
+ * grid-stride stages (0: one lane per queue entry; at most 64).  budget 0 restores one kernel
+ * per frame.  Generated scene kernels carry the queued stages only when the scene is compiled
+ * while queued frames are on (otherwise the table kernels run them).  Every image is the same
+ * bit for bit either way.  The environment variable MM3E_WAVEFRONT="budget,passes,bpc" sets
+ * the renderer's default. */
+int mm3e_renderer_set_wavefront(mm3e_renderer* r, uint32_t budget, uint32_t passes, uint32_t blocks_per_cu);
+/* A scene's own queued-frame setting (own != 0; same arguments as above), or back to the
+ * renderer's (own == 0).  No reference counterpart. */
+int mm3e_scene_set_wavefront(mm3e_scene* s, int own, uint32_t budget, uint32_t passes, uint32_t blocks_per_cu);
+/* Autotuner over the frame's form (no reference counterpart): renders reps frames of `cam`
+ * (1..127) with one kernel and with queued frames at several step budgets, pass counts and
+ * grid sizes, keeps the fastest render stage as the scene's own setting and writes one line
+ * per candidate into report.  Images are the same under every candidate. */
+int mm3e_scene_autotune(mm3e_scene* s, const mm3e_camera* cam, uint32_t reps, char* report, size_t report_len);
 /* Device self-test (no reference counterpart): the render kernels' square root (a shortened
  * correctly rounded sequence) against the compiler's IEEE sqrtf over all 2^32 f32 inputs;
  * *mismatches counts the inputs whose results differ (0 expected). */
@@ -180,7 +199,8 @@ int mm3e_selftest_sqrt(mm3e_renderer* r, uint64_t* mismatches);
  * blocks until it is READY or FAILED.  On FAILED the compiler's message goes to err. */
 int mm3e_scene_codegen_status(mm3e_scene* s, int wait, int* state, char* err, size_t errlen);
 /* wgsl::build_shader_checked twin (wgsl.rs:34-102): the generated HIP source of a scene IR (as
- * mm3e_scene_compile would build it).  With src == NULL or *src_len too small, *src_len is set
+ * mm3e_scene_compile would build it; with the queued-frame stages when MM3E_WAVEFRONT sets a
+ * budget).  With src == NULL or *src_len too small, *src_len is set
  * to the size needed (including the terminating NUL) and MM3E_ERR_INVALID is returned. */
 int mm3e_scene_emit_source(const void* scene_ir, size_t ir_len, char* src, size_t* src_len);
 
