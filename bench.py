@@ -16,7 +16,7 @@ HIP-event duration over the timed frames, against MI355X's vector FP32 peak.
 cpu_baseline: the CPU oracle (a C++ restatement of the reference's multithreaded CPU
 renderer) timed on this host on one full frame of the same workload.
 
-Two more legs ride on the same line (``--legs`` picks them):
+More legs ride on the same line (``--legs`` picks them):
   batched: 64 cameras over 8 seeded procedural scenes (20-200 primitives, resolutions from
            {320x180, 640x360, 1280x720, 1920x1080}) rendered once as 64 plain per-frame launches
            (each followed by a wait, as render() does) and once as one autotuned persistent
@@ -33,6 +33,9 @@ Two more legs ride on the same line (``--legs`` picks them):
   frame_gen: c2 at 960x540 as a G-buffer keyframe plus reprojected frames for the camera swinging
            0.01 rad per frame about its target (reproject_hybrid: warp + holes rendered for real); ms per
            G-buffer frame vs per generated frame (both including the RGBA8 readback), holes.
+  queued:  c2 compiled with the queued-frame stages (budgeted march passes over compacted ray
+           queues, hit-queue shading); the scene autotuner's render-stage ms for the one-kernel
+           frame and each queued candidate, its choice, and whether the images are equal.
   stress:  (not in the default set) one 3840x2160 frame of the 1000-primitive scene with a 128^3
            sampled volume, 32^3 device-baked GI, 2 bounces and fog; ms/frame.
 """
@@ -366,14 +369,42 @@ def frame_gen_leg(hip, configs, r, frames=8):
     return out
 
 
+def queued_leg(hip, configs, g_plain, reps=5):
+    """c2 as queued frames (budgeted march passes over compacted ray queues, hit-queue shading):
+    the scene autotuner's timing of the one-kernel frame and each queued candidate, its choice,
+    and whether the queued image equals the plain one."""
+    sc, cam = configs.c2_scene()
+    r = hip.Renderer()
+    r.set_codegen(hip.CODEGEN_SYNC)
+    r.set_wavefront(32, 2, 0)  # generated kernels carry the queued stages
+    t0 = time.perf_counter()
+    g = r.compile(sc.to_ir(), sc.width, sc.height)
+    state, _ = g.codegen_status(wait=True)
+    codegen_s = time.perf_counter() - t0
+    same = bool((g.render_rgba(cam) == g_plain.render_rgba(cam)).all())
+    rep = g.autotune(cam, reps=reps)
+    log("queued frames (c2):\n" + rep.rstrip())
+    cands = {}
+    for line in rep.strip().splitlines():
+        if line.startswith("chosen:"):
+            continue
+        k, v = line.rsplit(":", 1)
+        cands[k.strip()] = float(v.strip().split()[0])
+    chosen = rep.strip().splitlines()[-1]
+    g.close()
+    r.close()
+    return {"codegen_state": state, "compile_and_codegen_s": round(codegen_s, 2), "image_equals_plain": same,
+            "render_ms": cands, "chosen": chosen}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="plain,batched,served,mesh_bake,frame_gen",
-                    help="comma list of plain (always run), batched, served, mesh_bake, frame_gen, stress")
+    ap.add_argument("--legs", default="plain,batched,served,mesh_bake,frame_gen,queued",
+                    help="comma list of plain (always run), batched, served, mesh_bake, frame_gen, queued, stress")
     ap.add_argument("--bake-reps", type=int, default=20)
     ap.add_argument("--stress-frames", type=int, default=3)
     ap.add_argument("--batched-reps", type=int, default=3)
@@ -476,6 +507,8 @@ def main():
         out["mesh_bake"] = mesh_bake_leg(hip, configs, r, args.bake_reps, cpu=not args.no_cpu)
     if "frame_gen" in legs:
         out["frame_gen"] = frame_gen_leg(hip, configs, r)
+    if "queued" in legs:
+        out["queued"] = queued_leg(hip, configs, g)
     if "stress" in legs:
         out["stress"] = stress_leg(hip, configs, r, args.stress_frames)
     print(json.dumps(out), flush=True)
