@@ -184,8 +184,8 @@ MM3E_DEV void wf_march_cont(const DevScene* __restrict__ scp, const Tables& tb, 
   const uint32_t lane = __lane_id();
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
-  for (uint32_t base = wave * 64u; base < n; base += n_waves * 64u) {
-    const uint32_t i = base + lane;
+  for (uint64_t base = (uint64_t)wave * 64u; base < n; base += (uint64_t)n_waves * 64u) {
+    const uint32_t i = (uint32_t)base + lane;
     const bool live = i < n;
     uint32_t g = 0, mst = MARCH_DONE;
     MarchState m;
@@ -223,8 +223,8 @@ MM3E_DEV void wf_shade(const DevScene* __restrict__ scp, const Tables& tb, const
   const uint32_t lane = __lane_id();
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
-  for (uint32_t base = wave * 64u; base < n; base += n_waves * 64u) {
-    const uint32_t i = base + lane;
+  for (uint64_t base = (uint64_t)wave * 64u; base < n; base += (uint64_t)n_waves * 64u) {
+    const uint32_t i = (uint32_t)base + lane;
     if (i < n) {
       const uint32_t g = w.hslot[i];
       Hit h;

@@ -297,7 +297,13 @@ static bool launch_queued(mm3e_scene* s, const DevScene* dsc, const FrameParams&
                           uint32_t stage, bool general, std::string& msg) {
   mm3e_renderer* r = s->r;
   hipStream_t st = r->stream;
-  const uint32_t cap = grid.x * grid.y * block.x;
+  // sample slots are indexed in 32 bits (the queues hold at most every slot once)
+  const uint64_t slots = (uint64_t)grid.x * grid.y * block.x;
+  if (slots > 0xFFFFFF00ull) {
+    msg = "queued frame: " + std::to_string(slots) + " sample slots exceed the 32-bit queue index";
+    return false;
+  }
+  const uint32_t cap = (uint32_t)slots;
   if (!wf_ready(s, cap, msg)) return false;
   WfFrame w = wf_frame(s, grid.x);
   uint32_t budget, passes, bpc;
