@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 namespace mm3e {
@@ -443,6 +444,37 @@ Sphere world_bound(const HostScene& sc, const mm3e_ir_object& o) {  // lib.rs:47
   return Sphere{true, {wc.x, wc.y, wc.z}, radius * o.scale + std::fmax(o.round, 0.0f) + std::fabs(o.onion)};
 }
 
+// Local bound of a primitive whose SDF is exact: SDF(q) >= |q - c| - r with |c| returned in
+// *c.  False for the primitives whose SDF is only a bound (or not exact for these parameters).
+static bool exact_prim_bound(const mm3e_ir_object& o, double* c_out, double* r_out) {
+  double c = 0.0, r = 0.0;
+  const float* p = o.p;
+  auto n3 = [](double x, double y, double z) { return std::sqrt(x * x + y * y + z * z); };
+  switch (o.prim) {
+    case MM3E_PRIM_SPHERE: r = p[0]; break;
+    case MM3E_PRIM_BOX: r = n3(p[0], p[1], p[2]); break;
+    case MM3E_PRIM_ROUND_BOX:  // sdf.rs:44-46: exact while the radius fits every half extent
+      if (!(p[3] >= 0.0f && p[3] <= std::min(p[0], std::min(p[1], p[2])))) return false;
+      r = n3(p[0], p[1], p[2]);
+      break;
+    case MM3E_PRIM_TORUS: r = (double)p[0] + p[1]; break;
+    case MM3E_PRIM_CYLINDER: r = n3(p[0], p[1], 0.0); break;
+    case MM3E_PRIM_CAPSULE:
+      c = n3(0.5 * ((double)p[0] + p[3]), 0.5 * ((double)p[1] + p[4]), 0.5 * ((double)p[2] + p[5]));
+      r = 0.5 * n3((double)p[0] - p[3], (double)p[1] - p[4], (double)p[2] - p[5]) + p[6];
+      break;
+    case MM3E_PRIM_CONE:
+      if (!(std::fabs(p[0] - p[1]) < p[2])) return false;
+      c = 0.5 * std::fabs((double)p[2]);
+      r = 0.5 * std::fabs((double)p[2]) + std::max((double)p[0], (double)p[1]);
+      break;
+    default: return false;
+  }
+  *c_out = c;
+  *r_out = r;
+  return true;
+}
+
 // A sphere below the field of an object the reference leaves unbounded because of a bend
 // (lib.rs:478-480).  Mirror, twist and bend map the local point by reflections and rotations,
 // which keep |l|; elongation moves it by at most |h|; so for a primitive whose SDF is exact
@@ -455,33 +487,50 @@ Sphere skip_bound(const mm3e_ir_object& o) {
   const Sphere none{false, {0, 0, 0}, 0};
   if (o.bend == 0.0f || nonzero3(o.repeat) || !(o.scale > 0.0f) || !std::isfinite(o.scale)) return none;
   double c = 0.0, r = 0.0;
-  const float* p = o.p;
-  auto n3 = [](double x, double y, double z) { return std::sqrt(x * x + y * y + z * z); };
-  switch (o.prim) {
-    case MM3E_PRIM_SPHERE: r = p[0]; break;
-    case MM3E_PRIM_BOX: r = n3(p[0], p[1], p[2]); break;
-    case MM3E_PRIM_ROUND_BOX:  // sdf.rs:44-46: exact while the radius fits every half extent
-      if (!(p[3] >= 0.0f && p[3] <= std::min(p[0], std::min(p[1], p[2])))) return none;
-      r = n3(p[0], p[1], p[2]);
-      break;
-    case MM3E_PRIM_TORUS: r = (double)p[0] + p[1]; break;
-    case MM3E_PRIM_CYLINDER: r = n3(p[0], p[1], 0.0); break;
-    case MM3E_PRIM_CAPSULE:
-      c = n3(0.5 * ((double)p[0] + p[3]), 0.5 * ((double)p[1] + p[4]), 0.5 * ((double)p[2] + p[5]));
-      r = 0.5 * n3((double)p[0] - p[3], (double)p[1] - p[4], (double)p[2] - p[5]) + p[6];
-      break;
-    case MM3E_PRIM_CONE:
-      if (!(std::fabs(p[0] - p[1]) < p[2])) return none;
-      c = 0.5 * std::fabs((double)p[2]);
-      r = 0.5 * std::fabs((double)p[2]) + std::max((double)p[0], (double)p[1]);
-      break;
-    default: return none;
-  }
-  const double h = n3(o.elongate[0], o.elongate[1], o.elongate[2]);
+  if (!exact_prim_bound(o, &c, &r)) return none;
+  const double h = std::sqrt((double)o.elongate[0] * o.elongate[0] + (double)o.elongate[1] * o.elongate[1] +
+                             (double)o.elongate[2] * o.elongate[2]);
   double rad = (h + c + r) * (double)o.scale + std::max((double)o.round, 0.0) + std::fabs((double)o.onion);
   rad = rad * (1.0 + 1e-6) + 1e-6;
   if (!std::isfinite(rad) || !(rad > 0.0)) return none;
   return Sphere{true, {o.pos[0], o.pos[1], o.pos[2]}, (float)rad};
+}
+
+// A slab below the field of a repeated object (unbounded in the reference, lib.rs:478-480)
+// along a local axis a that is not repeated (rep[a] <= 0) and that the warps keep: twist
+// rotates x and z about y (keeps y), bend rotates x and y (keeps z).  Mirror keeps |l_a|,
+// elongation moves l_a by at most |h_a|, repetition leaves it, so the final local point q has
+// |q_a| >= |l_a| - |h_a| and, for an exact-SDF primitive, SDF(q) >= |q| - |c| - r >= |q_a| - |c| - r.
+// With l_a = dot(column_a, p - pos) / scale the field is at least
+// |dot(n, p) - d| - H,  n = column_a,  d = dot(n, pos),  H = (|h_a| + |c| + r) * scale + max(round, 0) + |onion|.
+// Returned as n (3), d and H (rounded up); `some` false when no axis qualifies.
+struct Slab {
+  bool some;
+  float n[3], d, h;
+};
+static Slab slab_bound(const mm3e_ir_object& o) {
+  Slab none{false, {0, 0, 0}, 0, 0};
+  if (!nonzero3(o.repeat) || !(o.scale > 0.0f) || !std::isfinite(o.scale)) return none;
+  double c = 0.0, r = 0.0;
+  if (!exact_prim_bound(o, &c, &r)) return none;
+  const bool twist = o.twist != 0.0f, bend = o.bend != 0.0f;
+  int axis = -1;
+  for (int a : {1, 2, 0}) {
+    if (o.repeat[a] > 0.0f) continue;
+    if ((a == 1 && bend) || (a == 0 && (twist || bend)) || (a == 2 && twist)) continue;
+    axis = a;
+    break;
+  }
+  if (axis < 0) return none;
+  const float* n = o.rot + 3 * axis;
+  double hh = (std::fabs((double)o.elongate[axis]) + c + r) * (double)o.scale + std::max((double)o.round, 0.0) +
+              std::fabs((double)o.onion);
+  hh = hh * (1.0 + 1e-6) + 1e-6;
+  const double d = (double)n[0] * o.pos[0] + (double)n[1] * o.pos[1] + (double)n[2] * o.pos[2];
+  if (!std::isfinite(hh) || !std::isfinite(d) || !(hh > 0.0)) return none;
+  for (int k = 0; k < 3; k++)
+    if (!std::isfinite(n[k])) return none;
+  return Slab{true, {n[0], n[1], n[2]}, (float)d, (float)hh};
 }
 
 bool gi_bounds(const HostScene& sc, float lo[3], float hi[3]) {
@@ -702,6 +751,10 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
   std::vector<Sphere> bounds(n_obj);       // Object::world_bound (the reference's plan and prune)
   std::vector<Sphere> grid_bounds(n_obj);  // the same, or a skip sphere (candidate grid only)
   float max_k = 0.0f;
+  // slab skip bounds of repeated objects (MM3E_SKIP_SLABS=off leaves them out; the field is the
+  // same either way)
+  const char* slab_env = std::getenv("MM3E_SKIP_SLABS");
+  const bool slabs = !(slab_env && (std::strcmp(slab_env, "off") == 0 || std::strcmp(slab_env, "0") == 0));
   for (uint64_t i = 0; i < n_obj; i++) {
     const mm3e_ir_object& o = objs[i];
     DObj& d = dobj[i];
@@ -749,11 +802,17 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
       d.bc_r = bounds[i].r;
     } else {
       Sphere k = skip_bound(o);
+      Slab sl = k.some || !slabs ? Slab{false, {0, 0, 0}, 0, 0} : slab_bound(o);
       if (k.some) {
         f |= OBJ_SKIP;
         std::memcpy(d.bc, k.c, 12);
         d.bc_r = k.r;
         grid_bounds[i] = k;
+      } else if (sl.some) {  // far tests only; the candidate grid keeps the object everywhere
+        f |= OBJ_SKIP | OBJ_SLAB;
+        std::memcpy(d.bc, sl.n, 12);
+        d.bd = sl.d;
+        d.bc_r = sl.h;
       }
     }
     d.flags = f;

@@ -123,13 +123,10 @@ def test_field_parity_bent_objects(renderer):
     assert bad.mean() <= 1e-3 and (mg == mo).mean() >= 0.999
 
 
-def test_field_parity_repeated_objects(renderer):
-    """Repeated objects (repetition folds space; the reference leaves them unbounded and
-    evaluates them exactly everywhere) in unions, smooth unions and tree-walked runs, with
-    twisted / mirrored / elongated members: the device fold must give the oracle's field."""
+def _repeated_scene(width=64, height=36):
     from mm3e.scene import Material, Object, Prim, Scene, Transform, from_euler, v3
     rng = configs.SplitMix64(78)
-    sc = Scene(64, 36)
+    sc = Scene(width, height)
     m = sc.material(Material.solid((0.6, 0.6, 0.6)))
     sc.add(Object(Prim.plane((0.0, 1.0, 0.0), 0.0), Transform(), m))
     for i in range(40):
@@ -150,7 +147,24 @@ def test_field_parity_repeated_objects(renderer):
             o.mirror(False, True, False)
         if i % 6 == 4:
             o.elongate((0.0, 0.15, 0.1))
+        if i % 8 == 5:
+            o.repeat((rng.uniform(2.0, 4.0), rng.uniform(2.0, 4.0), 0.0)).bend(rng.uniform(0.2, 0.6))
         sc.add(o)
+    return sc
+
+
+def _repeated_points():
+    rng2 = np.random.default_rng(78)
+    centres = rng2.uniform([-7, -0.3, -7], [7, 3.0, 7], (300, 3))
+    return np.concatenate([(centres[:, None, :] + rng2.uniform(-0.15, 0.15, (300, 64, 3))).reshape(-1, 3),
+                           rng2.uniform([-12, -3, -12], [12, 7, 12], (8000, 3))]).astype(np.float32)
+
+
+def test_field_parity_repeated_objects(renderer):
+    """Repeated objects (repetition folds space; the reference leaves them unbounded and
+    evaluates them exactly everywhere) in unions, smooth unions and tree-walked runs, with
+    twisted / mirrored / elongated / bent members: the device fold must give the oracle's field."""
+    sc = _repeated_scene()
     g, o = both(renderer, sc)
     rng2 = np.random.default_rng(78)
     centres = rng2.uniform([-7, -0.3, -7], [7, 3.0, 7], (300, 3))
@@ -163,6 +177,30 @@ def test_field_parity_repeated_objects(renderer):
     report("field repeated", {**g.info(), "max_abs_err": float(np.abs(dg - do).max()), "bad": int(bad.sum()),
                               "mat_agree": float((mg == mo).mean())})
     assert bad.mean() <= 1e-3 and (mg == mo).mean() >= 0.999
+
+
+def test_slab_skip_bounds_change_no_field_value(renderer, monkeypatch):
+    """Repeated objects carry a slab skip bound on the device (along a local axis that is not
+    repeated and that twist / bend keep); it only proves that folding the object changes
+    nothing.  With the slabs switched off (MM3E_SKIP_SLABS=off at compile) the device must give
+    the same field bit for bit, and the same image."""
+    sc = _repeated_scene(160, 90)
+    cam = configs.orbit_camera((0.0, 1.0, 0.0), 9.0, 0.7, 0.35, np.radians(55.0))
+    pts = _repeated_points()
+    g = renderer.compile(sc.to_ir(), sc.width, sc.height)
+    monkeypatch.setenv("MM3E_SKIP_SLABS", "off")
+    g0 = renderer.compile(sc.to_ir(), sc.width, sc.height)
+    monkeypatch.delenv("MM3E_SKIP_SLABS")
+    d1, m1 = g.field(pts)
+    d0, m0 = g0.field(pts)
+    i1, i0 = g.render_rgba(cam), g0.render_rgba(cam)
+    same = d1.view(np.uint32) == d0.view(np.uint32)
+    report("slab skip", {"field_equal": float(same.mean()), "mat_equal": float((m1 == m0).mean()),
+                         "image_equal": bool((i1 == i0).all())})
+    assert same.all() and (m1 == m0).all()
+    assert (i1 == i0).all()
+    g.close()
+    g0.close()
 
 
 def test_march_parity(renderer):
