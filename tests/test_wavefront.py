@@ -145,3 +145,27 @@ def test_scene_autotune_keeps_the_image():
     assert not g.kernel_name().startswith("mm3e_wf_shade")
     g.close()
     r.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("w,h,aa", [(1, 1, 2), (17, 9, 3), (33, 7, 1), (250, 3, 4)])
+def test_queued_frames_odd_sizes(w, h, aa):
+    """Images whose sizes leave partial tiles (and a single pixel), at several AA factors: the
+    queued frame's slots outside the image stay unused and every pixel equals the one-kernel
+    frame."""
+    sc, cam = configs.c1_scene(width=w, height=h)
+    sc.aa = aa
+    base = hip.Renderer()
+    base.set_codegen(hip.CODEGEN_OFF)
+    q = hip.Renderer()
+    q.set_codegen(hip.CODEGEN_OFF)
+    q.set_wavefront(3, 4, 1)
+    gb = base.compile(sc.to_ir(), w, h)
+    gq = q.compile(sc.to_ir(), w, h)
+    a, b = gb.render_linear(cam), gq.render_linear(cam)
+    print(f"\n[queued odd {w}x{h} aa{aa}] equal={bool((a.view(np.uint32) == b.view(np.uint32)).all())}", flush=True)
+    assert (a.view(np.uint32) == b.view(np.uint32)).all()
+    gb.close()
+    gq.close()
+    base.close()
+    q.close()
