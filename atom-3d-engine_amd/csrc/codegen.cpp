@@ -68,7 +68,7 @@ bool generated_needs_tables(const Blob& b) {
 
 bool generated_prefers_sm(const Blob& b) { return b.objs.size() > 24 || b.header.grid != nullptr; }
 
-std::string emit_scene_source(const Blob& b, uint32_t maxb, bool sm, bool queued) {
+std::string emit_scene_source(const Blob& b, uint32_t maxb, bool sm, bool queued, bool film) {
   const char* csg = has_csg(b) ? "true" : "false";
   std::string o;
   o.reserve(4096 + b.objs.size() * 900);
@@ -160,6 +160,21 @@ std::string emit_scene_source(const Blob& b, uint32_t maxb, bool sm, bool queued
        "  const Tables tb = stage_bytes ? stage_tables(sc, tables_lds, stage_bytes) : global_tables(sc);\n";
   o += "  render_ordered<GenMap, " + std::to_string(maxb) + ", " + (sm ? "true" : "false") +
        ">(sc, tb, fp, hdr, tw, th, o, smem, &start);\n}\n";
+  if (film) {  // render_film's sample pass and the G-buffer / hole pass (tables read from HBM)
+    const std::string tmpl = std::to_string(maxb) + ", " + (sm ? "true" : "false");
+    o += "extern \"C\" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY\n"
+         "    mm3e_gen_film(const DevScene* __restrict__ sc, FrameParams fp, FilmParams lp, float* __restrict__ beauty,\n"
+         "                  float* __restrict__ fg, float* __restrict__ alpha, uint32_t tw, uint32_t th) {\n"
+         "  __shared__ float smem[256 * 4];\n"
+         "  film_tile<GenMap, " + tmpl + ">(sc, global_tables(sc), fp, lp, beauty, fg, alpha, tw, th, blockIdx.x,\n"
+         "                                  blockIdx.y, smem);\n}\n";
+    o += "extern \"C\" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY\n"
+         "    mm3e_gen_gbuffer(const DevScene* __restrict__ sc, FrameParams fp, float exposure,\n"
+         "                     const GMover* __restrict__ movers, uint32_t n_movers, const uint32_t* __restrict__ list,\n"
+         "                     const uint32_t* __restrict__ n_list, uchar4* __restrict__ color, float* __restrict__ depth,\n"
+         "                     int32_t* __restrict__ obj) {\n"
+         "  gbuffer_tile<GenMap, " + tmpl + ">(sc, fp, exposure, movers, n_movers, list, n_list, color, depth, obj);\n}\n";
+  }
   if (queued) {
     const std::string tbl = "  extern __shared__ uint4 tables_lds[];\n"
                             "  const Tables tb = stage_bytes ? stage_tables(sc, tables_lds, stage_bytes) : global_tables(sc);\n";

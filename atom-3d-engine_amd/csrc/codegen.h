@@ -12,8 +12,9 @@ namespace mm3e {
 
 // Source of the scene's `GenMap` (a map functor like TableMap) plus its render kernel
 // `mm3e_gen_render` for reflection depth maxb; sm selects the state-machine path.  queued adds
-// the stages of the queued frame (mm3e_gen_wf_first / _cont / _shade, dev_wavefront.h).
-std::string emit_scene_source(const Blob& b, uint32_t maxb, bool sm, bool queued = false);
+// the stages of the queued frame (mm3e_gen_wf_first / _cont / _shade, dev_wavefront.h); film
+// adds the film and G-buffer kernels (mm3e_gen_film, mm3e_gen_gbuffer).
+std::string emit_scene_source(const Blob& b, uint32_t maxb, bool sm, bool queued = false, bool film = false);
 
 // Whether the generated kernel reads the object tables (union runs walked through their
 // bounding-sphere tree still use the table walk).

@@ -165,9 +165,8 @@ int mm3e_scene_render_gbuffer(mm3e_scene* s, const mm3e_camera* cam, const mm3e_
                 "upload movers", &msg);
   const FrameParams fp = make_frame(*cam, s->w, s->h);
   if (ok)
-    ok = hip_ok(launch_gbuffer(s->bounces, st, s->d_scene, s->general, fp, s->pp.exposure,
-                               reinterpret_cast<const GMover*>(g->d_param), n_movers, nullptr, nullptr, 0,
-                               g->d_color, g->d_depth, g->d_obj),
+    ok = hip_ok(scene_gbuffer(s, st, fp, reinterpret_cast<const GMover*>(g->d_param), n_movers, nullptr, nullptr, 0,
+                              g->d_color, g->d_depth, g->d_obj),
                 "gbuffer launch", &msg);
   // the mover table must not change under the kernel: wait before returning
   if (ok) ok = hip_ok(hipStreamSynchronize(st), "gbuffer", &msg);
@@ -219,8 +218,7 @@ int mm3e_scene_reproject_hybrid(mm3e_scene* s, mm3e_gframe* g, const mm3e_camera
   // the holes rendered for real at the new camera, over the frame's size (lib.rs:1059-1079)
   const FrameParams fp = make_frame(*cam, g->w, g->h);
   if (ok)
-    ok = hip_ok(launch_gbuffer(s->bounces, st, s->d_scene, s->general, fp, s->pp.exposure, nullptr, 0, g->d_list,
-                               g->d_count, g->w * g->h, g->d_out, nullptr, nullptr),
+    ok = hip_ok(scene_gbuffer(s, st, fp, nullptr, 0, g->d_list, g->d_count, g->w * g->h, g->d_out, nullptr, nullptr),
                 "hole render", &msg);
   if (ok) ok = hip_ok(hipMemcpyAsync(&holes, g->d_count, 4, hipMemcpyDeviceToHost, st), "hole count", &msg);
   if (ok) ok = read_out(g, out, nullptr, msg);

@@ -21,7 +21,7 @@ struct Entry {
   hipModule_t mod = nullptr;
   hipFunction_t fn = nullptr;
   hipFunction_t fn_o = nullptr;  // the ordered-tiles variant
-  hipFunction_t fn_wf[3] = {nullptr, nullptr, nullptr};
+  hipFunction_t fn_opt[GEN_N_OPT] = {};
   std::string err;
 };
 
@@ -71,7 +71,7 @@ bool compile(const std::string& src, std::vector<char>& code, std::string& err) 
 }  // namespace
 
 bool rtc_kernel(const std::string& src, int device, hipFunction_t* fn, hipFunction_t* fn_o, std::string& err,
-                hipFunction_t* fn_wf) {
+                hipFunction_t* fn_opt) {
   std::shared_ptr<Entry> e;
   {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -92,9 +92,12 @@ bool rtc_kernel(const std::string& src, int device, hipFunction_t* fn, hipFuncti
       e->err = "generated module has no mm3e_gen_render";
     }
     if (e->ok && hipModuleGetFunction(&e->fn_o, e->mod, "mm3e_gen_render_o") != hipSuccess) e->fn_o = nullptr;
-    static const char* wf_names[3] = {"mm3e_gen_wf_first", "mm3e_gen_wf_cont", "mm3e_gen_wf_shade"};
-    for (int k = 0; k < 3 && e->ok && src.find(wf_names[k]) != std::string::npos; k++)
-      if (hipModuleGetFunction(&e->fn_wf[k], e->mod, wf_names[k]) != hipSuccess) e->fn_wf[k] = nullptr;
+    static const char* opt_names[GEN_N_OPT] = {"mm3e_gen_wf_first", "mm3e_gen_wf_cont", "mm3e_gen_wf_shade",
+                                               "mm3e_gen_film", "mm3e_gen_gbuffer"};
+    for (int k = 0; k < GEN_N_OPT && e->ok; k++)
+      if (src.find(opt_names[k]) == std::string::npos ||
+          hipModuleGetFunction(&e->fn_opt[k], e->mod, opt_names[k]) != hipSuccess)
+        e->fn_opt[k] = nullptr;
     e->done = true;
   }
   if (!e->ok) {
@@ -103,8 +106,8 @@ bool rtc_kernel(const std::string& src, int device, hipFunction_t* fn, hipFuncti
   }
   *fn = e->fn;
   if (fn_o) *fn_o = e->fn_o;
-  if (fn_wf)
-    for (int k = 0; k < 3; k++) fn_wf[k] = e->fn_wf[k];
+  if (fn_opt)
+    for (int k = 0; k < GEN_N_OPT; k++) fn_opt[k] = e->fn_opt[k];
   return true;
 }
 

@@ -57,6 +57,31 @@ FrameParams make_frame(const mm3e_camera& c, uint32_t w, uint32_t h) {
   return fp;
 }
 
+static_assert(GEN_N_OPT == 5, "GenJob::fn_opt holds one entry per GenOpt");
+
+hipFunction_t gen_opt(const mm3e_scene* s, int k) {
+  const GenJob* j = s->gen.get();
+  if (!j || j->state.load(std::memory_order_acquire) != MM3E_CODEGEN_READY) return nullptr;
+  return j->fn_opt[k];
+}
+
+hipError_t scene_gbuffer(const mm3e_scene* s, hipStream_t st, const FrameParams& fp, const GMover* movers,
+                         uint32_t n_movers, const uint32_t* list, const uint32_t* n_list, uint32_t max_list,
+                         uchar4* color, float* depth, int32_t* obj) {
+  hipFunction_t fn = gen_opt(s, GEN_GBUFFER);
+  if (!fn)
+    return launch_gbuffer(s->bounces, st, s->d_scene, s->general, fp, s->pp.exposure, movers, n_movers, list, n_list,
+                          max_list, color, depth, obj);
+  if (list && max_list == 0) return hipSuccess;
+  const uint32_t gx = list ? (max_list + 255) / 256 : (fp.width + 15) / 16;
+  const uint32_t gy = list ? 1u : (fp.height + 15) / 16;
+  const DevScene* sc = s->d_scene;
+  FrameParams f = fp;
+  float exposure = s->pp.exposure;
+  void* args[] = {&sc, &f, &exposure, &movers, &n_movers, &list, &n_list, &color, &depth, &obj};
+  return hipModuleLaunchKernel(fn, gx, gy, 1, 256, 1, 1, 0, st, args, nullptr);
+}
+
 // Tile shape for one block of <= 256 lanes: tw*th pixels x aa^2 samples.
 void tile_shape(uint32_t aa, uint32_t* tw, uint32_t* th) {
   uint32_t ns = aa * aa;
@@ -169,13 +194,13 @@ static void gen_run(mm3e::GenJob& j) {
   hipFunction_t fn = nullptr;
   std::string err;
   hipFunction_t fn_o = nullptr;
-  hipFunction_t fn_wf[3] = {nullptr, nullptr, nullptr};
-  const bool ok = rtc_kernel(j.src, j.device, &fn, &fn_o, err, fn_wf);
+  hipFunction_t fn_opt[GEN_N_OPT] = {};
+  const bool ok = rtc_kernel(j.src, j.device, &fn, &fn_o, err, fn_opt);
   {
     std::lock_guard<std::mutex> lk(j.mu);
     j.fn = fn;
     j.fn_o = fn_o;
-    for (int k = 0; k < 3; k++) j.fn_wf[k] = fn_wf[k];
+    for (int k = 0; k < GEN_N_OPT; k++) j.fn_opt[k] = fn_opt[k];
     j.err = err;
     j.state.store(ok ? MM3E_CODEGEN_READY : MM3E_CODEGEN_FAILED, std::memory_order_release);
   }
@@ -208,7 +233,7 @@ static void gen_schedule(mm3e_scene* s) {
   Blob b;
   build_blob(s->host, {}, {}, s->aa, s->bounces, (uint64_t)(uintptr_t)s->d_tables, s->d_gi, false, b);
   auto j = std::make_shared<GenJob>();
-  j->src = emit_scene_source(b, s->bounces, generated_prefers_sm(b), r->wf_budget > 0);
+  j->src = emit_scene_source(b, s->bounces, generated_prefers_sm(b), r->wf_budget > 0, r->gen_film);
   j->device = r->device;
   s->gen_stage = generated_needs_tables(b) ? s->stage_bytes : 0u;
   s->gen = j;
@@ -313,7 +338,7 @@ static bool launch_queued(mm3e_scene* s, const DevScene* dsc, const FrameParams&
   const uint32_t gs = bpc ? std::min(full, r->cus * bpc) : full;
   const mm3e::GenJob* gj = s->gen.get();
   const bool gen = dsc == s->d_scene && gj && gj->state.load(std::memory_order_acquire) == MM3E_CODEGEN_READY &&
-                   gj->fn_wf[0] && gj->fn_wf[1] && gj->fn_wf[2];
+                   gj->fn_opt[GEN_WF_FIRST] && gj->fn_opt[GEN_WF_CONT] && gj->fn_opt[GEN_WF_SHADE];
   uint32_t gstage = gen ? s->gen_stage : stage;
   const DevScene* sc_arg = dsc;
   FrameParams fp_arg = fp;
@@ -327,10 +352,10 @@ static bool launch_queued(mm3e_scene* s, const DevScene* dsc, const FrameParams&
     if (gen) {
       if (k == 0) {
         void* args[] = {&sc_arg, &fp_arg, &w, &b, &gstage};
-        e = hipModuleLaunchKernel(gj->fn_wf[0], g.x, 1, 1, bl.x, 1, 1, gstage, st, args, nullptr);
+        e = hipModuleLaunchKernel(gj->fn_opt[GEN_WF_FIRST], g.x, 1, 1, bl.x, 1, 1, gstage, st, args, nullptr);
       } else {
         void* args[] = {&sc_arg, &fp_arg, &w, &kk, &b, &gstage};
-        e = hipModuleLaunchKernel(gj->fn_wf[1], g.x, 1, 1, bl.x, 1, 1, gstage, st, args, nullptr);
+        e = hipModuleLaunchKernel(gj->fn_opt[GEN_WF_CONT], g.x, 1, 1, bl.x, 1, 1, gstage, st, args, nullptr);
       }
     } else {
       e = launch_wf_march(st, general, k, g, bl, dsc, stage, fp, w, b);
@@ -340,7 +365,7 @@ static bool launch_queued(mm3e_scene* s, const DevScene* dsc, const FrameParams&
   hipError_t e;
   if (gen) {
     void* args[] = {&sc_arg, &fp_arg, &w, &gstage};
-    e = hipModuleLaunchKernel(gj->fn_wf[2], gs, 1, 1, 256, 1, 1, gstage, st, args, nullptr);
+    e = hipModuleLaunchKernel(gj->fn_opt[GEN_WF_SHADE], gs, 1, 1, 256, 1, 1, gstage, st, args, nullptr);
   } else {
     e = launch_wf_shade(s->bounces, dim3(gs), st, dsc, stage, general, fp, w);
   }
@@ -430,6 +455,12 @@ static void env_wavefront(uint32_t* budget, uint32_t* passes, uint32_t* bpc) {
   }
 }
 
+// Default of mm3e_renderer_set_codegen_film: MM3E_CODEGEN_FILM=1 / on.
+static bool env_codegen_film() {
+  const char* v = std::getenv("MM3E_CODEGEN_FILM");
+  return v && (std::strcmp(v, "1") == 0 || lower(v) == "on");
+}
+
 int mm3e_list_adapters(char* buf, size_t buflen) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
@@ -486,6 +517,7 @@ int mm3e_renderer_create(const char* adapter_filter, mm3e_renderer** out, char* 
   }
   if (const char* to = std::getenv("MM3E_TILE_ORDER")) r->tile_order = !(lower(to) == "off" || lower(to) == "0");
   env_wavefront(&r->wf_budget, &r->wf_passes, &r->wf_bpc);
+  r->gen_film = env_codegen_film();
   r->cus = (uint32_t)std::max(1, p.multiProcessorCount);
   r->name = describe(dev);
   // Tables staged in LDS share the block's LDS with the 3 KiB sample-reduction buffer; keep
@@ -762,7 +794,7 @@ int mm3e_scene_kernel_name(const mm3e_scene* s, char* buf, size_t buflen) {
   uint32_t wf_b, wf_n, wf_c;
   wf_settings(s, &wf_b, &wf_n, &wf_c);
   if (wf_b > 0) {  // queued frame: name its shading stage
-    if (gen_ready && s->gen->fn_wf[2]) std::snprintf(buf, buflen, "mm3e_gen_wf_shade");
+    if (gen_ready && s->gen->fn_opt[GEN_WF_SHADE]) std::snprintf(buf, buflen, "mm3e_gen_wf_shade");
     else std::snprintf(buf, buflen, "%s%u", s->general ? "mm3e_wf_shade_x_b" : "mm3e_wf_shade_b", s->bounces);
     return MM3E_OK;
   }
@@ -873,10 +905,19 @@ int mm3e_scene_render_film(mm3e_scene* s, const mm3e_camera* cam, const mm3e_len
   FilmParams lp{lens->aperture_radius, lens->focus_distance, 0u, 0u};
   dim3 grid((s->w + s->tw - 1) / s->tw, (s->h + s->th - 1) / s->th);
   dim3 block(s->tw * s->th * s->aa * s->aa);
-  if (ok)
-    ok = hip_ok(launch_film(s->bounces, grid, block, st, s->d_scene, s->general, fp, lp, d_beauty, d_fg, d_alpha,
-                            s->tw, s->th),
-                "film launch", &msg);
+  if (ok) {
+    hipError_t e;
+    if (hipFunction_t fn = gen_opt(s, GEN_FILM)) {
+      const DevScene* sc = s->d_scene;
+      uint32_t tw = s->tw, th = s->th;
+      void* args[] = {&sc, &fp, &lp, &d_beauty, &d_fg, &d_alpha, &tw, &th};
+      e = hipModuleLaunchKernel(fn, grid.x, grid.y, 1, block.x, 1, 1, 0, st, args, nullptr);
+    } else {
+      e = launch_film(s->bounces, grid, block, st, s->d_scene, s->general, fp, lp, d_beauty, d_fg, d_alpha, s->tw,
+                      s->th);
+    }
+    ok = hip_ok(e, "film launch", &msg);
+  }
   if (ok) ok = hip_ok(launch_primary_aov_table(st, s->d_scene, fp, d_hit, d_ids, d_depth, nullptr, d_nrm), "aov", &msg);
   if (ok) ok = hip_ok(hipStreamSynchronize(st), "film", &msg);
   struct Out {
@@ -1167,6 +1208,13 @@ int mm3e_scene_autotune(mm3e_scene* s, const mm3e_camera* cam, uint32_t reps, ch
   return MM3E_OK;
 }
 
+int mm3e_renderer_set_codegen_film(mm3e_renderer* r, int on) {
+  if (!r) return MM3E_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(r->mu);
+  r->gen_film = on != 0;
+  return MM3E_OK;
+}
+
 int mm3e_renderer_set_codegen(mm3e_renderer* r, int mode) {
   if (!r || mode < MM3E_CODEGEN_OFF || mode > MM3E_CODEGEN_SYNC) return MM3E_ERR_INVALID;
   r->codegen = mode;
@@ -1200,7 +1248,7 @@ int mm3e_scene_emit_source(const void* ir, size_t ir_len, char* src, size_t* src
   build_blob(hs, {}, {}, hs.hdr.aa == 0 ? 1u : hs.hdr.aa, hs.hdr.bounces, 0, nullptr, false, b);
   uint32_t wb = 0, wn = 2, wc = 0;
   env_wavefront(&wb, &wn, &wc);
-  const std::string text = emit_scene_source(b, hs.hdr.bounces, generated_prefers_sm(b), wb > 0);
+  const std::string text = emit_scene_source(b, hs.hdr.bounces, generated_prefers_sm(b), wb > 0, env_codegen_film());
   const size_t need = text.size() + 1;
   if (!src || *src_len < need) {
     *src_len = need;

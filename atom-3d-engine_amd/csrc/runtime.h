@@ -23,9 +23,9 @@ struct GenJob {
   std::atomic<int> state{MM3E_CODEGEN_QUEUED};
   hipFunction_t fn = nullptr;
   hipFunction_t fn_o = nullptr;  // ordered-tiles variant
-  // stages of the queued frame (first march pass, continuation pass, shading); null unless
-  // the source was generated with them
-  hipFunction_t fn_wf[3] = {nullptr, nullptr, nullptr};
+  // optional kernels (GenOpt order: queued-frame stages, film, G-buffer); null unless the
+  // source was generated with them
+  hipFunction_t fn_opt[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   std::string err;
   std::mutex mu;
   std::condition_variable cv;
@@ -45,6 +45,7 @@ struct mm3e_renderer {
   // continuation passes, blocks per CU of the grid-stride stages (0: one lane per entry)
   uint32_t wf_budget = 0, wf_passes = 2, wf_bpc = 0;
   uint32_t cus = 0;              // compute units of the device
+  bool gen_film = false;         // generated modules also hold the film and G-buffer kernels
   std::mutex gen_mu;
   std::condition_variable gen_cv;
   std::deque<std::shared_ptr<mm3e::GenJob>> gen_queue;
@@ -102,4 +103,11 @@ void set_err(char* err, size_t n, const std::string& s);
 bool hip_ok(hipError_t e, const char* what, std::string* msg);
 FrameParams make_frame(const mm3e_camera& c, uint32_t w, uint32_t h);
 void tile_shape(uint32_t aa, uint32_t* tw, uint32_t* th);
+// The scene's generated optional kernel `k` (GenOpt order) when its module is ready, else null.
+hipFunction_t gen_opt(const mm3e_scene* s, int k);
+// G-buffer pass of a scene (its generated kernel when ready, else the table kernels): the
+// frame (list == nullptr) or the *n_list listed pixels (at most max_list).
+hipError_t scene_gbuffer(const mm3e_scene* s, hipStream_t st, const FrameParams& fp, const GMover* movers,
+                         uint32_t n_movers, const uint32_t* list, const uint32_t* n_list, uint32_t max_list,
+                         uchar4* color, float* depth, int32_t* obj);
 }  // namespace mm3e

@@ -27,7 +27,7 @@ EXPORTS = (
     "mm3e_batch_destroy", "mm3e_render_batch", "mm3e_server_default_config", "mm3e_server_create",
     "mm3e_server_add_scene", "mm3e_server_submit", "mm3e_server_poll", "mm3e_server_wait", "mm3e_server_stats",
     "mm3e_server_destroy", "mm3e_scene_kernel_name",
-    "mm3e_scene_info", "mm3e_renderer_set_codegen", "mm3e_renderer_set_tile_order", "mm3e_renderer_set_wavefront", "mm3e_scene_set_wavefront", "mm3e_scene_autotune", "mm3e_selftest_sqrt", "mm3e_scene_codegen_status", "mm3e_scene_emit_source",
+    "mm3e_scene_info", "mm3e_renderer_set_codegen", "mm3e_renderer_set_tile_order", "mm3e_renderer_set_wavefront", "mm3e_renderer_set_codegen_film", "mm3e_scene_set_wavefront", "mm3e_scene_autotune", "mm3e_selftest_sqrt", "mm3e_scene_codegen_status", "mm3e_scene_emit_source",
     "mm3e_scene_parse_text", "mm3e_scene_to_text",
     "mm3e_mesh_parse_obj", "mm3e_mesh_sdf_layout", "mm3e_mesh_bake_sdf", "mm3e_mesh_baker_create",
     "mm3e_mesh_baker_layout", "mm3e_mesh_baker_info", "mm3e_mesh_baker_run", "mm3e_mesh_baker_wait",
@@ -75,6 +75,7 @@ def lib():
         L.mm3e_renderer_set_codegen.argtypes = [P, C.c_int]
         L.mm3e_renderer_set_tile_order.argtypes = [P, C.c_int]
         L.mm3e_renderer_set_wavefront.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.mm3e_renderer_set_codegen_film.argtypes = [P, C.c_int]
         L.mm3e_scene_set_wavefront.argtypes = [P, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32]
         L.mm3e_scene_autotune.argtypes = [P, P, C.c_uint32, C.c_char_p, C.c_size_t]
         L.mm3e_selftest_sqrt.argtypes = [P, C.POINTER(C.c_uint64)]
@@ -444,6 +445,12 @@ class Renderer:
         rc = lib().mm3e_renderer_set_tile_order(self.h, 1 if on else 0)
         if rc != 0:
             raise Mm3eError(rc, "set_tile_order failed")
+
+    def set_codegen_film(self, on: bool):
+        """Generated modules of scenes compiled afterwards also hold the film and G-buffer kernels."""
+        rc = lib().mm3e_renderer_set_codegen_film(self.h, 1 if on else 0)
+        if rc != 0:
+            raise Mm3eError(rc, "set_codegen_film failed")
 
     def set_wavefront(self, budget: int, passes: int = 2, blocks_per_cu: int = 0):
         """Queued frames: march passes of at most `budget` evaluations over compacted queues of

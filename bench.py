@@ -31,8 +31,9 @@ More legs ride on the same line (``--legs`` picks them):
            longest axis (124x63x128) on the device; ms per bake (row-parity + nearest-triangle
            kernels, HIP events), Msamples/s, BVH tests per sample, and the CPU oracle's rate.
   frame_gen: c2 at 960x540 as a G-buffer keyframe plus reprojected frames for the camera swinging
-           0.01 rad per frame about its target (reproject_hybrid: warp + holes rendered for real); ms per
-           G-buffer frame vs per generated frame (both including the RGBA8 readback), holes.
+           0.01 rad per frame about its target (reproject_hybrid: warp + holes rendered for real), with
+           the scene's generated G-buffer / hole kernels; ms per G-buffer frame vs per generated frame
+           (both including the RGBA8 readback), holes.
   queued:  c2 compiled with the queued-frame stages (budgeted march passes over compacted ray
            queues, hit-queue shading); the scene autotuner's render-stage ms for the one-kernel
            frame and each queued candidate, its choice, and whether the images are equal.
@@ -336,11 +337,18 @@ def mesh_bake_leg(hip, configs, r, reps, resolution=128, padding=0.35, cpu=True)
     return out
 
 
-def frame_gen_leg(hip, configs, r, frames=8):
-    """render_gbuffer keyframe + reproject_hybrid frames on c2 (lib.rs:951-1080)."""
+def frame_gen_leg(hip, configs, r_main, frames=8):
+    """render_gbuffer keyframe + reproject_hybrid frames on c2 (lib.rs:951-1080), with the scene's
+    generated G-buffer / hole kernels."""
     import math
     sc, cam = configs.c2_scene()
+    r = hip.Renderer()
+    r.set_codegen(hip.CODEGEN_SYNC)
+    r.set_codegen_film(True)
+    t0 = time.perf_counter()
     g = r.compile(sc.to_ir(), sc.width, sc.height)
+    cg_state, _ = g.codegen_status(wait=True)
+    codegen_s = time.perf_counter() - t0
     gf = hip.GFrame(r, sc.width, sc.height)
     # the c2 camera (eye (0, 2, 7.5) looking at (0, 1.6, 0)) swung about the vertical through its target
     cams = [configs.look_at((7.5 * math.sin(0.01 * k), 2.0, 7.5 * math.cos(0.01 * k)), (0.0, 1.6, 0.0),
@@ -360,10 +368,12 @@ def frame_gen_leg(hip, configs, r, frames=8):
     gen_ms = t / frames * 1e3
     gf.close()
     g.close()
+    r.close()
     px = sc.width * sc.height
     out = {"workload": f"c2 960x540, the c2 camera as keyframe, {frames} frames swinging 0.01 rad each about "
                        f"the vertical through its target", "gbuffer_ms": round(key_ms, 3), "hybrid_ms": round(gen_ms, 3),
-           "speedup": round(key_ms / gen_ms, 2), "mean_hole_fraction": round(sum(holes) / len(holes) / px, 5)}
+           "speedup": round(key_ms / gen_ms, 2), "mean_hole_fraction": round(sum(holes) / len(holes) / px, 5),
+           "generated_kernels": cg_state == hip.CODEGEN_READY, "compile_and_codegen_s": round(codegen_s, 2)}
     log(f"frame_gen: keyframe {out['gbuffer_ms']} ms, generated {out['hybrid_ms']} ms "
         f"({out['speedup']}x), holes {out['mean_hole_fraction']}")
     return out

@@ -166,6 +166,12 @@ enum { MM3E_CODEGEN_OFF = 0, MM3E_CODEGEN_BACKGROUND = 1, MM3E_CODEGEN_SYNC = 2 
 enum { MM3E_CODEGEN_NONE = 0, MM3E_CODEGEN_QUEUED = 1, MM3E_CODEGEN_READY = 2, MM3E_CODEGEN_FAILED = 3 };
 /* Mode for scenes compiled afterwards on this renderer. */
 int mm3e_renderer_set_codegen(mm3e_renderer* r, int mode);
+/* Generated modules of scenes compiled afterwards also hold the film (render_film, film.rs:63-198)
+ * and G-buffer / hole (render_gbuffer / reproject_hybrid, lib.rs:951-1080) kernels, which then
+ * replace the table-driven ones for those calls; default off (the module takes about twice as
+ * long to compile).  Images are the same bit for bit either way.  MM3E_CODEGEN_FILM=1 sets the
+ * default for new renderers (and makes mm3e_scene_emit_source include them). */
+int mm3e_renderer_set_codegen_film(mm3e_renderer* r, int on);
 /* Tile order for generated kernels of scenes compiled afterwards (default on; the environment
  * variable MM3E_TILE_ORDER=off sets the default off): the frame's tiles are dispatched most
  * expensive first, ordered by the previous frame's per-tile costs, so a frame whose cost sits
