@@ -169,7 +169,7 @@ bool ensure_order(mm3e_batch* b, std::string& msg) {
 
 TileOrder batch_order(const mm3e_batch* b) {
   if (!b->d_tq) return TileOrder{};
-  return TileOrder{b->d_tq, b->d_tq + b->tq_tiles, b->d_tq + 2 * b->tq_tiles, 0u, 0u};
+  return TileOrder{b->d_tq, b->d_tq + b->tq_tiles, b->d_tq + 2 * b->tq_tiles, nullptr, nullptr, 0u, 0u};
 }
 
 // Persistent render of the table jobs on `ts`, then the next replay's order.

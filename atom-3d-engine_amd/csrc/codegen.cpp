@@ -53,7 +53,7 @@ void put_obj(std::string& o, const DObj& d) {
   put_fa(o, d.rep, 3); o += ", "; put_f(o, d.twist); o += ", ";
   put_fa(o, d.elo, 3); o += ", "; put_f(o, d.bend); o += ", ";
   put_f(o, d.round); o += ", "; put_f(o, d.onion); o += ", "; put_f(o, d.bc_r); o += ", "; put_f(o, d.k); o += ", ";
-  put_fa(o, d.bc, 3); o += ", "; put_f(o, d.bd); o += "}";
+  put_fa(o, d.bc, 3); o += ", 0u}";
 }
 
 bool has_csg(const Blob& b) { return b.header.csg != nullptr; }

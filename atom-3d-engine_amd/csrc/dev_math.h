@@ -324,16 +324,7 @@ MM3E_DEV float object_distance(const DevScene& sc, const DObj& o, V3 p) {
 //   smooth, k>0:  thr = max(acc + k, slack)      ((lower - acc)/k >= 1 -> h = 1 -> acc)
 //   subtract:     thr = max(-acc, slack)         (max(acc, -lower) = acc)
 // No sqrt and no smooth-union division are spent on objects that cannot matter.
-// For a slab skip bound (OBJ_SLAB) the lower bound is |dot(n, p) - bd| - bc_r; the rounding of
-// this dot product and of the object's own local coordinate (p - pos, the rotation, 1/scale)
-// is covered by an absolute margin of 2e-6 * (|p|_1 + |pos|_1), some ten ulps of every term
-// (|n_i| <= 1), on top of the same relative margin.
 MM3E_DEV bool far_noop(const DObj& o, V3 p, float thr) {
-  if (o.flags & OBJ_SLAB) {
-    const float e = rabs(dot(ld3(o.bc), p) - o.bd);
-    const float m = 2e-6f * (((rabs(p.x) + rabs(p.y)) + rabs(p.z)) + ((rabs(o.pos[0]) + rabs(o.pos[1])) + rabs(o.pos[2])));
-    return e - m > (thr + o.bc_r) * 1.00002f + 1e-6f;
-  }
   const float t = (thr + o.bc_r) * 1.00002f + 1e-6f;
   V3 d = p - ld3(o.bc);
   return t > 0.0f && dot(d, d) > t * t;

@@ -2,9 +2,9 @@
 // implementation (`TableMap` walks the compiled scene tables; generated maps specialise it).
 //
 //   render_tile      one lane per (pixel, AA sample): march + shade + reflection chain
-//                    (lib.rs:1223-1360), nested loops or a per-lane state machine; the block's
-//                    samples meet in LDS and the first lane of each pixel sums them in the
-//                    reference's sy-major order.
+//                    (lib.rs:1223-1360), nested loops or a per-lane state machine; the first
+//                    lane of each pixel sums its samples in the reference's sy-major order, by
+//                    cross-lane reads when aa^2 divides 64, else through LDS.
 //   gather_rays      GI bake rays (lib.rs:686-711): one lane per (probe, face, ray).
 //   gi_reduce        weighted mean per cube face (gi.rs:327-341).
 //   bloom_h / bloom_v_resolve / resolve_only   post.rs:36-115 + framebuffer.rs:102-113.
@@ -391,6 +391,21 @@ MM3E_DEV void render_tile(const DevScene* __restrict__ scp, const Tables& tb, co
     }
     return;
   }
+  if ((64u % ns) == 0u) {
+    // A pixel's samples sit in adjacent lanes of one wave (lane t = pixel * ns + sample with
+    // ns | 64): its first lane sums them by cross-lane reads in sample order, the same additions
+    // as below, and no wave waits for the rest of the block.
+    V3 acc = V3{0.0f, 0.0f, 0.0f} + rad;
+    for (uint32_t k = 1; k < ns; k++)
+      acc = acc + V3{__shfl_down(rad.x, k), __shfl_down(rad.y, k), __shfl_down(rad.z, k)};
+    if (valid && s == 0) {
+      const float inv = 1.0f / (float)ns;
+      acc = scale(acc, inv);
+      float* o = hdr + 3 * ((size_t)py * fp.width + px);
+      o[0] = acc.x; o[1] = acc.y; o[2] = acc.z;
+    }
+    return;
+  }
   smem[3 * t + 0] = rad.x;
   smem[3 * t + 1] = rad.y;
   smem[3 * t + 2] = rad.z;
@@ -412,27 +427,40 @@ MM3E_DEV void render_tile(const DevScene* __restrict__ scp, const Tables& tb, co
 // Ordered tiles: a launch of one block per tile whose block b renders tile order[b], so the
 // tiles the previous frame found most expensive are dispatched first.  A frame whose cost sits
 // in a few long tiles otherwise ends with those tiles running alone on an idle chip; started
-// first, they overlap the cheap ones.  Each block draws a ticket from one global counter when
-// it starts and another when it ends; the difference (how many blocks started or finished
-// meanwhile) is the tile's cost for the next frame's order.  Every tile is rendered exactly
+// first, they overlap the cheap ones.  The block's first wave draws a ticket from one global
+// counter when it starts, and the tile's last wave to finish draws another; the difference (how
+// many tiles started or finished meanwhile) is the tile's cost for the next frame's order.  The
+// waves of a tile finish independently (a per-tile count of finished waves, reset by the last
+// one), so no wave waits at a barrier for the tile's slowest.  Every tile is rendered exactly
 // as render_tile renders it, whatever the order.
 // ---------------------------------------------------------------------------------------
 struct TileOrder {
   const uint32_t* order;  // tile indices (ty * tiles_x + tx), most expensive first
-  uint32_t* cost;         // per tile: tickets drawn by other blocks while it ran
+  uint32_t* cost;         // per tile: tickets drawn by other tiles while it ran
   uint32_t* ticket;       // running ticket counter (wraps; only differences are used)
+  uint32_t* start;        // per tile: its start ticket (plain launches only)
+  uint32_t* done;         // per tile: waves finished this frame (plain launches only; 0 between frames)
   uint32_t tiles_x, pad;
 };
 
 template <class Map, int MAXB, bool SM>
 MM3E_DEV void render_ordered(const DevScene* __restrict__ scp, const Tables& tb, const FrameParams& fp,
                              float* __restrict__ hdr, uint32_t tw, uint32_t th, const TileOrder& o, float* smem,
-                             uint32_t* start) {
+                             uint32_t* /*unused*/) {
   const uint32_t tile = o.order[blockIdx.x];
-  if (threadIdx.x == 0) *start = atomicAdd(o.ticket, 1u);
+  // Device-scope atomics only: a __threadfence here writes back the XCD's L2 and costs more
+  // than the frame (measured: c2 1.4 -> 2.2 ms).  The cost is a heuristic for the next order,
+  // so the start ticket is published by an atomic exchange and read back by an atomic add of 0.
+  if (threadIdx.x == 0) atomicExch(o.start + tile, atomicAdd(o.ticket, 1u));
   render_tile<Map, MAXB, SM>(scp, tb, fp, hdr, tw, th, tile % o.tiles_x, tile / o.tiles_x, smem);
-  __syncthreads();
-  if (threadIdx.x == 0) o.cost[tile] = atomicAdd(o.ticket, 1u) - *start;
+  if (__lane_id() == 0) {  // every lane of the wave is here: render_tile returns convergent
+    const uint32_t waves = (blockDim.x + 63u) / 64u;
+    if (atomicAdd(o.done + tile, 1u) == waves - 1u) {  // the tile's last wave
+      const uint32_t t0 = atomicAdd(o.start + tile, 0u);
+      o.cost[tile] = atomicAdd(o.ticket, 1u) - t0;
+      atomicExch(o.done + tile, 0u);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------------------

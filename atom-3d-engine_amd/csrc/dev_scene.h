@@ -31,9 +31,6 @@ enum : uint32_t {
   // sphere that provably lies below its field (see skip_bound in scene_compile.cpp): used only
   // to prove that folding the object changes nothing, never as its field value.
   OBJ_SKIP = 1u << 18,
-  // With OBJ_SKIP: the skip bound is a slab, not a sphere (repeated objects, see slab_bound in
-  // scene_compile.cpp): field >= |dot(bc, p) - bd| - bc_r with bc a unit rotation column.
-  OBJ_SLAB = 1u << 19,
 };
 
 // One placed object (mm3e-orchestrator/src/lib.rs:330-336) with its world bounding sphere
@@ -55,8 +52,8 @@ struct alignas(16) DObj {
   float round, onion;
   float bc_r;       // bounding-sphere radius
   float k;          // smooth-union radius (Combine::Smooth)
-  float bc[3];      // bounding-sphere centre (OBJ_SLAB: the slab normal)
-  float bd;         // OBJ_SLAB: the slab's offset along bc
+  float bc[3];      // bounding-sphere centre
+  uint32_t pad1;
 };
 
 // accel.rs:13-19 plus a parent link so the device walks the tree without a stack.

@@ -259,7 +259,7 @@ static bool tile_order_ready(mm3e_scene* s, uint32_t n, std::string& msg) {
     hipFree(s->d_to);
     s->d_to = nullptr;
   }
-  std::vector<uint32_t> init(2 * (size_t)n + 1, 0u);
+  std::vector<uint32_t> init(4 * (size_t)n + 1, 0u);  // [order][cost][start][done][ticket]
   for (uint32_t i = 0; i < n; i++) init[i] = i;
   if (!hip_ok(hipMalloc((void**)&s->d_to, init.size() * 4), "hipMalloc(tile order)", &msg) ||
       !hip_ok(hipMemcpy(s->d_to, init.data(), init.size() * 4, hipMemcpyHostToDevice), "upload tile order", &msg)) {
@@ -270,6 +270,12 @@ static bool tile_order_ready(mm3e_scene* s, uint32_t n, std::string& msg) {
   }
   s->to_tiles = n;
   return true;
+}
+
+static TileOrder scene_tile_order(const mm3e_scene* s, uint32_t tiles_x) {
+  const size_t n = s->to_tiles;
+  uint32_t* d = s->d_to;
+  return TileOrder{d, d + n, d + 4 * n, d + 2 * n, d + 3 * n, tiles_x, 0u};
 }
 
 // The queued-frame setting in force for a scene: its own, else the renderer's.
@@ -402,7 +408,7 @@ bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std
     float* hdr_arg = s->d_hdr;
     uint32_t tw = s->tw, th = s->th, gstage = s->gen_stage;
     if (gj->fn_o && s->to_on && tile_order_ready(s, grid.x * grid.y, msg)) {
-      TileOrder o{s->d_to, s->d_to + s->to_tiles, s->d_to + 2 * s->to_tiles, grid.x, 0u};
+      TileOrder o = scene_tile_order(s, grid.x);
       void* args[] = {&sc_arg, &fp_arg, &hdr_arg, &tw, &th, &gstage, &o};
       if (!hip_ok(hipModuleLaunchKernel(gj->fn_o, s->to_tiles, 1, 1, block.x, 1, 1, gstage, st, args, nullptr),
                   "generated render launch", &msg))
@@ -415,7 +421,7 @@ bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std
         return false;
     }
   } else if (dsc == s->d_scene && s->to_on && tile_order_ready(s, grid.x * grid.y, msg)) {
-    TileOrder o{s->d_to, s->d_to + s->to_tiles, s->d_to + 2 * s->to_tiles, grid.x, 0u};
+    TileOrder o = scene_tile_order(s, grid.x);
     if (!hip_ok(launch_render_ordered(s->bounces, s->to_tiles, block, st, dsc, stage, general, fp, s->d_hdr, s->tw,
                                       s->th, o),
                 "render launch", &msg))

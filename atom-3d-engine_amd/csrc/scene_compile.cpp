@@ -496,43 +496,6 @@ Sphere skip_bound(const mm3e_ir_object& o) {
   return Sphere{true, {o.pos[0], o.pos[1], o.pos[2]}, (float)rad};
 }
 
-// A slab below the field of a repeated object (unbounded in the reference, lib.rs:478-480)
-// along a local axis a that is not repeated (rep[a] <= 0) and that the warps keep: twist
-// rotates x and z about y (keeps y), bend rotates x and y (keeps z).  Mirror keeps |l_a|,
-// elongation moves l_a by at most |h_a|, repetition leaves it, so the final local point q has
-// |q_a| >= |l_a| - |h_a| and, for an exact-SDF primitive, SDF(q) >= |q| - |c| - r >= |q_a| - |c| - r.
-// With l_a = dot(column_a, p - pos) / scale the field is at least
-// |dot(n, p) - d| - H,  n = column_a,  d = dot(n, pos),  H = (|h_a| + |c| + r) * scale + max(round, 0) + |onion|.
-// Returned as n (3), d and H (rounded up); `some` false when no axis qualifies.
-struct Slab {
-  bool some;
-  float n[3], d, h;
-};
-static Slab slab_bound(const mm3e_ir_object& o) {
-  Slab none{false, {0, 0, 0}, 0, 0};
-  if (!nonzero3(o.repeat) || !(o.scale > 0.0f) || !std::isfinite(o.scale)) return none;
-  double c = 0.0, r = 0.0;
-  if (!exact_prim_bound(o, &c, &r)) return none;
-  const bool twist = o.twist != 0.0f, bend = o.bend != 0.0f;
-  int axis = -1;
-  for (int a : {1, 2, 0}) {
-    if (o.repeat[a] > 0.0f) continue;
-    if ((a == 1 && bend) || (a == 0 && (twist || bend)) || (a == 2 && twist)) continue;
-    axis = a;
-    break;
-  }
-  if (axis < 0) return none;
-  const float* n = o.rot + 3 * axis;
-  double hh = (std::fabs((double)o.elongate[axis]) + c + r) * (double)o.scale + std::max((double)o.round, 0.0) +
-              std::fabs((double)o.onion);
-  hh = hh * (1.0 + 1e-6) + 1e-6;
-  const double d = (double)n[0] * o.pos[0] + (double)n[1] * o.pos[1] + (double)n[2] * o.pos[2];
-  if (!std::isfinite(hh) || !std::isfinite(d) || !(hh > 0.0)) return none;
-  for (int k = 0; k < 3; k++)
-    if (!std::isfinite(n[k])) return none;
-  return Slab{true, {n[0], n[1], n[2]}, (float)d, (float)hh};
-}
-
 bool gi_bounds(const HostScene& sc, float lo[3], float hi[3]) {
   F3 l{INFINITY, INFINITY, INFINITY}, h{-INFINITY, -INFINITY, -INFINITY};
   bool any = false;
@@ -751,10 +714,6 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
   std::vector<Sphere> bounds(n_obj);       // Object::world_bound (the reference's plan and prune)
   std::vector<Sphere> grid_bounds(n_obj);  // the same, or a skip sphere (candidate grid only)
   float max_k = 0.0f;
-  // slab skip bounds of repeated objects (MM3E_SKIP_SLABS=off leaves them out; the field is the
-  // same either way)
-  const char* slab_env = std::getenv("MM3E_SKIP_SLABS");
-  const bool slabs = !(slab_env && (std::strcmp(slab_env, "off") == 0 || std::strcmp(slab_env, "0") == 0));
   for (uint64_t i = 0; i < n_obj; i++) {
     const mm3e_ir_object& o = objs[i];
     DObj& d = dobj[i];
@@ -802,17 +761,11 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
       d.bc_r = bounds[i].r;
     } else {
       Sphere k = skip_bound(o);
-      Slab sl = k.some || !slabs ? Slab{false, {0, 0, 0}, 0, 0} : slab_bound(o);
       if (k.some) {
         f |= OBJ_SKIP;
         std::memcpy(d.bc, k.c, 12);
         d.bc_r = k.r;
         grid_bounds[i] = k;
-      } else if (sl.some) {  // far tests only; the candidate grid keeps the object everywhere
-        f |= OBJ_SKIP | OBJ_SLAB;
-        std::memcpy(d.bc, sl.n, 12);
-        d.bd = sl.d;
-        d.bc_r = sl.h;
       }
     }
     d.flags = f;

@@ -179,30 +179,6 @@ def test_field_parity_repeated_objects(renderer):
     assert bad.mean() <= 1e-3 and (mg == mo).mean() >= 0.999
 
 
-def test_slab_skip_bounds_change_no_field_value(renderer, monkeypatch):
-    """Repeated objects carry a slab skip bound on the device (along a local axis that is not
-    repeated and that twist / bend keep); it only proves that folding the object changes
-    nothing.  With the slabs switched off (MM3E_SKIP_SLABS=off at compile) the device must give
-    the same field bit for bit, and the same image."""
-    sc = _repeated_scene(160, 90)
-    cam = configs.orbit_camera((0.0, 1.0, 0.0), 9.0, 0.7, 0.35, np.radians(55.0))
-    pts = _repeated_points()
-    g = renderer.compile(sc.to_ir(), sc.width, sc.height)
-    monkeypatch.setenv("MM3E_SKIP_SLABS", "off")
-    g0 = renderer.compile(sc.to_ir(), sc.width, sc.height)
-    monkeypatch.delenv("MM3E_SKIP_SLABS")
-    d1, m1 = g.field(pts)
-    d0, m0 = g0.field(pts)
-    i1, i0 = g.render_rgba(cam), g0.render_rgba(cam)
-    same = d1.view(np.uint32) == d0.view(np.uint32)
-    report("slab skip", {"field_equal": float(same.mean()), "mat_equal": float((m1 == m0).mean()),
-                         "image_equal": bool((i1 == i0).all())})
-    assert same.all() and (m1 == m0).all()
-    assert (i1 == i0).all()
-    g.close()
-    g0.close()
-
-
 def test_march_parity(renderer):
     sc, cam = configs.c1_scene()
     g, o = both(renderer, sc)
