@@ -330,12 +330,20 @@ MM3E_DEV bool far_noop(const DObj& o, V3 p, float thr) {
   return t > 0.0f && dot(d, d) > t * t;
 }
 
-// accel.rs:187-207 WorldPlan::contrib.
+// accel.rs:187-207 WorldPlan::contrib.  The bound's root is skipped where its test must fail:
+// with t = (slack + r) * (1 - 2e-5) - 1e-6 > 0, |p - c|^2 < t^2 puts |p - c| - r below slack by
+// a margin far above the rounding of the root and the subtraction, so the reference's
+// `lower > slack` is false there and the exact field follows either way.
 template <bool CSG>
 MM3E_DEV Field contrib(const DevScene& sc, const DObj& o, V3 p) {
   if (o.flags & OBJ_BOUNDED) {
-    float lower = length(p - ld3(o.bc)) - o.bc_r;
-    if (lower > sc.slack) return Field{lower, o.mat};
+    const V3 d = p - ld3(o.bc);
+    const float d2 = dot(d, d);
+    const float t = (sc.slack + o.bc_r) * 0.99998f - 1e-6f;
+    if (!(t > 0.0f && d2 < t * t)) {
+      const float lower = crsqrt(d2) - o.bc_r;
+      if (lower > sc.slack) return Field{lower, o.mat};
+    }
   }
   return Field{object_distance<CSG>(sc, o, p), o.mat};
 }
