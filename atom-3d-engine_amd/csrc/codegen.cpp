@@ -74,11 +74,15 @@ std::string emit_scene_source(const Blob& b, uint32_t maxb, bool sm, bool queued
   o.reserve(4096 + b.objs.size() * 900);
   o += "// generated: world fold of one compiled scene (objects: " + std::to_string(b.objs.size()) +
        ", segments: " + std::to_string(b.segs.size()) + ")\n";
-  // Waves per SIMD the generated kernel is register-allocated for: 6 (at most 80 VGPRs).  The
-  // kernel is bound by dependency latency more than by issue; measured on MI355X against 3, 4,
-  // 5 and 8: c2 960x540 1.62 / 1.62 / 1.43 / 1.35 (6) / 1.74 ms, a 40-object 1280x720 scene
-  // 15.4 / 12.6 / 11.8 / 11.9 / 16.2 ms.  MM3E_GEN_WAVES overrides it.
-  int waves = 6;
+  // Waves per SIMD the generated kernel is register-allocated for.  Nested path: 6 (at most 80
+  // VGPRs); it is bound by dependency latency more than by issue (c2 960x540 at 4 / 5 / 6 / 7
+  // waves: 1.59 / 1.38 / 1.32 / 1.34 ms; c1 and a CSG scene at 1280x720 also fastest at 6).
+  // State-machine path: 4 (at most 128 VGPRs) -- its per-lane path state spills in the hot loop
+  // below that (1280x720 procedural scenes of 13 / 21 / 41 objects at 4 / 5 / 6 waves:
+  // 2.08 / 4.06 / 6.83, 5.40 / 6.38 / 7.61, 6.37 / 8.15 / 10.05 ms) -- except for long folds,
+  // where latency hiding wins again (61 objects: 22.2 / 20.6 / 20.4 ms): 5 above 48 objects.
+  // MM3E_GEN_WAVES overrides.
+  int waves = sm ? (b.objs.size() > 48 ? 5 : 4) : 6;
   if (const char* wv = std::getenv("MM3E_GEN_WAVES")) {
     const int n = std::atoi(wv);
     if (n >= 1 && n <= 8) waves = n;

@@ -223,13 +223,20 @@ static void gen_worker(mm3e_renderer* r) {
 
 // Emit the scene's generated kernel and compile it (SYNC) or queue it (BACKGROUND).  Scenes
 // with more than kCodegenMaxObjects objects keep the table kernels: measured on procedural
-// scenes at 1280x720, the generated fold is 1.13-1.27x faster up to ~48 objects, even at
-// 64-96 and 0.89-0.92x at 128 (its code no longer fits the instruction cache, while the
-// table walk's does).
-static const size_t kCodegenMaxObjects = 64;
+// scenes at 1280x720 (ordered tiles, the generated kernel's occupancy picked by codegen), the
+// generated fold is 1.24x faster at 40 objects, 1.26x at 60, 1.09x at 80, 0.95x at 100 and
+// 0.93x at 130 (its code no longer fits the instruction cache, while the table walk's does).
+static const size_t kCodegenMaxObjects = 80;
+static size_t codegen_max_objects() {  // MM3E_CODEGEN_MAX_OBJECTS overrides (tuning)
+  if (const char* v = std::getenv("MM3E_CODEGEN_MAX_OBJECTS")) {
+    const long n = std::atol(v);
+    if (n >= 0) return (size_t)n;
+  }
+  return kCodegenMaxObjects;
+}
 static void gen_schedule(mm3e_scene* s) {
   mm3e_renderer* r = s->r;
-  if (s->host.objects.size() > kCodegenMaxObjects) return;
+  if (s->host.objects.size() > codegen_max_objects()) return;
   Blob b;
   build_blob(s->host, {}, {}, s->aa, s->bounces, (uint64_t)(uintptr_t)s->d_tables, s->d_gi, false, b);
   auto j = std::make_shared<GenJob>();
