@@ -19,10 +19,24 @@ def _scenes():
     csg, camc = configs.csg_scene(320, 180)
     refl = configs.procedural_scene(43, 40, 240, 135)
     refl.bounces, refl.fog_density, refl.aa = 2, 0.03, 2
+    p20 = configs.procedural_scene(40, 20, 320, 180)  # segment grid, nested generated kernel
     return [("c1", sc1, cam1), ("c2", sc2, cam2),
+            ("procedural20", p20, configs.orbit_pose(configs.SplitMix64(40))),
             ("procedural60", p60, configs.orbit_pose(configs.SplitMix64(41))),
             ("procedural40", p40, configs.orbit_pose(configs.SplitMix64(42))),
             ("csg", csg, camc), ("bounces2", refl, configs.orbit_pose(configs.SplitMix64(43)))]
+
+
+def test_generated_kernel_family():
+    """Up to 24 objects the generated kernel takes the nested path, segment grid or not; above,
+    the state machine, register-allocated for 4 waves per SIMD (5 above 48 objects)."""
+    small = hip.emit_source(configs.procedural_scene(40, 20, 64, 36).to_ir())
+    assert "grid_select" in small and "render_tile<GenMap, 1, false>" in small
+    assert "#define MM3E_RENDER_WAVES 6" in small
+    mid = hip.emit_source(configs.procedural_scene(42, 40, 64, 36).to_ir())
+    assert "render_tile<GenMap, 1, true>" in mid and "#define MM3E_RENDER_WAVES 4" in mid
+    big = hip.emit_source(configs.procedural_scene(41, 60, 64, 36).to_ir())
+    assert "#define MM3E_RENDER_WAVES 5" in big
 
 
 def test_emitted_source_names_every_segment():
