@@ -18,7 +18,9 @@
 #include <vector>
 
 #include "batch.h"
+#include "codegen.h"
 #include "launch.h"
+#include "rtc.h"
 #include "runtime.h"
 
 struct mm3e_batch {
@@ -56,6 +58,11 @@ struct mm3e_batch {
   // records tile costs and re-sorts the order for the next one (expensive tiles first)
   uint32_t* d_tq = nullptr;
   uint32_t tq_tiles = 0;
+  // Generated batch module: the maps of the batch's scenes that get generated kernels, in one
+  // module with the persistent loop (mm3e_gen_batch); built by the autotuner, used when faster.
+  std::vector<const mm3e_scene*> gen_scenes;  // map k renders jobs of gen_scenes[k]
+  hipFunction_t gen_fn = nullptr;
+  bool use_gen = false;
 };
 
 namespace mm3e {
@@ -114,6 +121,9 @@ void build_tables(mm3e_batch* b) {
       J.tile_base = tile;
       tile += J.tiles_x * tiles_y;
       J.stage_bytes = s->hdr.stage_bytes;
+      J.gen = kGenTable;
+      for (size_t g = 0; g < b->gen_scenes.size(); g++)
+        if (b->gen_scenes[g] == s) J.gen = (uint32_t)g;
       stage_max = std::max(stage_max, J.stage_bytes);
       all_fit = all_fit && J.stage_bytes <= b->r->lds_budget;
     }
@@ -176,10 +186,21 @@ TileOrder batch_order(const mm3e_batch* b) {
 bool enqueue_persistent(mm3e_batch* b, hipStream_t ts, std::string& msg) {
   const TileOrder q = batch_order(b);
   if (!hip_ok(launch_batch_reset(ts, b->d_counter), "batch reset", &msg)) return false;
-  if (!hip_ok(launch_batch_render(b->maxb, b->cus * b->blocks_per_cu, ts, b->d_jobs, (uint32_t)b->jobs.size(),
-                                  b->total_tiles, b->d_counter, b->lds_bytes, q),
-              "batch render", &msg))
+  if (b->use_gen && b->gen_fn) {
+    const BatchJob* jobs = b->d_jobs;
+    uint32_t n_jobs = (uint32_t)b->jobs.size(), total = b->total_tiles;
+    uint32_t* counter = b->d_counter;
+    TileOrder qq = q;
+    void* args[] = {&jobs, &n_jobs, &total, &counter, &qq};
+    if (!hip_ok(hipModuleLaunchKernel(b->gen_fn, b->cus * b->blocks_per_cu, 1, 1, 256, 1, 1, b->lds_bytes, ts, args,
+                                      nullptr),
+                "generated batch render", &msg))
+      return false;
+  } else if (!hip_ok(launch_batch_render(b->maxb, b->cus * b->blocks_per_cu, ts, b->d_jobs,
+                                         (uint32_t)b->jobs.size(), b->total_tiles, b->d_counter, b->lds_bytes, q),
+                     "batch render", &msg)) {
     return false;
+  }
   if (q.order && !hip_ok(launch_tile_order(ts, q.cost, b->tq_tiles, b->d_tq), "batch tile order", &msg)) return false;
   return true;
 }
@@ -256,6 +277,35 @@ bool capture(mm3e_batch* b, std::string& msg) {
   if (!hip_ok(e, "end capture", &msg)) return false;
   b->graph = g;
   return hip_ok(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "graph instantiate", &msg);
+}
+
+// The generated batch module over the batch's scenes that qualify for generated kernels (code
+// generation on, at most codegen_object_limit() objects): their maps and the persistent loop in
+// one hipRTC module.  False (and no module) when no scene qualifies or compilation fails.
+bool build_gen_module(mm3e_batch* b, std::string& msg) {
+  b->gen_scenes.clear();
+  b->gen_fn = nullptr;
+  if (b->r->codegen == MM3E_CODEGEN_OFF) return false;
+  for (const auto& j : b->in) {
+    const mm3e_scene* s = j.scene;
+    if (s->host.objects.size() > codegen_object_limit()) continue;
+    if (std::find(b->gen_scenes.begin(), b->gen_scenes.end(), s) == b->gen_scenes.end()) b->gen_scenes.push_back(s);
+  }
+  if (b->gen_scenes.empty()) return false;
+  std::vector<Blob> blobs(b->gen_scenes.size());
+  std::vector<const Blob*> maps;
+  for (size_t k = 0; k < b->gen_scenes.size(); k++) {
+    const mm3e_scene* s = b->gen_scenes[k];
+    build_blob(s->host, {}, {}, s->aa, s->bounces, (uint64_t)(uintptr_t)s->d_tables, s->d_gi, false, blobs[k]);
+    maps.push_back(&blobs[k]);
+  }
+  const std::string src = emit_batch_source(maps, b->maxb, b->lds_bytes > 0);
+  if (!rtc_function(src, b->r->device, "mm3e_gen_batch", &b->gen_fn, msg)) {
+    b->gen_scenes.clear();
+    b->gen_fn = nullptr;
+    return false;
+  }
+  return true;
 }
 
 }  // namespace
@@ -468,6 +518,39 @@ int mm3e_batch_autotune(mm3e_batch* b, uint32_t reps, char* report, size_t repor
   b->blocks_per_cu = best_bpc;
   b->tile_aa1[0] = best_t[0];
   b->tile_aa1[1] = best_t[1];
+  // Generated batch module: the scenes' own maps inside the persistent loop.  Opt-in
+  // (MM3E_BATCH_GEN=auto: keep it when faster; =1: keep it whenever it builds): measured on
+  // MI355X it is not faster than the table walk in the loop (a 7-job test batch 19.4 against
+  // 18.1-18.5 ms; the 64-job c3 load unchanged) and its hipRTC build of several scene maps in one
+  // module takes over a minute.
+  const char* gen_env = std::getenv("MM3E_BATCH_GEN");
+  const bool gen_force = gen_env && std::strcmp(gen_env, "1") == 0;
+  const bool gen_skip = !(gen_force || (gen_env && std::strcmp(gen_env, "auto") == 0));
+  b->use_gen = false;
+  build_tables(b);
+  std::string gmsg;
+  if (!gen_skip && build_gen_module(b, gmsg)) {
+    build_tables(b);  // jobs now carry their map index
+    if (!upload(b, msg)) return MM3E_ERR_DEVICE;
+    b->use_gen = true;
+    if (!enqueue(b, msg)) return MM3E_ERR_DEVICE;  // warm-up
+    hipEventRecord(b->ev0, st);
+    for (uint32_t k = 0; k < reps; k++)
+      if (!enqueue(b, msg)) return MM3E_ERR_DEVICE;
+    hipEventRecord(b->ev1, st);
+    if (hipEventSynchronize(b->ev1) != hipSuccess) return MM3E_ERR_DEVICE;
+    float gen_ms = 0.0f;
+    hipEventElapsedTime(&gen_ms, b->ev0, b->ev1);
+    gen_ms /= (float)reps;
+    char line[160];
+    std::snprintf(line, sizeof line, "generated maps of %zu scenes, tile %ux%u blocks/CU %u: %.3f ms\n",
+                  b->gen_scenes.size(), best_t[0], best_t[1], best_bpc, gen_ms);
+    rep += line;
+    if (gen_ms < best || gen_force) best = gen_ms;
+    else b->use_gen = false;
+  } else if (!gmsg.empty()) {
+    rep += "generated batch module not built: " + gmsg.substr(0, 200) + "\n";
+  }
   // mixed mode (generated kernels as their own launches on several streams) when any job's
   // scene has one; kept if faster than the best persistent-only configuration
   bool any_gen = false;
@@ -497,7 +580,8 @@ int mm3e_batch_autotune(mm3e_batch* b, uint32_t reps, char* report, size_t repor
   if (b->mixed)
     std::snprintf(line, sizeof line, "chosen: mixed (%.3f ms)\n", mixed_ms);
   else
-    std::snprintf(line, sizeof line, "chosen: tile %ux%u blocks/CU %u (%.3f ms)\n", best_t[0], best_t[1], best_bpc, best);
+    std::snprintf(line, sizeof line, "chosen: %stile %ux%u blocks/CU %u (%.3f ms)\n",
+                  b->use_gen ? "generated maps, " : "", best_t[0], best_t[1], best_bpc, best);
   rep += line;
   set_err(report, report_len, rep);
   return MM3E_OK;

@@ -15,64 +15,24 @@
 
 using namespace mm3e;
 
-// With q.order, queue position k holds tile q.order[k] (most expensive first, from the previous
-// replay's costs) and each tile's cost (tickets drawn meanwhile from q.ticket) goes to q.cost;
-// without, tiles are taken in index order.
-template <bool STAGED>
-__device__ __forceinline__ void batch_render_body(const BatchJob* __restrict__ jobs, uint32_t n_jobs,
-                                                  uint32_t total_tiles, uint32_t* __restrict__ counter,
-                                                  const TileOrder& q, uint4* tables_lds) {
-  __shared__ float smem[256 * 3];
-  __shared__ uint32_t s_tile, s_start;
-  const DevScene* staged = nullptr;
-  for (;;) {
-    if (threadIdx.x == 0) {
-      const uint32_t k = atomicAdd(counter, 1u);
-      s_tile = (k < total_tiles && q.order) ? q.order[k] : k;
-      if (q.order) s_start = atomicAdd(q.ticket, 1u);
-    }
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    __syncthreads();  // every lane has read s_tile before lane 0 may overwrite it
-    if (tile >= total_tiles) break;  // uniform: the whole block leaves together
-    const uint32_t j = find_job(jobs, n_jobs, tile, [](const BatchJob& b) { return b.tile_base; });
-    const BatchJob& J = jobs[j];
-    const DevScene* sc = J.sc;
-    const uint32_t local = tile - J.tile_base;
-    const uint32_t tx = local % J.tiles_x, ty = local / J.tiles_x;
-    if constexpr (STAGED) {
-      if (sc != staged) {
-        // all lanes are past the previous tile's last LDS read (the loop-top barriers)
-        const uint32_t n16 = J.stage_bytes / 16;
-        const uint4* src = reinterpret_cast<const uint4*>(sc->obj);
-        for (uint32_t i = threadIdx.x; i < n16; i += blockDim.x) tables_lds[i] = src[i];
-        __syncthreads();
-        staged = sc;
-      }
-      const char* b = reinterpret_cast<const char*>(tables_lds);
-      const Tables tb{reinterpret_cast<const DObj*>(b), reinterpret_cast<const DSeg*>(b + sc->st_seg),
-                      reinterpret_cast<const DNode*>(b + sc->st_node),
-                      reinterpret_cast<const uint32_t*>(b + sc->st_unb)};
-      render_tile<TableMap, MM3E_MAXB, true>(sc, tb, J.fp, J.hdr, J.tw, J.th, tx, ty, smem);
-    } else {
-      render_tile<TableMap, MM3E_MAXB, true>(sc, global_tables(sc), J.fp, J.hdr, J.tw, J.th, tx, ty, smem);
-    }
-    __syncthreads();  // smem is reused by the next tile
-    if (q.order && threadIdx.x == 0) q.cost[tile] = atomicAdd(q.ticket, 1u) - s_start;
+// The table-walk tile render of a batch job (general fold, state machine).
+struct TableTile {
+  MM3E_DEV void operator()(const BatchJob& J, const Tables& tb, uint32_t tx, uint32_t ty, float* smem) const {
+    render_tile<TableMap, MM3E_MAXB, true>(J.sc, tb, J.fp, J.hdr, J.tw, J.th, tx, ty, smem);
   }
-}
+};
 
 extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
     MM3E_CAT(mm3e_batch_render_lds_b, MM3E_MAXB)(const BatchJob* __restrict__ jobs, uint32_t n_jobs,
                                                  uint32_t total_tiles, uint32_t* __restrict__ counter, TileOrder q) {
   extern __shared__ uint4 tables_lds[];
-  batch_render_body<true>(jobs, n_jobs, total_tiles, counter, q, tables_lds);
+  batch_loop<true>(jobs, n_jobs, total_tiles, counter, q, tables_lds, TableTile{});
 }
 
 extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
     MM3E_CAT(mm3e_batch_render_table_b, MM3E_MAXB)(const BatchJob* __restrict__ jobs, uint32_t n_jobs,
                                                    uint32_t total_tiles, uint32_t* __restrict__ counter, TileOrder q) {
-  batch_render_body<false>(jobs, n_jobs, total_tiles, counter, q, nullptr);
+  batch_loop<false>(jobs, n_jobs, total_tiles, counter, q, nullptr, TableTile{});
 }
 
 namespace mm3e {

@@ -58,52 +58,15 @@ void put_obj(std::string& o, const DObj& d) {
 
 bool has_csg(const Blob& b) { return b.header.csg != nullptr; }
 
-}  // namespace
-
-bool generated_needs_tables(const Blob& b) {
-  for (const DSeg& s : b.segs)
-    if (s.kind == 1) return true;
-  return false;
-}
-
-// The state machine above 24 objects, nested loops below, with or without a segment grid:
-// 1280x720 procedural scenes of 13 / 21 / 41 objects take 2.12 / 3.74 / 7.08 ms nested (6
-// waves) against 2.08 / 5.40 / 6.32 ms as a state machine (4 waves); the nested module takes
-// 15-24 s to compile instead of 4.  MM3E_GEN_SM=0/1 overrides.
-bool generated_prefers_sm(const Blob& b) {
-  if (const char* v = std::getenv("MM3E_GEN_SM")) return std::strcmp(v, "1") == 0;
-  return b.objs.size() > 24;
-}
-
-std::string emit_scene_source(const Blob& b, uint32_t maxb, bool sm, bool queued, bool film) {
+// The scene's world fold as a map functor `name` (TableMap's interface) inside namespace mm3e.
+void emit_map(std::string& o, const Blob& b, const std::string& name) {
   const char* csg = has_csg(b) ? "true" : "false";
-  std::string o;
-  o.reserve(4096 + b.objs.size() * 900);
-  o += "// generated: world fold of one compiled scene (objects: " + std::to_string(b.objs.size()) +
-       ", segments: " + std::to_string(b.segs.size()) + ")\n";
-  // Waves per SIMD the generated kernel is register-allocated for.  Nested path: 6 (at most 80
-  // VGPRs); it is bound by dependency latency more than by issue (c2 960x540 at 4 / 5 / 6 / 7
-  // waves: 1.59 / 1.38 / 1.32 / 1.34 ms; c1 and a CSG scene at 1280x720 also fastest at 6).
-  // State-machine path: 4 (at most 128 VGPRs) -- its per-lane path state spills in the hot loop
-  // below that (1280x720 procedural scenes of 13 / 21 / 41 objects at 4 / 5 / 6 waves:
-  // 2.08 / 4.06 / 6.83, 5.40 / 6.38 / 7.61, 6.37 / 8.15 / 10.05 ms) -- except for long folds,
-  // where latency hiding wins again (61 objects: 22.2 / 20.6 / 20.4 ms): 5 above 48 objects.
-  // MM3E_GEN_WAVES overrides.
-  int waves = sm ? (b.objs.size() > 48 ? 5 : 4) : 6;
-  if (const char* wv = std::getenv("MM3E_GEN_WAVES")) {
-    const int n = std::atoi(wv);
-    if (n >= 1 && n <= 8) waves = n;
-  }
-  o += "#define MM3E_RENDER_WAVES " + std::to_string(waves) + "\n";
-  o += "#ifndef __HIPCC_RTC__\n#include <hip/hip_runtime.h>\n#endif\n#include \"dev_render.h\"\n";
-  if (queued) o += "#include \"dev_wavefront.h\"\n";
-  o += "namespace mm3e {\n";
-  o += "struct GenMap {\n  const DevScene* sc;\n  Tables tb;\n  MM3E_DEV Field operator()(V3 p) const;\n};\n";
+  o += "struct " + name + " {\n  const DevScene* sc;\n  Tables tb;\n  MM3E_DEV Field operator()(V3 p) const;\n};\n";
   // The fold is the loop of world_eval with the per-segment body specialised: a uniform switch
   // on the segment index holds each One / linear-run segment's code once; tree runs share the
   // table walk in the default case.  The stepping (every segment, or the set bits of the
   // wave's grid masks) is world_eval's.
-  o += "MM3E_DEV Field GenMap::operator()(V3 p) const {\n  const DevScene& S = *sc;\n";
+  o += "MM3E_DEV Field " + name + "::operator()(V3 p) const {\n  const DevScene& S = *sc;\n";
   o += "  Field acc{__builtin_inff(), 0u};\n  bool seeded = false;\n";
   const size_t n = b.segs.size();
   if (n == 0) {
@@ -151,6 +114,48 @@ std::string emit_scene_source(const Blob& b, uint32_t maxb, bool sm, bool queued
     }
     o += "    if (++si >= nseg) return acc;\n  }\n}\n";
   }
+}
+
+}  // namespace
+
+bool generated_needs_tables(const Blob& b) {
+  for (const DSeg& s : b.segs)
+    if (s.kind == 1) return true;
+  return false;
+}
+
+// The state machine above 24 objects, nested loops below, with or without a segment grid:
+// 1280x720 procedural scenes of 13 / 21 / 41 objects take 2.12 / 3.74 / 7.08 ms nested (6
+// waves) against 2.08 / 5.40 / 6.32 ms as a state machine (4 waves); the nested module takes
+// 15-24 s to compile instead of 4.  MM3E_GEN_SM=0/1 overrides.
+bool generated_prefers_sm(const Blob& b) {
+  if (const char* v = std::getenv("MM3E_GEN_SM")) return std::strcmp(v, "1") == 0;
+  return b.objs.size() > 24;
+}
+
+std::string emit_scene_source(const Blob& b, uint32_t maxb, bool sm, bool queued, bool film) {
+  std::string o;
+  o.reserve(4096 + b.objs.size() * 900);
+  o += "// generated: world fold of one compiled scene (objects: " + std::to_string(b.objs.size()) +
+       ", segments: " + std::to_string(b.segs.size()) + ")\n";
+  // Waves per SIMD the generated kernel is register-allocated for.  Nested path: 6 (at most 80
+  // VGPRs); it is bound by dependency latency more than by issue (c2 960x540 at 4 / 5 / 6 / 7
+  // waves: 1.59 / 1.38 / 1.32 / 1.34 ms; c1 and a CSG scene at 1280x720 also fastest at 6).
+  // State-machine path: 4 (at most 128 VGPRs) -- its per-lane path state spills in the hot loop
+  // below that (1280x720 procedural scenes of 13 / 21 / 41 objects at 4 / 5 / 6 waves:
+  // 2.08 / 4.06 / 6.83, 5.40 / 6.38 / 7.61, 6.37 / 8.15 / 10.05 ms) -- except for long folds,
+  // where latency hiding wins again (61 objects: 22.2 / 20.6 / 20.4 ms): 5 above 48 objects.
+  // MM3E_GEN_WAVES overrides.
+  int waves = sm ? (b.objs.size() > 48 ? 5 : 4) : 6;
+  if (const char* wv = std::getenv("MM3E_GEN_WAVES")) {
+    const int n = std::atoi(wv);
+    if (n >= 1 && n <= 8) waves = n;
+  }
+  o += "#define MM3E_RENDER_WAVES " + std::to_string(waves) + "\n";
+  o += "#ifndef __HIPCC_RTC__\n#include <hip/hip_runtime.h>\n#endif\n#include \"dev_render.h\"\n";
+  if (queued) o += "#include \"dev_wavefront.h\"\n";
+  o += "namespace mm3e {\n";
+  emit_map(o, b, "GenMap");
   o += "}  // namespace mm3e\n";
   o += "using namespace mm3e;\n";
   o += "extern \"C\" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY\n"
@@ -199,6 +204,32 @@ std::string emit_scene_source(const Blob& b, uint32_t maxb, bool sm, bool queued
          "    mm3e_gen_wf_shade(const DevScene* __restrict__ sc, FrameParams fp, WfFrame w, uint32_t stage_bytes) {\n" +
          tbl + "  wf_shade<GenMap, " + std::to_string(maxb) + ", " + (sm ? "true" : "false") + ">(sc, tb, fp, w);\n}\n";
   }
+  return o;
+}
+
+std::string emit_batch_source(const std::vector<const Blob*>& maps, uint32_t maxb, bool staged) {
+  std::string o;
+  o += "// generated: batch module with " + std::to_string(maps.size()) + " scene maps\n";
+  // the table kernels' occupancy: every map runs the state machine, as the table walk does
+  o += "#define MM3E_RENDER_WAVES 4\n";
+  o += "#ifndef __HIPCC_RTC__\n#include <hip/hip_runtime.h>\n#endif\n#include \"dev_batch.h\"\nnamespace mm3e {\n";
+  for (size_t k = 0; k < maps.size(); k++) emit_map(o, *maps[k], "GenMap" + std::to_string(k));
+  o += "}  // namespace mm3e\nusing namespace mm3e;\n";
+  const std::string mb = std::to_string(maxb);
+  o += "struct GenTiles {\n"
+       "  MM3E_DEV void operator()(const BatchJob& J, const Tables& tb, uint32_t tx, uint32_t ty, float* smem) const {\n"
+       "    switch (J.gen) {\n";
+  for (size_t k = 0; k < maps.size(); k++)
+    o += "      case " + std::to_string(k) + "u: render_tile<GenMap" + std::to_string(k) + ", " + mb +
+         ", true>(J.sc, tb, J.fp, J.hdr, J.tw, J.th, tx, ty, smem); return;\n";
+  o += "      default: render_tile<TableMap, " + mb + ", true>(J.sc, tb, J.fp, J.hdr, J.tw, J.th, tx, ty, smem);\n"
+       "    }\n  }\n};\n";
+  o += "extern \"C\" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY\n"
+       "    mm3e_gen_batch(const BatchJob* __restrict__ jobs, uint32_t n_jobs, uint32_t total_tiles,\n"
+       "                   uint32_t* __restrict__ counter, TileOrder q) {\n"
+       "  extern __shared__ uint4 tables_lds[];\n";
+  o += std::string("  batch_loop<") + (staged ? "true" : "false") +
+       ">(jobs, n_jobs, total_tiles, counter, q, tables_lds, GenTiles{});\n}\n";
   return o;
 }
 

@@ -5,6 +5,7 @@
 #pragma once
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "scene_compile.h"
 
@@ -15,6 +16,11 @@ namespace mm3e {
 // the stages of the queued frame (mm3e_gen_wf_first / _cont / _shade, dev_wavefront.h); film
 // adds the film and G-buffer kernels (mm3e_gen_film, mm3e_gen_gbuffer).
 std::string emit_scene_source(const Blob& b, uint32_t maxb, bool sm, bool queued = false, bool film = false);
+
+// Source of a batch module: each listed scene's map as GenMap<k> and the persistent batch kernel
+// `mm3e_gen_batch` (dev_batch.h's loop; a job with gen == k renders with GenMap<k>, any other
+// with the table walk), for reflection depth maxb, tables staged in LDS when staged.
+std::string emit_batch_source(const std::vector<const Blob*>& maps, uint32_t maxb, bool staged);
 
 // Whether the generated kernel reads the object tables (union runs walked through their
 // bounding-sphere tree still use the table walk).

@@ -70,35 +70,46 @@ bool compile(const std::string& src, std::vector<char>& code, std::string& err) 
 
 }  // namespace
 
-bool rtc_kernel(const std::string& src, int device, hipFunction_t* fn, hipFunction_t* fn_o, std::string& err,
-                hipFunction_t* fn_opt) {
-  std::shared_ptr<Entry> e;
-  {
-    std::lock_guard<std::mutex> lk(g_mu);
-    auto& slot = g_cache[{device, src}];
-    if (!slot) slot = std::make_shared<Entry>();
-    e = slot;
+namespace {
+
+std::shared_ptr<Entry> entry_for(const std::string& src, int device) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto& slot = g_cache[{device, src}];
+  if (!slot) slot = std::make_shared<Entry>();
+  return slot;
+}
+
+// Compile and load (e.mu held); a scene module's kernels are looked up once here.
+void build(Entry& e, const std::string& src) {
+  std::vector<char> code;
+  e.ok = compile(src, code, e.err);
+  if (e.ok && hipModuleLoadData(&e.mod, code.data()) != hipSuccess) {
+    e.ok = false;
+    e.err = "hipModuleLoadData of the generated scene kernel failed";
   }
-  std::lock_guard<std::mutex> lk(e->mu);
-  if (!e->done) {
-    std::vector<char> code;
-    e->ok = compile(src, code, e->err);
-    if (e->ok && hipModuleLoadData(&e->mod, code.data()) != hipSuccess) {
-      e->ok = false;
-      e->err = "hipModuleLoadData of the generated scene kernel failed";
-    }
-    if (e->ok && hipModuleGetFunction(&e->fn, e->mod, "mm3e_gen_render") != hipSuccess) {
-      e->ok = false;
-      e->err = "generated module has no mm3e_gen_render";
-    }
-    if (e->ok && hipModuleGetFunction(&e->fn_o, e->mod, "mm3e_gen_render_o") != hipSuccess) e->fn_o = nullptr;
+  if (e.ok && src.find("mm3e_gen_render(") != std::string::npos) {
+    if (hipModuleGetFunction(&e.fn, e.mod, "mm3e_gen_render") != hipSuccess) e.fn = nullptr;
+    if (hipModuleGetFunction(&e.fn_o, e.mod, "mm3e_gen_render_o") != hipSuccess) e.fn_o = nullptr;
     static const char* opt_names[GEN_N_OPT] = {"mm3e_gen_wf_first", "mm3e_gen_wf_cont", "mm3e_gen_wf_shade",
                                                "mm3e_gen_film", "mm3e_gen_gbuffer"};
-    for (int k = 0; k < GEN_N_OPT && e->ok; k++)
+    for (int k = 0; k < GEN_N_OPT; k++)
       if (src.find(opt_names[k]) == std::string::npos ||
-          hipModuleGetFunction(&e->fn_opt[k], e->mod, opt_names[k]) != hipSuccess)
-        e->fn_opt[k] = nullptr;
-    e->done = true;
+          hipModuleGetFunction(&e.fn_opt[k], e.mod, opt_names[k]) != hipSuccess)
+        e.fn_opt[k] = nullptr;
+  }
+  e.done = true;
+}
+
+}  // namespace
+
+bool rtc_kernel(const std::string& src, int device, hipFunction_t* fn, hipFunction_t* fn_o, std::string& err,
+                hipFunction_t* fn_opt) {
+  std::shared_ptr<Entry> e = entry_for(src, device);
+  std::lock_guard<std::mutex> lk(e->mu);
+  if (!e->done) build(*e, src);
+  if (e->ok && !e->fn) {
+    e->ok = false;
+    e->err = "generated module has no mm3e_gen_render";
   }
   if (!e->ok) {
     err = e->err;
@@ -108,6 +119,21 @@ bool rtc_kernel(const std::string& src, int device, hipFunction_t* fn, hipFuncti
   if (fn_o) *fn_o = e->fn_o;
   if (fn_opt)
     for (int k = 0; k < GEN_N_OPT; k++) fn_opt[k] = e->fn_opt[k];
+  return true;
+}
+
+bool rtc_function(const std::string& src, int device, const char* name, hipFunction_t* fn, std::string& err) {
+  std::shared_ptr<Entry> e = entry_for(src, device);
+  std::lock_guard<std::mutex> lk(e->mu);
+  if (!e->done) build(*e, src);
+  if (!e->ok) {
+    err = e->err;
+    return false;
+  }
+  if (hipModuleGetFunction(fn, e->mod, name) != hipSuccess) {
+    err = std::string("generated module has no ") + name;
+    return false;
+  }
   return true;
 }
 

@@ -19,5 +19,7 @@ enum GenOpt : int { GEN_WF_FIRST = 0, GEN_WF_CONT, GEN_WF_SHADE, GEN_FILM, GEN_G
 // once.
 bool rtc_kernel(const std::string& src, int device, hipFunction_t* fn, hipFunction_t* fn_o, std::string& err,
                 hipFunction_t* fn_opt = nullptr);
+// Compile (or find in the cache) a generated module and return its kernel `name`.
+bool rtc_function(const std::string& src, int device, const char* name, hipFunction_t* fn, std::string& err);
 
 }  // namespace mm3e

@@ -227,7 +227,7 @@ static void gen_worker(mm3e_renderer* r) {
 // generated fold is 1.24x faster at 40 objects, 1.26x at 60, 1.09x at 80, 0.95x at 100 and
 // 0.93x at 130 (its code no longer fits the instruction cache, while the table walk's does).
 static const size_t kCodegenMaxObjects = 80;
-static size_t codegen_max_objects() {  // MM3E_CODEGEN_MAX_OBJECTS overrides (tuning)
+size_t codegen_object_limit() {  // MM3E_CODEGEN_MAX_OBJECTS overrides (tuning)
   if (const char* v = std::getenv("MM3E_CODEGEN_MAX_OBJECTS")) {
     const long n = std::atol(v);
     if (n >= 0) return (size_t)n;
@@ -236,7 +236,7 @@ static size_t codegen_max_objects() {  // MM3E_CODEGEN_MAX_OBJECTS overrides (tu
 }
 static void gen_schedule(mm3e_scene* s) {
   mm3e_renderer* r = s->r;
-  if (s->host.objects.size() > codegen_max_objects()) return;
+  if (s->host.objects.size() > codegen_object_limit()) return;
   Blob b;
   build_blob(s->host, {}, {}, s->aa, s->bounces, (uint64_t)(uintptr_t)s->d_tables, s->d_gi, false, b);
   auto j = std::make_shared<GenJob>();

@@ -103,6 +103,8 @@ void set_err(char* err, size_t n, const std::string& s);
 bool hip_ok(hipError_t e, const char* what, std::string* msg);
 FrameParams make_frame(const mm3e_camera& c, uint32_t w, uint32_t h);
 void tile_shape(uint32_t aa, uint32_t* tw, uint32_t* th);
+// Largest scene (objects) that gets a generated kernel.
+size_t codegen_object_limit();
 // The scene's generated optional kernel `k` (GenOpt order) when its module is ready, else null.
 hipFunction_t gen_opt(const mm3e_scene* s, int k);
 // G-buffer pass of a scene (its generated kernel when ready, else the table kernels): the

@@ -475,6 +475,44 @@ def test_mixed_batch_with_generated_kernels_equals_per_frame_renders():
     r.close()
 
 
+def test_generated_batch_module_equals_per_frame_renders(monkeypatch):
+    """The batch module holding the generated maps of the batch's scenes (40-object and c2
+    scenes; the 90-object one keeps the table walk inside the same loop), forced on through
+    MM3E_BATCH_GEN=1: every job equals the per-frame render bit for bit, across camera changes."""
+    monkeypatch.setenv("MM3E_BATCH_GEN", "1")
+    r = hip.Renderer()
+    r.set_codegen(hip.CODEGEN_SYNC)
+    scenes = [configs.procedural_scene(61, 40, 320, 180), configs.procedural_scene(62, 90, 320, 180),
+              configs.c2_scene(width=160, height=90, gi_dims=(4, 4, 4), gi_rays=5)[0]]
+    compiled = [r.compile(sc.to_ir(), sc.width, sc.height) for sc in scenes]
+    rng = configs.SplitMix64(79)
+    sizes = [(320, 180), (200, 120), (96, 54), (320, 180), (160, 90), (130, 77), (256, 144)]
+    jobs = [(compiled[i % 3], configs.orbit_pose(rng), wh) for i, wh in enumerate(sizes)]
+    b = hip.Batch(r, jobs)
+    report = b.autotune(reps=1)
+    print("\n" + report, flush=True)
+    assert "generated maps of 2 scenes" in report
+    plain = hip.Renderer()
+    plain.set_codegen(hip.CODEGEN_OFF)
+    for rep in range(2):
+        if rep:
+            for i in range(len(jobs)):
+                cam = configs.orbit_pose(rng)
+                jobs[i] = (jobs[i][0], cam, jobs[i][2])
+                b.set_camera(i, cam)
+        b.launch()
+        b.wait()
+        for i, (g, cam, (w, h)) in enumerate(jobs):
+            ref = plain.compile(scenes[i % 3].to_ir(), w, h)
+            assert (b.read(i) == ref.render_rgba(cam)).all(), f"replay {rep} job {i}"
+            ref.close()
+    b.close()
+    for g in compiled:
+        g.close()
+    plain.close()
+    r.close()
+
+
 @pytest.mark.parametrize("preset", ["fast", "balanced"])
 def test_quality_presets_parity(renderer, preset):
     """Quality::fast / balanced (lib.rs:82-107) written into a procedural scene: LOD footprint
