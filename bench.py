@@ -24,8 +24,9 @@ More legs ride on the same line (``--legs`` picks them):
   served:  requests over 32 resident scenes (16^3 GI probes each; 854x480 50 %, 1280x720 35 %,
            1920x1080 15 %) arriving as a seeded Poisson stream at offered rates set from the
            plain path's measured capacity, plus a burst; p50/p99 latency (arrival -> RGBA8 in
-           host memory) and Mpix/s for the continuous-batching server and for a FIFO server
-           that renders each request by a plain per-frame launch.
+           host memory) and Mpix/s for the continuous-batching server (8 batches of at most 2
+           requests in flight) and for a FIFO server that renders each request by a plain
+           per-frame launch.
   mesh_bake: the reference mesh demo's torus knot (240 x 36 ring vertices, 17,280 triangles)
            imported from OBJ text and baked to a signed-distance grid with 128 samples along the
            longest axis (124x63x128) on the device; ms per bake (row-parity + nearest-triangle
@@ -137,7 +138,7 @@ def batched_leg(hip, configs, r, reps):
     return out
 
 
-def served_leg(hip, configs, r, n_req, n_scenes=32):
+def served_leg(hip, configs, r, n_req, n_scenes=32, inflight=8, batch=2):
     """Poisson request stream: continuous-batching server vs a FIFO server of plain launches."""
     import numpy as np
     t0 = time.perf_counter()
@@ -170,7 +171,7 @@ def served_leg(hip, configs, r, n_req, n_scenes=32):
         span = time.perf_counter() - t_start
         return lat, span
 
-    srv = hip.Server(r, hbm_budget_bytes=8 << 30, max_batch_requests=32, max_inflight=2)
+    srv = hip.Server(r, hbm_budget_bytes=8 << 30, max_batch_requests=batch, max_inflight=inflight)
     ids = [srv.add_scene(g) for g in base]
 
     def served_run(arrivals):
@@ -419,6 +420,8 @@ def main():
     ap.add_argument("--stress-frames", type=int, default=3)
     ap.add_argument("--batched-reps", type=int, default=3)
     ap.add_argument("--served-requests", type=int, default=48)
+    ap.add_argument("--served-inflight", type=int, default=8, help="server batches in flight (1..8)")
+    ap.add_argument("--served-batch", type=int, default=2, help="requests per server batch at most")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 256 MiB device copy (torch), used to scale rocprof "
                          "FETCH_SIZE/WRITE_SIZE readings to bytes")
@@ -512,7 +515,8 @@ def main():
     if "batched" in legs:
         out["batched"] = batched_leg(hip, configs, r, args.batched_reps)
     if "served" in legs:
-        out["served"] = served_leg(hip, configs, r, args.served_requests)
+        out["served"] = served_leg(hip, configs, r, args.served_requests, inflight=args.served_inflight,
+                                   batch=args.served_batch)
     if "mesh_bake" in legs:
         out["mesh_bake"] = mesh_bake_leg(hip, configs, r, args.bake_reps, cpu=not args.no_cpu)
     if "frame_gen" in legs:

@@ -357,8 +357,8 @@ int mm3e_render_batch(mm3e_renderer* r, const mm3e_batch_job* jobs, uint32_t n_j
 typedef struct mm3e_server mm3e_server;
 typedef struct mm3e_server_config {
   uint64_t hbm_budget_bytes;    /* framebuffer slab (default 4 GiB) */
-  uint32_t max_batch_requests;  /* requests per batch (default 64) */
-  uint32_t max_inflight;        /* batches on the device at once, 1..8 (default 2) */
+  uint32_t max_batch_requests;  /* requests per batch (default 4) */
+  uint32_t max_inflight;        /* batches on the device at once, 1..8 (default 8) */
   uint32_t queue_capacity;      /* pending requests before submit returns MM3E_ERR_BUSY (4096) */
   uint32_t blocks_per_cu;       /* persistent blocks per CU (default 4) */
 } mm3e_server_config;

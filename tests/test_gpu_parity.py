@@ -394,12 +394,15 @@ def test_batch_equals_per_frame_renders_across_replays(renderer):
     b.close()
 
 
-def test_server_results_equal_alone_and_in_mix(renderer):
+@pytest.mark.parametrize("shape", [(8, 2), None])
+def test_server_results_equal_alone_and_in_mix(renderer, shape):
     """Continuous batching: requests submitted in a burst (so they share batches and overlap
-    on the device) return exactly the per-frame render of the same request alone."""
+    on the device) return exactly the per-frame render of the same request alone; with
+    (requests per batch, batches in flight) = (8, 2) and with the defaults (4, 8)."""
     scenes = [configs.procedural_scene(61 + i, 30 + 20 * i, 200, 120) for i in range(3)]
     compiled = [renderer.compile(sc.to_ir(), 200, 120) for sc in scenes]
-    srv = hip.Server(renderer, hbm_budget_bytes=256 << 20, max_batch_requests=8, max_inflight=2)
+    cfg = {} if shape is None else {"max_batch_requests": shape[0], "max_inflight": shape[1]}
+    srv = hip.Server(renderer, hbm_budget_bytes=256 << 20, **cfg)
     ids = [srv.add_scene(g) for g in compiled]
     rng = configs.SplitMix64(99)
     sizes = [(200, 120), (96, 64), (160, 90), (200, 120), (64, 48), (128, 72)]
