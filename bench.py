@@ -215,6 +215,7 @@ def served_leg(hip, configs, r, n_req, n_scenes=32, inflight=8, batch=2):
                        "16^3 GI probes baked on device), resolution 854x480 50 % / 1280x720 35 % / 1920x1080 15 %, "
                        "seeded orbit poses; latency = arrival -> RGBA8 in host memory",
            "requests": len(reqs), "mpix_requested": round(px / 1e6, 3),
+           "server": {"max_inflight": inflight, "max_batch_requests": batch},
            "burst": {"plain": {"mpix_s": round(px / plain_span / 1e6, 3), "p50_ms": round(pct(plain_lat, 50), 2),
                                "p99_ms": round(pct(plain_lat, 99), 2)},
                      "served": {"mpix_s": round(px / served_span / 1e6, 3),
