@@ -1137,26 +1137,30 @@ static void blur_axis(const std::vector<V3<float>>& src, uint32_t w, uint32_t h,
     }
   }
 }
+// post.rs:41-46 + bloom_pass (post.rs:61-75): color += blur(bright-pass(color)) * intensity.
+static void bloom_add(std::vector<V3<float>>& color, uint32_t w, uint32_t h, float threshold, float intensity,
+                      uint32_t radius) {
+  size_t n = (size_t)w * h;
+  std::vector<V3<float>> bright(n);
+  for (size_t i = 0; i < n; i++) {
+    float l = luminance(color[i]);
+    if (l > threshold) bright[i] = scale(color[i], (l - threshold) / std::fmax(l, 1e-4f));
+  }
+  std::vector<float> k = gaussian_kernel(radius);
+  std::vector<V3<float>> hz, bl;
+  if (w > 0 && h > 0) {
+    blur_axis(bright, w, h, k, true, hz);
+    blur_axis(hz, w, h, k, false, bl);
+  } else {
+    bl.assign(n, V3<float>());
+  }
+  for (size_t i = 0; i < n; i++) color[i] = color[i] + scale(bl[i], intensity);
+}
 static void resolve(const Scene& sc, const float* hdr, uint32_t w, uint32_t h, uint8_t* rgba) {
   size_t n = (size_t)w * h;
   std::vector<V3<float>> color(n);
   for (size_t i = 0; i < n; i++) color[i] = V3<float>(hdr[3 * i], hdr[3 * i + 1], hdr[3 * i + 2]);
-  if (sc.bloom && sc.bloom_intensity > 0.0f) {
-    std::vector<V3<float>> bright(n);
-    for (size_t i = 0; i < n; i++) {
-      float l = luminance(color[i]);
-      if (l > sc.bloom_threshold) bright[i] = scale(color[i], (l - sc.bloom_threshold) / std::fmax(l, 1e-4f));
-    }
-    std::vector<float> k = gaussian_kernel(sc.bloom_radius);
-    std::vector<V3<float>> hz, bl;
-    if (w > 0 && h > 0) {
-      blur_axis(bright, w, h, k, true, hz);
-      blur_axis(hz, w, h, k, false, bl);
-    } else {
-      bl.assign(n, V3<float>());
-    }
-    for (size_t i = 0; i < n; i++) color[i] = color[i] + scale(bl[i], sc.bloom_intensity);
-  }
+  if (sc.bloom && sc.bloom_intensity > 0.0f) bloom_add(color, w, h, sc.bloom_threshold, sc.bloom_intensity, sc.bloom_radius);
   for (size_t i = 0; i < n; i++) {
     V3<float> c = clamp01(gamma(aces(scale(color[i], sc.exposure))));  // Rgba::from_vec3 clamps
     // framebuffer.rs:102-113 with an opaque pixel: to_u8 = (c*255 + 0.5) as u8
@@ -1870,6 +1874,40 @@ void orc_camera_look_at(const float* eye, const float* target, const float* up, 
   V3<float> u = cross(right, fwd);
   float c[13] = {e.x, e.y, e.z, fwd.x, fwd.y, fwd.z, right.x, right.y, right.z, u.x, u.y, u.z, tanf(fov_y * 0.5f)};
   std::memcpy(cam13, c, sizeof c);
+}
+// The HDR image after the bloom add of post::resolve (post.rs:41-46), before exposure and
+// tone mapping: hdr and out are w*h*3 floats.
+void orc_bloom(const float* hdr, uint32_t w, uint32_t h, float threshold, float intensity, uint32_t radius, float* out) {
+  size_t n = (size_t)w * h;
+  std::vector<V3<float>> color(n);
+  for (size_t i = 0; i < n; i++) color[i] = V3<float>(hdr[3 * i], hdr[3 * i + 1], hdr[3 * i + 2]);
+  bloom_add(color, w, h, threshold, intensity, radius);
+  for (size_t i = 0; i < n; i++) { out[3 * i] = color[i].x; out[3 * i + 1] = color[i].y; out[3 * i + 2] = color[i].z; }
+}
+// Scene::ibl_irradiance (lib.rs:1379-1398) of the scene's sky for a unit normal n.
+void orc_ibl_irradiance(const orc_scene* s, const float* n, float* out3) {
+  Renderer<float> rd(&s->sc);
+  V3<float> r = rd.ibl_irradiance(f3<float>(n));
+  out3[0] = r.x; out3[1] = r.y; out3[2] = r.z;
+}
+// Light::toward / attenuation / shadow_k (lib.rs:528-556) of one light seen from p:
+// out[0..3] = direction, out[3] = distance, out[4] = attenuation, out[5] = shadow k.
+void orc_light_eval(int directional, const float* vec3, float radius, const float* p, float* out6) {
+  Lgt l{};
+  l.directional = directional != 0;
+  std::memcpy(l.vec, vec3, 12);
+  l.radius = radius;
+  V3<float> d;
+  float dist;
+  light_toward<float>(l, f3<float>(p), d, dist);
+  out6[0] = d.x; out6[1] = d.y; out6[2] = d.z; out6[3] = dist;
+  out6[4] = light_atten<float>(l, dist);
+  out6[5] = light_shadow_k<float>(l, dist);
+}
+// shade::apply_fog (shade.rs:113-116).
+void orc_apply_fog(const float* c, const float* fog, float t, float density, float* out3) {
+  V3<float> r = apply_fog(f3<float>(c), f3<float>(fog), t, density);
+  out3[0] = r.x; out3[1] = r.y; out3[2] = r.z;
 }
 void orc_gaussian_kernel(uint32_t radius, float* out, uint32_t* n) {
   auto k = gaussian_kernel(radius);

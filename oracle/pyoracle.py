@@ -61,6 +61,15 @@ def lib():
         L.orc_volume_sample.restype = C.c_float
         L.orc_camera_look_at.argtypes = [P, P, P, C.c_float, P]
         L.orc_gaussian_kernel.argtypes = [C.c_uint32, P, P]
+        L.orc_gaussian_kernel.restype = None
+        L.orc_bloom.argtypes = [P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_uint32, P]
+        L.orc_bloom.restype = None
+        L.orc_ibl_irradiance.argtypes = [P, P, P]
+        L.orc_ibl_irradiance.restype = None
+        L.orc_light_eval.argtypes = [C.c_int, P, C.c_float, P, P]
+        L.orc_light_eval.restype = None
+        L.orc_apply_fog.argtypes = [P, P, C.c_float, C.c_float, P]
+        L.orc_apply_fog.restype = None
         L.orc_count_rows.argtypes = [P, P, C.c_uint, C.c_uint32, C.c_uint32, P]
         L.orc_phase_count.restype = C.c_int
         L.orc_to_bmp.argtypes = [P, C.c_uint32, C.c_uint32, P]
@@ -231,6 +240,11 @@ class OracleScene:
     def ao(self, p, n):
         return lib().orc_ao(self.h, _p(f32(p)), _p(f32(n)))
 
+    def ibl_irradiance(self, n):
+        out = np.zeros(3, np.float32)
+        lib().orc_ibl_irradiance(self.h, _p(f32(n)), _p(out))
+        return out
+
     def gi_sample(self, p, n):
         out = np.zeros(3, np.float32)
         lib().orc_gi_sample(self.h, _p(f32(p)), _p(f32(n)), _p(out))
@@ -373,3 +387,33 @@ def reproject(g, cam, deltas=(), fill=True):
                         _p(np.ascontiguousarray(g["obj"], np.int32)), _p(f32(cam)), _p(d) if len(d) else None,
                         len(d), 1 if fill else 0, _p(out), _p(filled))
     return out, filled
+
+
+def gaussian_kernel(radius):
+    n = C.c_uint32(0)
+    lib().orc_gaussian_kernel(radius, None, C.byref(n))
+    out = np.zeros(n.value, np.float32)
+    lib().orc_gaussian_kernel(radius, _p(out), C.byref(n))
+    return out
+
+
+def bloom(hdr, threshold, intensity, radius):
+    """post::resolve's HDR after the bloom add (before exposure / tone mapping)."""
+    hdr = f32(hdr)
+    h, w = hdr.shape[:2]
+    out = np.zeros_like(hdr)
+    lib().orc_bloom(_p(hdr), w, h, threshold, intensity, radius, _p(out))
+    return out
+
+
+def light_eval(directional, vec, radius, p):
+    """(direction, distance, attenuation, shadow k) of Light::toward/attenuation/shadow_k."""
+    out = np.zeros(6, np.float32)
+    lib().orc_light_eval(1 if directional else 0, _p(f32(vec)), radius, _p(f32(p)), _p(out))
+    return out[:3], out[3], out[4], out[5]
+
+
+def apply_fog(c, fog, t, density):
+    out = np.zeros(3, np.float32)
+    lib().orc_apply_fog(_p(f32(c)), _p(f32(fog)), t, density, _p(out))
+    return out
