@@ -461,7 +461,7 @@ static void env_wavefront(uint32_t* budget, uint32_t* passes, uint32_t* bpc) {
   if (!wf) return;
   unsigned b = 0, n = *passes, c = *bpc;
   const int got = std::sscanf(wf, "%u,%u,%u", &b, &n, &c);
-  if (got >= 1 && n + 1 <= WF_MAX_PASSES && c <= 64) {
+  if (got >= 1 && n < WF_MAX_PASSES && c <= 64) {
     *budget = b;
     *passes = n;
     *bpc = c;
@@ -1146,7 +1146,7 @@ int mm3e_renderer_set_tile_order(mm3e_renderer* r, int on) {
 }
 
 int mm3e_renderer_set_wavefront(mm3e_renderer* r, uint32_t budget, uint32_t passes, uint32_t blocks_per_cu) {
-  if (!r || passes + 1 > WF_MAX_PASSES || blocks_per_cu > 64) return MM3E_ERR_INVALID;
+  if (!r || passes >= WF_MAX_PASSES || blocks_per_cu > 64) return MM3E_ERR_INVALID;
   std::lock_guard<std::mutex> lk(r->mu);
   r->wf_budget = budget;
   r->wf_passes = passes;
@@ -1155,7 +1155,7 @@ int mm3e_renderer_set_wavefront(mm3e_renderer* r, uint32_t budget, uint32_t pass
 }
 
 int mm3e_scene_set_wavefront(mm3e_scene* s, int own, uint32_t budget, uint32_t passes, uint32_t blocks_per_cu) {
-  if (!s || passes + 1 > WF_MAX_PASSES || blocks_per_cu > 64) return MM3E_ERR_INVALID;
+  if (!s || passes >= WF_MAX_PASSES || blocks_per_cu > 64) return MM3E_ERR_INVALID;
   std::lock_guard<std::mutex> lk(s->r->mu);
   s->wf_own = own != 0;
   s->wf_set[0] = budget;

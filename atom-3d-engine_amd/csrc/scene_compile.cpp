@@ -390,11 +390,20 @@ bool parse_scene_ir(const void* ir, size_t len_, HostScene& sc, std::string& err
     return false;
   }
   for (const auto& v : sc.volumes) {
-    uint64_t n = (uint64_t)v.dims[0] * v.dims[1] * v.dims[2];
-    if ((uint64_t)v.offset + n > h.n_volume_floats) {
+    // products in 128 bits: three u32 dims can exceed 2^64 (volume.rs:44, 77-80 use checked_mul)
+    const unsigned __int128 n = (unsigned __int128)v.dims[0] * v.dims[1] * v.dims[2];
+    if ((unsigned __int128)v.offset + n > h.n_volume_floats) {
       err = "SdfVolume samples lie outside the IR volume data";
       return false;
     }
+  }
+  // probe counts above 2^32 (1.5 TiB of cubes) cannot be allocated; rejecting them also keeps
+  // every later probe-count product inside 64 bits
+  const unsigned __int128 gi_probes = (unsigned __int128)std::max(h.gi_dims[0], 1u) * std::max(h.gi_dims[1], 1u) *
+                                      std::max(h.gi_dims[2], 1u);
+  if (h.gi_mode != MM3E_GI_NONE && gi_probes > ((unsigned __int128)1 << 32)) {
+    err = "GI probe volume dims overflow";
+    return false;
   }
   if (h.gi_mode == MM3E_GI_BAKED) {
     uint64_t n = (uint64_t)h.gi_dims[0] * h.gi_dims[1] * h.gi_dims[2] * 18;

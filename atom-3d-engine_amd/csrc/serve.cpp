@@ -38,6 +38,7 @@ struct Request {
   uint8_t* staging = nullptr;  // pinned RGBA8
   int status = MM3E_OK;
   bool complete = false;
+  bool claimed = false;  // a thread is waiting on it (each id may be waited for once)
 };
 
 // First-fit allocator over one HBM slab (256-byte granules).
@@ -394,6 +395,8 @@ int mm3e_server_wait(mm3e_server* S, uint64_t request_id, double* latency_ms) {
   auto it = S->all.find(request_id);
   if (it == S->all.end()) return MM3E_ERR_INVALID;
   Request* q = it->second;
+  if (q->claimed) return MM3E_ERR_INVALID;  // another thread waits for it and will release it
+  q->claimed = true;
   S->cv_done.wait(lk, [&] { return q->complete; });
   if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(q->done - q->arrival).count();
   const int st = q->status;

@@ -373,7 +373,9 @@ int mm3e_server_add_scene(mm3e_server* s, mm3e_scene* scene, uint32_t* scene_id)
 int mm3e_server_submit(mm3e_server* s, uint32_t scene_id, const mm3e_camera* cam, uint32_t w, uint32_t h, uint8_t* out,
                        uint64_t* request_id);
 int mm3e_server_poll(mm3e_server* s, uint64_t request_id, int* done);
-/* Block until the request completes; returns its status and releases it. */
+/* Block until the request completes; returns its status and releases it.  Each request id may
+ * be waited for once: a second wait on an id (while the first is blocked, or after it returned)
+ * returns MM3E_ERR_INVALID. */
 int mm3e_server_wait(mm3e_server* s, uint64_t request_id, double* latency_ms);
 int mm3e_server_stats(mm3e_server* s, uint64_t* batches, uint64_t* completed, uint64_t* rejected, uint64_t* max_batch,
                       uint64_t* pixels);

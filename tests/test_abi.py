@@ -3,6 +3,7 @@ validates scenes with the reference's messages, and its host helpers match the o
 import ctypes as C
 import os
 import re
+import struct
 
 import numpy as np
 import pytest
@@ -81,3 +82,51 @@ def test_bmp_matches_oracle_encoder():
         img = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
         img[..., 3] = 255
         assert hip.write_bmp(img) == po.to_bmp(img)
+
+
+def _patch_header_u32(ir: bytes, field_offset: int, values) -> bytes:
+    b = bytearray(ir)
+    struct.pack_into("<%dI" % len(values), b, field_offset, *values)
+    return bytes(b)
+
+
+def test_volume_and_gi_dims_that_overflow_are_rejected():
+    """Volume dims whose product wraps 64 bits, and GI probe counts beyond 2^32, are rejected at
+    validation (volume.rs:44, 77-80 check their products) instead of passing a wrapped bound."""
+    import numpy as np
+    from mm3e.scene import Object, Prim, Scene, Transform, Volume
+    sc = Scene(8, 8)
+    vid = sc.volume(Volume((2, 2, 2), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), np.zeros(8, np.float32)))
+    sc.add(Object(Prim.volume(vid), Transform(), 0))
+    ir = sc.to_ir()
+    assert hip.scene_check(ir, 8, 8) is None
+    vol_at = _ir_offsets(ir)["volumes"]
+    for dims in ((1 << 22, 1 << 22, 1 << 20), (0xFFFFFFFF, 0xFFFFFFFF, 2), (3, 2, 2)):
+        bad = _patch_header_u32(ir, vol_at, dims)
+        assert hip.scene_check(bad, 8, 8) == "SdfVolume samples lie outside the IR volume data", dims
+    # GI: a bake request and baked cubes whose probe count exceeds 2^32
+    for mode in (1, 2):
+        sc2 = Scene(8, 8)
+        sc2.add(Object(Prim.sphere(1.0), Transform(), 0))
+        if mode == 2:
+            sc2.bake_gi((4, 4, 4), 1)
+        else:
+            sc2.set_baked_gi((0, 0, 0), (1, 1, 1), (2, 2, 2), np.zeros(2 * 2 * 2 * 18, np.float32))
+        ir2 = sc2.to_ir()
+        assert hip.scene_check(ir2, 8, 8) is None
+        bad = _patch_header_u32(ir2, _ir_offsets(ir2)["gi_dims"], (1 << 22, 1 << 22, 1 << 20))
+        assert hip.scene_check(bad, 8, 8) == "GI probe volume dims overflow"
+        bad = _patch_header_u32(ir2, _ir_offsets(ir2)["gi_dims"], (0xFFFFFFFF, 0xFFFFFFFF, 0xFFFFFFFF))
+        assert hip.scene_check(bad, 8, 8) == "GI probe volume dims overflow"
+
+
+def _ir_offsets(ir: bytes) -> dict:
+    """Byte offsets inside an IR blob: of the first volume record and of the header's gi_dims."""
+    # fixed header words, in Scene.to_ir order: magic(8) version total | 8 I | 4x(3f f) |
+    # marcher 12 words | post 8 words | counts 8 words | gi dims 3 + off | gi min 4 | gi size 4 |
+    # offsets 8 words
+    pos = 8 + 8 + 32 + 64 + 48 + 32 + 32
+    gi_dims = pos
+    pos += 16 + 16 + 16
+    offs = struct.unpack_from("<8I", ir, pos)
+    return {"gi_dims": gi_dims, "volumes": offs[3]}

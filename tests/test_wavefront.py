@@ -37,7 +37,26 @@ def test_set_wavefront_rejects_out_of_range():
         r.set_wavefront(8, passes=8)
     with pytest.raises(hip.Mm3eError):
         r.set_wavefront(8, blocks_per_cu=65)
+    # passes + 1 must not wrap: -1 reaches the library as 0xFFFFFFFF
+    for bad in (-1, 0xFFFFFFFF, 0xFFFFFFFE):
+        with pytest.raises(hip.Mm3eError):
+            r.set_wavefront(8, passes=bad)
+    sc, _ = configs.c1_scene(width=16, height=9)
+    g = r.compile(sc.to_ir(), 16, 9)
+    for bad in (-1, 0xFFFFFFFF, 8):
+        with pytest.raises(hip.Mm3eError):
+            g.set_wavefront(8, passes=bad)
+    g.close()
     r.close()
+
+
+@pytest.mark.parametrize("value,queued", [("8,1", True), ("8,7", True), ("8,8", False), ("8,-1", False),
+                                          ("8,4294967295", False), ("8,1,65", False)])
+def test_env_wavefront_range_is_checked(monkeypatch, value, queued):
+    """MM3E_WAVEFRONT out of range is ignored (no queued stages in the emitted source); CPU only."""
+    sc, _ = configs.c1_scene(width=16, height=9)
+    monkeypatch.setenv("MM3E_WAVEFRONT", value)
+    assert ("mm3e_gen_wf_" in hip.emit_source(sc.to_ir())) == queued
 
 
 @pytest.mark.gpu
