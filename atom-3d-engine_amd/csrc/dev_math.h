@@ -208,6 +208,18 @@ MM3E_DEV float vol_sample(const DVol& v, const float* __restrict__ data, V3 p) {
   return dd;
 }
 
+// sin / cos as the reference's f32 arithmetic gets them from the host libm, which returns the
+// correctly rounded value for nearly every input: evaluated in f64 and rounded once (the
+// f32 library routines on the device are accurate to a few ulp and round differently far more
+// often; twist / bend and the shadow-jitter hash amplify such differences into visible pixels).
+MM3E_DEV float cr_sin(float x) { return (float)sin((double)x); }
+MM3E_DEV void cr_sincos(float x, float& s, float& c) {
+  double sd, cd;
+  sincos((double)x, &sd, &cd);
+  s = (float)sd;
+  c = (float)cd;
+}
+
 // lib.rs:400-419 object-local point.
 MM3E_DEV V3 local_point(const DObj& o, V3 p) {
   V3 v = p - ld3(o.pos);
@@ -236,12 +248,14 @@ MM3E_DEV V3 local_point(const DObj& o, V3 p) {
   }
   if (f & OBJ_TWIST) {  // sdf.rs:220-224
     float a = o.twist * l.y;
-    float s = sinf(a), c = cosf(a);
+    float s, c;
+    cr_sincos(a, s, c);
     l = V3{c * l.x - s * l.z, l.y, s * l.x + c * l.z};
   }
   if (f & OBJ_BEND) {  // sdf.rs:227-231
     float a = o.bend * l.x;
-    float s = sinf(a), c = cosf(a);
+    float s, c;
+    cr_sincos(a, s, c);
     l = V3{c * l.x - s * l.y, s * l.x + c * l.y, l.z};
   }
   return l;
@@ -948,7 +962,7 @@ MM3E_DEV bool shadow_init(const DevScene& sc, ShadowState& s, V3 o, V3 d, float 
   s.k = k;
   s.res = 1.0f;
   float hb = o.x * 127.1f + o.y * 311.7f + o.z * 74.7f;
-  float sv = sinf(hb) * 43758.547f;
+  float sv = cr_sin(hb) * 43758.547f;
   float jitter = rabs(sv - __builtin_truncf(sv));  // f32::fract
   s.t = 0.02f + jitter * 0.16f;
   s.s = 0;

@@ -38,7 +38,7 @@ More legs ride on the same line (``--legs`` picks them):
   queued:  c2 compiled with the queued-frame stages (budgeted march passes over compacted ray
            queues, hit-queue shading); the scene autotuner's render-stage ms for the one-kernel
            frame and each queued candidate, its choice, and whether the images are equal.
-  stress:  (not in the default set) one 3840x2160 frame of the 1000-primitive scene with a 128^3
+  stress:  one 3840x2160 frame of the 1000-primitive scene with a 128^3
            sampled volume, 32^3 device-baked GI, 2 bounces and fog; ms/frame.
 """
 import argparse
@@ -60,9 +60,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(ir, cam, threads, frames=2):
-    """Oracle CPU renderer (row-band threads, the reference's render_with_threads) on one full
-    frame; 1 untimed warm-up frame, then `frames` timed frames."""
+def cpu_baseline(ir, cam, threads, frames=3):
+    """Oracle CPU renderer (row-band threads, the reference's render_with_threads) on full
+    frames: 1 untimed warm-up frame, then `frames` timed frames; plus a 1-thread figure on
+    four bands of rows spread over the frame (1/8 of its rows)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     s = po.OracleScene(ir, threads=threads)
@@ -71,7 +72,21 @@ def cpu_baseline(ir, cam, threads, frames=2):
     for _ in range(frames):
         s.render_rgba(cam, threads=threads)
     dt = (time.perf_counter() - t0) / frames
-    return s.width * s.height / dt / 1e6, dt
+    band = max(1, s.height // 32)
+    rows = 0
+    t0 = time.perf_counter()
+    for k in range(4):
+        y0 = (2 * k + 1) * s.height // 8 - band // 2
+        s.render_linear_rows(cam, y0, y0 + band, threads=1)
+        rows += band
+    dt1 = time.perf_counter() - t0
+    return {"mpix_s": s.width * s.height / dt / 1e6, "s_per_frame": dt,
+            "mpix_s_1thread": s.width * rows / dt1 / 1e6, "rows_1thread": rows, "s_1thread": dt1}
+
+
+def flops_record(name):
+    with open(os.path.join(ROOT, "tests", "golden", "flops.json")) as f:
+        return json.load(f).get(name)
 
 
 def pct(xs, q):
@@ -125,9 +140,17 @@ def batched_leg(hip, configs, r, reps):
     b.close()
     for g in scenes.values():
         g.close()
+    fl = flops_record("c3")
+    roof = None
+    if fl and fl.get("pixels") == px:
+        ach = fl["flops_per_frame"] / (min(gpu) * 1e-3) / 1e12
+        roof = {"bound": "valu", "kernel": "persistent batch render (queue reset + render + post)",
+                "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach / PEAK_FP32_TFLOPS, 4), "frac_nonpacked_peak": round(ach / PEAK_FP32_NONPACKED_TFLOPS, 4),
+                "flops_per_launch": fl["flops_per_frame"], "flops_sample": fl["sample"]}
     out = {"workload": "64 cameras over 8 seeded procedural scenes (20-200 primitives, 30 % smooth, 20 % domain "
                        "ops), per-camera resolution uniform over {320x180, 640x360, 1280x720, 1920x1080}, aa=1, "
-                       "1 reflection bounce",
+                       "1 reflection bounce", "roofline": roof,
            "jobs": len(jobs), "mpix_per_batch": round(px / 1e6, 3),
            "plain_mpix_s": round(px / plain_s / 1e6, 3), "plain_s": round(plain_s, 4),
            "batched_mpix_s": round(px / batched_s / 1e6, 3), "batched_s": round(batched_s, 4),
@@ -258,7 +281,14 @@ def stress_leg(hip, configs, r, frames):
            "mpix_s": round(sc.width * sc.height / dt / 1e6, 3), "render_kernel": g.kernel_name(),
            "render_kernel_ms": round(render_ms, 2), "post_ms": round(post_ms, 3), "compile_s": round(compile_s, 2),
            "codegen_s": round(codegen_s, 1), "objects": info["objects"], "segments": info["segments"],
-           "grid_cells": info["grid_cells"]}
+           "grid_cells": info["grid_cells"], "roofline": None}
+    fl = flops_record("c5")
+    if fl and fl.get("width") == sc.width and fl.get("height") == sc.height and n:
+        ach = fl["flops_per_frame"] / (render_ms * 1e-3) / 1e12
+        out["roofline"] = {"bound": "valu", "kernel": g.kernel_name(), "achieved": round(ach, 3),
+                           "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4),
+                           "frac_nonpacked_peak": round(ach / PEAK_FP32_NONPACKED_TFLOPS, 4),
+                           "flops_per_launch": fl["flops_per_frame"], "flops_sample": fl["sample"]}
     log(f"stress: {out['ms_per_frame']} ms/frame ({out['mpix_s']} Mpix/s) with {out['render_kernel']}")
     g.close()
     return out
@@ -415,8 +445,8 @@ def main():
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="plain,batched,served,mesh_bake,frame_gen,queued",
-                    help="comma list of plain (always run), batched, served, mesh_bake, frame_gen, queued, stress")
+    ap.add_argument("--legs", default="plain,batched,served,stress,mesh_bake,frame_gen,queued",
+                    help="comma list of plain (always run), batched, served, stress, mesh_bake, frame_gen, queued")
     ap.add_argument("--bake-reps", type=int, default=20)
     ap.add_argument("--stress-frames", type=int, default=3)
     ap.add_argument("--batched-reps", type=int, default=3)
@@ -506,12 +536,16 @@ def main():
     }
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-        log(f"cpu baseline: oracle on {threads} threads, one full frame (1 warm-up + 2 timed)")
-        cv, cdt = cpu_baseline(ir, cam, threads)
-        out["cpu_baseline"] = {"value": round(cv, 4), "unit": "Mpix/s", "cores": threads, "kind": "port",
-                               "sample": f"full c2 960x540 frame (aa=2), 2 timed frames after 1 warm-up, "
-                                         f"{cdt:.2f} s/frame"}
-        log(f"cpu baseline {cv:.4f} Mpix/s ({cdt:.2f} s/frame)")
+        log(f"cpu baseline: oracle on {threads} threads, full frames (1 warm-up + 3 timed), then 1 thread")
+        cb = cpu_baseline(ir, cam, threads)
+        out["cpu_baseline"] = {"value": round(cb["mpix_s"], 4), "unit": "Mpix/s", "cores": threads, "kind": "port",
+                               "sample": f"full c2 960x540 frames (aa=2), 3 timed after 1 warm-up, "
+                                         f"{cb['s_per_frame']:.2f} s/frame",
+                               "value_1thread": round(cb["mpix_s_1thread"], 4),
+                               "sample_1thread": f"{cb['rows_1thread']} rows of the same frame in 4 bands on 1 thread, "
+                                                 f"{cb['s_1thread']:.2f} s"}
+        log(f"cpu baseline {cb['mpix_s']:.4f} Mpix/s ({cb['s_per_frame']:.2f} s/frame), "
+            f"1 thread {cb['mpix_s_1thread']:.4f} Mpix/s")
     legs = set(args.legs.split(","))
     if "batched" in legs:
         out["batched"] = batched_leg(hip, configs, r, args.batched_reps)

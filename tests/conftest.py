@@ -23,3 +23,32 @@ def renderer():
     r.set_codegen(hip.CODEGEN_OFF)
     yield r
     r.close()
+
+
+@pytest.fixture(scope="session")
+def gen_renderer():
+    """A renderer in the bench's configuration: each scene's generated kernel is compiled inside
+    compile (hipRTC), tile order on, default waves per SIMD."""
+    from mm3e import hip
+    r = hip.Renderer()
+    r.set_codegen(hip.CODEGEN_SYNC)
+    yield r
+    r.close()
+
+
+_ORACLE_CACHE = {}
+
+
+@pytest.fixture(scope="session")
+def oracle_c2():
+    """The CPU oracle's c2 frame at 960x540 (GI baked on the CPU): IR, camera, RGBA8, AOVs and
+    the raw GI volume, computed once per session."""
+    if "c2" not in _ORACLE_CACHE:
+        import pyoracle as po
+        from mm3e import configs
+        sc, cam = configs.c2_scene()
+        ir = sc.to_ir()
+        o = po.OracleScene(ir)
+        _ORACLE_CACHE["c2"] = {"ir": ir, "cam": cam, "rgba": o.render_rgba(cam), "aov": o.render_aov(cam),
+                               "gi": o.gi_raw()}
+    return _ORACLE_CACHE["c2"]
