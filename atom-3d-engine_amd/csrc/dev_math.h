@@ -9,6 +9,7 @@
 // f32::min/max are IEEE minNum/maxNum (v_min_f32/v_max_f32); f32::clamp is spelled out as
 // compare-select so NaN and signed zero behave as in the reference.
 #pragma once
+#include "cr_trig.h"
 #include "dev_scene.h"
 
 #ifndef MM3E_DEV
@@ -206,18 +207,6 @@ MM3E_DEV float vol_sample(const DVol& v, const float* __restrict__ data, V3 p) {
   if (!__builtin_isfinite(dd)) return __builtin_inff();
   if (outside > 0.0f) return rmax(outside, dd - outside);
   return dd;
-}
-
-// sin / cos as the reference's f32 arithmetic gets them from the host libm, which returns the
-// correctly rounded value for nearly every input: evaluated in f64 and rounded once (the
-// f32 library routines on the device are accurate to a few ulp and round differently far more
-// often; twist / bend and the shadow-jitter hash amplify such differences into visible pixels).
-MM3E_DEV float cr_sin(float x) { return (float)sin((double)x); }
-MM3E_DEV void cr_sincos(float x, float& s, float& c) {
-  double sd, cd;
-  sincos((double)x, &sd, &cd);
-  s = (float)sd;
-  c = (float)cd;
 }
 
 // lib.rs:400-419 object-local point.

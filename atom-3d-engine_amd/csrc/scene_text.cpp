@@ -3,8 +3,21 @@
 // write a scene IR plus camera back to text.  Line-oriented: a directive keyword followed by
 // whitespace-separated values, `#` starts a comment.  Numbers print as the shortest decimal
 // that reads back to the same f32 (Rust's `{}` for f32), so text -> IR -> text is stable.
-// The format carries no CSG, volumes, GI, shadow/AO budgets or normal_h (scene_io.rs:47-62,
-// 103-105); parsed scenes take Scene::new's values for those.
+// The reference's format drops CSG, volumes, GI, the shadow/AO budgets, LOD, the march boosts
+// and normal_h (scene_io.rs:47-62, 103-105).  This library extends it losslessly with
+// directives the reference's parser does not know, written only when a scene needs them (a
+// scene inside the reference's subset is written exactly in the reference's format):
+//   sun_raw x y z                       sun direction as stored (when re-normalising changes it)
+//   marcher_ext shadow_steps ao_samples lod_footprint subitize secant normal_h
+//   gi bake nx ny nz samples            Scene::bake_gi request (lib.rs:665-711)
+//   gi baked nx ny nz minx miny minz sx sy sz <cubes>   a baked GiVolume, [probe][6][3] f32
+//   volume sdfv <bytes>                 an SdfVolume as its `.sdfv` file (volume.rs:115-187)
+//   volume raw nx ny nz minx miny minz cx cy cz <samples>   a volume .sdfv would reject
+//   csgnode op prim a b p0..p7 r0..r8 px py pz scale   one csg::Expr node (csg.rs:17-91)
+//   csgexpr root                        one registered expression (Scene::csg)
+//   obj csg <id> ... / obj volume <id> ...   Prim::Csg / Prim::Volume objects
+// <bytes>, <cubes>, <samples>: base64 (RFC 4648) of the little-endian bytes.  Text -> IR is
+// then bit-exact for every IR the library accepts: IR -> text -> IR returns the same bytes.
 #include <charconv>
 #include <cmath>
 #include <cstdio>
@@ -46,6 +59,56 @@ bool parse_u32(const std::string& t, uint32_t& v) {
   if (s < e && *s == '+') s++;
   auto r = std::from_chars(s, e, v);
   return r.ec == std::errc() && r.ptr == e && s != e;
+}
+
+// ---- base64 (RFC 4648 with padding) for the binary payloads of the extension ------------
+const char kB64[] = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/";
+std::string b64_encode(const uint8_t* p, size_t n) {
+  std::string out;
+  out.reserve((n + 2) / 3 * 4);
+  size_t i = 0;
+  for (; i + 2 < n; i += 3) {
+    const uint32_t v = (uint32_t)p[i] << 16 | (uint32_t)p[i + 1] << 8 | p[i + 2];
+    out += kB64[v >> 18];
+    out += kB64[(v >> 12) & 63];
+    out += kB64[(v >> 6) & 63];
+    out += kB64[v & 63];
+  }
+  if (i < n) {
+    const uint32_t v = (uint32_t)p[i] << 16 | (i + 1 < n ? (uint32_t)p[i + 1] << 8 : 0u);
+    out += kB64[v >> 18];
+    out += kB64[(v >> 12) & 63];
+    out += i + 1 < n ? kB64[(v >> 6) & 63] : '=';
+    out += '=';
+  }
+  return out;
+}
+bool b64_decode(const std::string& s, std::vector<uint8_t>& out) {
+  if (s.size() % 4) return false;
+  auto val = [](char c) -> int {
+    if (c >= 'A' && c <= 'Z') return c - 'A';
+    if (c >= 'a' && c <= 'z') return c - 'a' + 26;
+    if (c >= '0' && c <= '9') return c - '0' + 52;
+    if (c == '+') return 62;
+    if (c == '/') return 63;
+    return -1;
+  };
+  out.clear();
+  out.reserve(s.size() / 4 * 3);
+  for (size_t i = 0; i < s.size(); i += 4) {
+    const bool last = i + 4 == s.size();
+    const int a = val(s[i]), b = val(s[i + 1]);
+    const int c = (last && s[i + 2] == '=') ? 0 : val(s[i + 2]);
+    const int d = (last && s[i + 3] == '=') ? 0 : val(s[i + 3]);
+    if (a < 0 || b < 0 || c < 0 || d < 0) return false;
+    if (!last && (s[i + 2] == '=' || s[i + 3] == '=')) return false;
+    if (last && s[i + 2] == '=' && s[i + 3] != '=') return false;
+    const uint32_t v = (uint32_t)a << 18 | (uint32_t)b << 12 | (uint32_t)c << 6 | (uint32_t)d;
+    out.push_back((uint8_t)(v >> 16));
+    if (!(last && s[i + 2] == '=')) out.push_back((uint8_t)(v >> 8));
+    if (!(last && s[i + 3] == '=')) out.push_back((uint8_t)v);
+  }
+  return true;
 }
 
 // ---- Vec3 helpers in f32, in the reference's operation order (vec.rs) ---------------------
@@ -157,26 +220,62 @@ mm3e_ir_material default_material() {  // color.rs:47-60
   return m;
 }
 
-std::vector<uint8_t> write_ir(mm3e_ir_header h, const std::vector<mm3e_ir_object>& objs,
-                              const std::vector<mm3e_ir_material>& mats, const std::vector<mm3e_ir_light>& lights) {
-  h.n_objects = (uint32_t)objs.size();
-  h.n_materials = (uint32_t)mats.size();
-  h.n_lights = (uint32_t)lights.size();
-  uint32_t off = sizeof(mm3e_ir_header);
-  h.off_objects = off;
-  off += (uint32_t)(objs.size() * sizeof(mm3e_ir_object));
-  h.off_materials = off;
-  off += (uint32_t)(mats.size() * sizeof(mm3e_ir_material));
-  h.off_lights = off;
-  off += (uint32_t)(lights.size() * sizeof(mm3e_ir_light));
-  h.off_volumes = h.off_volume_data = h.off_csg_nodes = h.off_csg_roots = h.off_gi_cubes = off;
-  h.total_bytes = off;
-  std::vector<uint8_t> out(off);
+// The parsed scene before it is laid out as IR.
+struct TextScene {
+  mm3e_ir_header h;
+  std::vector<mm3e_ir_object> objs;
+  std::vector<mm3e_ir_material> mats;
+  std::vector<mm3e_ir_light> lights;
+  std::vector<mm3e_ir_volume> vols;
+  std::vector<float> vdata;
+  std::vector<mm3e_ir_csg_node> csg;
+  std::vector<uint32_t> roots;
+  std::vector<float> cubes;
+};
+
+// IR sections in a fixed order: header, objects, materials, lights, volumes, volume samples,
+// CSG nodes, CSG roots, GI cubes.
+bool write_ir(TextScene& t, std::vector<uint8_t>& out, std::string& err) {
+  mm3e_ir_header& h = t.h;
+  h.n_objects = (uint32_t)t.objs.size();
+  h.n_materials = (uint32_t)t.mats.size();
+  h.n_lights = (uint32_t)t.lights.size();
+  h.n_volumes = (uint32_t)t.vols.size();
+  h.n_volume_floats = (uint32_t)t.vdata.size();
+  h.n_csg_nodes = (uint32_t)t.csg.size();
+  h.n_csg_exprs = (uint32_t)t.roots.size();
+  uint64_t off = sizeof(mm3e_ir_header);
+  auto place = [&](uint32_t& field, uint64_t bytes) {
+    field = (uint32_t)off;
+    off += bytes;
+  };
+  place(h.off_objects, t.objs.size() * sizeof(mm3e_ir_object));
+  place(h.off_materials, t.mats.size() * sizeof(mm3e_ir_material));
+  place(h.off_lights, t.lights.size() * sizeof(mm3e_ir_light));
+  place(h.off_volumes, t.vols.size() * sizeof(mm3e_ir_volume));
+  place(h.off_volume_data, t.vdata.size() * 4);
+  place(h.off_csg_nodes, t.csg.size() * sizeof(mm3e_ir_csg_node));
+  place(h.off_csg_roots, t.roots.size() * 4);
+  place(h.off_gi_cubes, t.cubes.size() * 4);
+  if (off > 0xFFFFFFFFull) {
+    err = "scene too large for the IR (more than 4 GiB)";
+    return false;
+  }
+  h.total_bytes = (uint32_t)off;
+  out.assign(off, 0);
   std::memcpy(out.data(), &h, sizeof h);
-  if (!objs.empty()) std::memcpy(out.data() + h.off_objects, objs.data(), objs.size() * sizeof(mm3e_ir_object));
-  if (!mats.empty()) std::memcpy(out.data() + h.off_materials, mats.data(), mats.size() * sizeof(mm3e_ir_material));
-  if (!lights.empty()) std::memcpy(out.data() + h.off_lights, lights.data(), lights.size() * sizeof(mm3e_ir_light));
-  return out;
+  auto put = [&](uint32_t at, const void* src, size_t bytes) {
+    if (bytes) std::memcpy(out.data() + at, src, bytes);
+  };
+  put(h.off_objects, t.objs.data(), t.objs.size() * sizeof(mm3e_ir_object));
+  put(h.off_materials, t.mats.data(), t.mats.size() * sizeof(mm3e_ir_material));
+  put(h.off_lights, t.lights.data(), t.lights.size() * sizeof(mm3e_ir_light));
+  put(h.off_volumes, t.vols.data(), t.vols.size() * sizeof(mm3e_ir_volume));
+  put(h.off_volume_data, t.vdata.data(), t.vdata.size() * 4);
+  put(h.off_csg_nodes, t.csg.data(), t.csg.size() * sizeof(mm3e_ir_csg_node));
+  put(h.off_csg_roots, t.roots.data(), t.roots.size() * 4);
+  put(h.off_gi_cubes, t.cubes.data(), t.cubes.size() * 4);
+  return true;
 }
 
 bool positive3(const float* v) { return v[0] > 0.0f && v[1] > 0.0f && v[2] > 0.0f; }
@@ -250,7 +349,12 @@ bool parse_object(Cur& c, mm3e_ir_object& o) {
   else if (name == "octahedron") { o.prim = MM3E_PRIM_OCTAHEDRON; ok = c.f("s", p[0]); }
   else if (name == "hexprism") { o.prim = MM3E_PRIM_HEX_PRISM; ok = c.f("r", p[0]) && c.f("h", p[1]); }
   else if (name == "plane") { o.prim = MM3E_PRIM_PLANE; ok = c.v3("n", p) && c.f("h", p[3]); }
-  else {
+  else if (name == "csg" || name == "volume") {  // extension: Prim::Csg { id } / Prim::Volume { id }
+    uint32_t id = 0;
+    o.prim = name == "csg" ? MM3E_PRIM_CSG : MM3E_PRIM_VOLUME;
+    ok = c.u(name == "csg" ? "csg id" : "volume id", id);
+    std::memcpy(&p[0], &id, 4);
+  } else {
     c.err = "line " + std::to_string(c.line) + ": unknown primitive '" + name + "'";
     return false;
   }
@@ -302,11 +406,12 @@ bool parse_object(Cur& c, mm3e_ir_object& o) {
 
 // scene_io.rs:200-331 parse
 bool parse_text(const std::string& src, std::vector<uint8_t>& ir, mm3e_camera& cam, std::string& err) {
-  mm3e_ir_header h;
+  TextScene ts;
+  mm3e_ir_header& h = ts.h;
   scene_defaults(h, 640, 360);
-  std::vector<mm3e_ir_object> objs;
-  std::vector<mm3e_ir_material> mats;  // Scene::new's default material is cleared
-  std::vector<mm3e_ir_light> lights;
+  std::vector<mm3e_ir_object>& objs = ts.objs;
+  std::vector<mm3e_ir_material>& mats = ts.mats;  // Scene::new's default material is cleared
+  std::vector<mm3e_ir_light>& lights = ts.lights;
   {
     const float eye[3] = {0.0f, 1.0f, 6.0f}, target[3] = {0, 0, 0}, up[3] = {0.0f, 1.0f, 0.0f};
     mm3e_camera_look_at(eye, target, up, 1.0f, &cam);
@@ -440,6 +545,101 @@ bool parse_text(const std::string& src, std::vector<uint8_t>& ir, mm3e_camera& c
       mm3e_ir_object o;
       ok = parse_object(c, o);
       if (ok) objs.push_back(o);
+    } else if (d == "sun_raw") {
+      ok = c.v3("sun", h.sun_dir);
+    } else if (d == "marcher_ext") {
+      ok = c.u("shadow_steps", h.shadow_steps) && c.u("ao_samples", h.ao_samples) &&
+           c.f("lod_footprint", h.lod_footprint) && c.f("subitize", h.subitize) && c.b("secant", h.secant) &&
+           c.f("normal_h", h.normal_h);
+    } else if (d == "gi") {
+      std::string kind;
+      ok = c.next("gi kind", kind);
+      if (ok && kind == "bake") {
+        h.gi_mode = MM3E_GI_BAKE;
+        ok = c.u("gi nx", h.gi_dims[0]) && c.u("gi ny", h.gi_dims[1]) && c.u("gi nz", h.gi_dims[2]) &&
+             c.u("gi samples", h.gi_samples);
+      } else if (ok && kind == "baked") {
+        h.gi_mode = MM3E_GI_BAKED;
+        h.gi_samples = 0;
+        std::string b64;
+        std::vector<uint8_t> bytes;
+        ok = c.u("gi nx", h.gi_dims[0]) && c.u("gi ny", h.gi_dims[1]) && c.u("gi nz", h.gi_dims[2]) &&
+             c.v3("gi min", h.gi_min) && c.v3("gi size", h.gi_size) && c.next("gi cubes", b64);
+        if (ok) {
+          const unsigned __int128 want = (unsigned __int128)h.gi_dims[0] * h.gi_dims[1] * h.gi_dims[2] * 18 * 4;
+          if (!b64_decode(b64, bytes) || (unsigned __int128)bytes.size() != want) {
+            err = L + "gi cubes must be base64 of nx*ny*nz*18 f32";
+            return false;
+          }
+          ts.cubes.resize(bytes.size() / 4);
+          std::memcpy(ts.cubes.data(), bytes.data(), bytes.size());
+        }
+      } else if (ok) {
+        err = L + "unknown gi kind '" + kind + "'";
+        return false;
+      }
+    } else if (d == "volume") {
+      std::string kind, b64;
+      std::vector<uint8_t> bytes;
+      mm3e_ir_volume v;
+      std::memset(&v, 0, sizeof v);
+      ok = c.next("volume kind", kind);
+      if (ok && kind == "sdfv") {
+        ok = c.next("sdfv bytes", b64);
+        if (ok) {
+          if (!b64_decode(b64, bytes)) {
+            err = L + "volume sdfv bytes are not base64";
+            return false;
+          }
+          mm3e_sdf_layout lay;
+          char e2[256];
+          if (mm3e_sdfv_decode(bytes.data(), bytes.size(), &lay, nullptr, 0, e2, sizeof e2) != MM3E_OK) {
+            err = L + e2;
+            return false;
+          }
+          const size_t n = (size_t)lay.dims[0] * lay.dims[1] * lay.dims[2];
+          const size_t at = ts.vdata.size();
+          ts.vdata.resize(at + n);
+          if (mm3e_sdfv_decode(bytes.data(), bytes.size(), &lay, ts.vdata.data() + at, n, e2, sizeof e2) != MM3E_OK) {
+            err = L + e2;
+            return false;
+          }
+          std::memcpy(v.dims, lay.dims, 12);
+          std::memcpy(v.min, lay.min, 12);
+          std::memcpy(v.cell, lay.cell, 12);
+          v.offset = (uint32_t)at;
+          ts.vols.push_back(v);
+        }
+      } else if (ok && kind == "raw") {
+        ok = c.u("nx", v.dims[0]) && c.u("ny", v.dims[1]) && c.u("nz", v.dims[2]) && c.v3("volume min", v.min) &&
+             c.v3("volume cell", v.cell) && c.next("volume samples", b64);
+        if (ok) {
+          const unsigned __int128 want = (unsigned __int128)v.dims[0] * v.dims[1] * v.dims[2] * 4;
+          if (!b64_decode(b64, bytes) || (unsigned __int128)bytes.size() != want) {
+            err = L + "volume samples must be base64 of nx*ny*nz f32";
+            return false;
+          }
+          v.offset = (uint32_t)ts.vdata.size();
+          ts.vdata.resize(ts.vdata.size() + bytes.size() / 4);
+          std::memcpy(ts.vdata.data() + v.offset, bytes.data(), bytes.size());
+          ts.vols.push_back(v);
+        }
+      } else if (ok) {
+        err = L + "unknown volume kind '" + kind + "'";
+        return false;
+      }
+    } else if (d == "csgnode") {
+      mm3e_ir_csg_node n;
+      std::memset(&n, 0, sizeof n);
+      ok = c.u("csg op", n.op) && c.u("csg prim", n.prim) && c.u("csg a", n.a) && c.u("csg b", n.b);
+      for (int k = 0; ok && k < 8; k++) ok = c.f("csg p", n.p[k]);
+      for (int k = 0; ok && k < 9; k++) ok = c.f("csg rot", n.rot[k]);
+      ok = ok && c.v3("csg pos", n.pos) && c.f("csg scale", n.scale);
+      if (ok) ts.csg.push_back(n);
+    } else if (d == "csgexpr") {
+      uint32_t root = 0;
+      ok = c.u("csg root", root);
+      if (ok) ts.roots.push_back(root);
     } else {
       err = L + "unknown directive '" + d + "'";
       return false;
@@ -457,20 +657,19 @@ bool parse_text(const std::string& src, std::vector<uint8_t>& ir, mm3e_camera& c
             " out of range (have " + std::to_string(mats.size()) + ")";
       return false;
     }
-  ir = write_ir(h, objs, mats, lights);
-  return true;
+  return write_ir(ts, ir, err);
 }
 
 // scene_io.rs:32-164 try_serialize
 bool write_text(const HostScene& sc, const mm3e_camera& cam, std::string& s, std::string& err) {
   const mm3e_ir_header& h = sc.hdr;
+  if (h.appearance != 0) {
+    err = "UV textures are not representable in legacy .mm3e; use the native editor format";
+    return false;
+  }
   for (const auto& o : sc.objects) {
     if (o.prim == MM3E_PRIM_SURFACE) {
       err = "triangle surfaces are not representable in the legacy .mm3e text format; use the durable editor format";
-      return false;
-    }
-    if (o.prim == MM3E_PRIM_CSG) {
-      err = "object-local CSG is not representable in the legacy .mm3e text format";
       return false;
     }
   }
@@ -481,6 +680,11 @@ bool write_text(const HostScene& sc, const mm3e_camera& cam, std::string& s, std
   s += "shadows " + std::to_string(h.shadows ? 1 : 0) + "\n";
   s += "ao " + std::to_string(h.ao ? 1 : 0) + "\n";
   s += "sun " + fmt3(h.sun_dir) + "\n";
+  {  // the parser re-normalises `sun`; keep the stored vector when that would change its bits
+    const F3 n = normalize(F3{h.sun_dir[0], h.sun_dir[1], h.sun_dir[2]});
+    const float nn[3] = {n.x, n.y, n.z};
+    if (std::memcmp(nn, h.sun_dir, 12) != 0) s += "sun_raw " + fmt3(h.sun_dir) + "\n";
+  }
   s += "ambient " + fmt3(h.ambient) + "\n";
   s += "sky_ambient " + fmt3(h.sky_ambient) + "\n";
   s += "fog " + fmt3(h.fog) + " " + fmt(h.fog_density) + "\n";
@@ -488,6 +692,26 @@ bool write_text(const HostScene& sc, const mm3e_camera& cam, std::string& s, std
   s += "post " + fmt(h.exposure) + " " + std::to_string(h.bloom ? 1 : 0) + " " + fmt(h.bloom_threshold) + " " +
        fmt(h.bloom_intensity) + " " + std::to_string(h.bloom_radius) + "\n";
   s += std::string("mode ") + kModeNames[h.mode < 6 ? h.mode : 0] + "\n";
+  {  // marcher budgets the reference's `marcher` line leaves out (Marcher::default otherwise)
+    mm3e_ir_header d;
+    scene_defaults(d, 1, 1);
+    const bool same = h.shadow_steps == d.shadow_steps && h.ao_samples == d.ao_samples &&
+                      std::memcmp(&h.lod_footprint, &d.lod_footprint, 4) == 0 &&
+                      std::memcmp(&h.subitize, &d.subitize, 4) == 0 && (h.secant != 0) == (d.secant != 0) &&
+                      std::memcmp(&h.normal_h, &d.normal_h, 4) == 0;
+    if (!same)
+      s += "marcher_ext " + std::to_string(h.shadow_steps) + " " + std::to_string(h.ao_samples) + " " +
+           fmt(h.lod_footprint) + " " + fmt(h.subitize) + " " + std::to_string(h.secant ? 1 : 0) + " " +
+           fmt(h.normal_h) + "\n";
+  }
+  if (h.gi_mode == MM3E_GI_BAKE) {
+    s += "gi bake " + std::to_string(h.gi_dims[0]) + " " + std::to_string(h.gi_dims[1]) + " " +
+         std::to_string(h.gi_dims[2]) + " " + std::to_string(h.gi_samples) + "\n";
+  } else if (h.gi_mode == MM3E_GI_BAKED) {
+    s += "gi baked " + std::to_string(h.gi_dims[0]) + " " + std::to_string(h.gi_dims[1]) + " " +
+         std::to_string(h.gi_dims[2]) + " " + fmt3(h.gi_min) + " " + fmt3(h.gi_size) + " " +
+         b64_encode(reinterpret_cast<const uint8_t*>(sc.gi_cubes.data()), sc.gi_cubes.size() * 4) + "\n";
+  }
   // eye, target = eye + forward, up, vertical FOV in degrees: (atan(fov_scale) * 2).to_degrees()
   const float target[3] = {cam.eye[0] + cam.forward[0], cam.eye[1] + cam.forward[1], cam.eye[2] + cam.forward[2]};
   const float fov_deg = (std::atan(cam.fov_scale) * 2.0f) * 57.2957795130823208767981548141051703f;
@@ -498,18 +722,44 @@ bool write_text(const HostScene& sc, const mm3e_camera& cam, std::string& s, std
   for (const auto& l : sc.lights)
     s += std::string("light ") + (l.directional ? "dir" : "point") + " " + fmt3(l.vec) + " " + fmt3(l.color) + " " +
          fmt(l.radius) + "\n";
-  static const char* names[11] = {"sphere", "box", "roundbox", "torus", "cylinder", "capsule",
-                                  "cone", "ellipsoid", "octahedron", "hexprism", "plane"};
-  for (const auto& o : sc.objects) {
-    if (o.prim == MM3E_PRIM_VOLUME) {
-      uint32_t id;
-      std::memcpy(&id, &o.p[0], 4);
-      s += "# volume object omitted (baked SDF volume id " + std::to_string(id) + " is not text-serializable)\n";
-      continue;
+  for (const auto& v : sc.volumes) {  // SdfVolume as its .sdfv bytes when the codec takes it
+    const size_t n = (size_t)v.dims[0] * v.dims[1] * v.dims[2];
+    const float* data = sc.volume_data.data() + v.offset;
+    mm3e_sdf_layout lay;
+    std::memcpy(lay.dims, v.dims, 12);
+    std::memcpy(lay.min, v.min, 12);
+    std::memcpy(lay.cell, v.cell, 12);
+    size_t len = 0;
+    mm3e_sdfv_encode(&lay, data, n, nullptr, &len, nullptr, 0);
+    std::vector<uint8_t> bytes(len);
+    if (len && mm3e_sdfv_encode(&lay, data, n, bytes.data(), &len, nullptr, 0) == MM3E_OK) {
+      s += "volume sdfv " + b64_encode(bytes.data(), len) + "\n";
+    } else {
+      s += "volume raw " + std::to_string(v.dims[0]) + " " + std::to_string(v.dims[1]) + " " +
+           std::to_string(v.dims[2]) + " " + fmt3(v.min) + " " + fmt3(v.cell) + " " +
+           b64_encode(reinterpret_cast<const uint8_t*>(data), n * 4) + "\n";
     }
+  }
+  for (const auto& n : sc.csg_nodes) {
+    std::string t = "csgnode " + std::to_string(n.op) + " " + std::to_string(n.prim) + " " + std::to_string(n.a) +
+                    " " + std::to_string(n.b);
+    for (int k = 0; k < 8; k++) t += " " + fmt(n.p[k]);
+    for (int k = 0; k < 9; k++) t += " " + fmt(n.rot[k]);
+    s += t + " " + fmt3(n.pos) + " " + fmt(n.scale) + "\n";
+  }
+  for (uint32_t r : sc.csg_roots) s += "csgexpr " + std::to_string(r) + "\n";
+  static const char* names[13] = {"sphere", "box", "roundbox", "torus", "cylinder", "capsule", "cone",
+                                  "ellipsoid", "octahedron", "hexprism", "plane", "csg", "volume"};
+  for (const auto& o : sc.objects) {
     const float* p = o.p;
     std::string t = std::string("obj ") + names[o.prim] + " ";
     switch (o.prim) {
+      case MM3E_PRIM_CSG: case MM3E_PRIM_VOLUME: {
+        uint32_t id;
+        std::memcpy(&id, &p[0], 4);
+        t += std::to_string(id);
+        break;
+      }
       case MM3E_PRIM_SPHERE: case MM3E_PRIM_OCTAHEDRON: t += fmt(p[0]); break;
       case MM3E_PRIM_BOX: case MM3E_PRIM_ELLIPSOID: t += fmt3(p); break;
       case MM3E_PRIM_ROUND_BOX: t += fmt3(p) + " " + fmt(p[3]); break;

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -113,6 +114,9 @@ struct mm3e_server {
   std::thread worker;
   uint32_t cus = 256;
   std::multimap<size_t, uint8_t*> pinned_free;  // reusable pinned staging buffers by size
+  // scenes compiled from request text, owned by the server: IR bytes -> scene id
+  std::map<std::string, uint32_t> text_scenes;
+  std::vector<mm3e_scene*> owned;
   // stats
   uint64_t batches = 0, completed = 0, rejected = 0, max_batch_seen = 0;
   uint64_t pixels_done = 0;
@@ -405,6 +409,59 @@ int mm3e_server_wait(mm3e_server* S, uint64_t request_id, double* latency_ms) {
   return st;
 }
 
+int mm3e_server_submit_text(mm3e_server* S, const char* text, size_t text_len, uint32_t w, uint32_t h, uint8_t* out,
+                            uint64_t* request_id, char* err, size_t errlen) {
+  if (!S || !text || !request_id) return MM3E_ERR_INVALID;
+  size_t n = 0;
+  mm3e_camera cam;
+  mm3e_scene_parse_text(text, text_len, nullptr, &n, nullptr, nullptr, 0);  // size query
+  std::vector<uint8_t> ir(n);
+  if (n == 0 || mm3e_scene_parse_text(text, text_len, ir.data(), &n, &cam, err, errlen) != MM3E_OK) {
+    if (n == 0) mm3e_scene_parse_text(text, text_len, nullptr, &n, nullptr, err, errlen);  // the parse message
+    return MM3E_ERR_INVALID;
+  }
+  ir.resize(n);
+  mm3e_ir_header hdr;
+  std::memcpy(&hdr, ir.data(), sizeof hdr);
+  if (w == 0 && h == 0) {
+    w = hdr.width;
+    h = hdr.height;
+  }
+  std::string key(reinterpret_cast<const char*>(ir.data()), ir.size());
+  uint32_t sid = 0;
+  bool known = false;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->text_scenes.find(key);
+    if (it != S->text_scenes.end()) {
+      sid = it->second;
+      known = true;
+    }
+  }
+  if (!known) {  // compile outside the server lock (tables, device GI bake, generated kernel)
+    mm3e_scene* sc = nullptr;
+    const int rc = mm3e_scene_compile(S->r, ir.data(), ir.size(), hdr.width, hdr.height, &sc, err, errlen);
+    if (rc != MM3E_OK) return rc;
+    if (sc->host.hdr.mode != 0) {
+      mm3e_scene_destroy(sc);
+      mm3e::set_err(err, errlen, "debug render modes are rendered per frame, not served");
+      return MM3E_ERR_INVALID;
+    }
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->text_scenes.find(key);
+    if (it != S->text_scenes.end()) {  // another thread compiled the same scene meanwhile
+      sid = it->second;
+      S->owned.push_back(sc);           // released with the server
+    } else {
+      S->scenes.push_back(sc);
+      S->owned.push_back(sc);
+      sid = (uint32_t)(S->scenes.size() - 1);
+      S->text_scenes.emplace(std::move(key), sid);
+    }
+  }
+  return mm3e_server_submit(S, sid, &cam, w, h, out, request_id);
+}
+
 int mm3e_server_poll(mm3e_server* S, uint64_t request_id, int* done) {
   if (!S || !done) return MM3E_ERR_INVALID;
   std::lock_guard<std::mutex> lk(S->mu);
@@ -449,6 +506,7 @@ void mm3e_server_destroy(mm3e_server* S) {
   }
   for (auto& kv : S->pinned_free) hipHostFree(kv.second);
   if (S->d_slab) hipFree(S->d_slab);
+  for (mm3e_scene* sc : S->owned) mm3e_scene_destroy(sc);
   delete S;
 }
 

@@ -8,6 +8,7 @@ The library is the HIP/gfx950 render path; there is no CPU fallback, and constru
 from __future__ import annotations
 
 import ctypes as C
+import struct
 import os
 from typing import Optional, Sequence
 
@@ -34,6 +35,7 @@ EXPORTS = (
     "mm3e_mesh_baker_read", "mm3e_mesh_baker_destroy", "mm3e_sdfv_encode", "mm3e_sdfv_decode",
     "mm3e_scene_render_film", "mm3e_gframe_create", "mm3e_gframe_destroy", "mm3e_gframe_upload",
     "mm3e_gframe_read", "mm3e_scene_render_gbuffer", "mm3e_gframe_reproject", "mm3e_scene_reproject_hybrid",
+    "mm3e_server_submit_text",
 )
 
 CODEGEN_OFF, CODEGEN_BACKGROUND, CODEGEN_SYNC = 0, 1, 2
@@ -56,6 +58,8 @@ def lib():
         L = C.CDLL(LIB_PATH)
         P = C.c_void_p
         L.mm3e_renderer_create.argtypes = [C.c_char_p, C.POINTER(P), C.c_char_p, C.c_size_t]
+        L.mm3e_server_submit_text.argtypes = [P, C.c_char_p, C.c_size_t, C.c_uint32, C.c_uint32, P,
+                                              C.POINTER(C.c_uint64), C.c_char_p, C.c_size_t]
         L.mm3e_renderer_adapter_name.argtypes = [P]
         L.mm3e_renderer_adapter_name.restype = C.c_char_p
         L.mm3e_renderer_wait_idle.argtypes = [P]
@@ -801,6 +805,22 @@ class Server:
         rc = lib().mm3e_server_submit(self.h, scene_id, _p(_cam(cam)), w, h, _p(out), C.byref(rid))
         if rc != 0:
             raise Mm3eError(rc, "submit refused (queue full)" if rc == 3 else "submit failed")
+        self._outs[rid.value] = out
+        return rid.value
+
+    def submit_text(self, text: str, w: int = 0, h: int = 0, keep_image=True) -> int:
+        """A request as `.mm3e` text (scene + camera); compiled once per distinct scene and kept
+        resident.  w = h = 0 renders at the text's `size`."""
+        b = text.encode()
+        ir, _ = parse_text(text)
+        hw = struct.unpack_from("<II", ir, 16)
+        ww, hh = (w, h) if (w or h) else hw
+        out = np.zeros((hh, ww, 4), np.uint8) if keep_image else None
+        rid = C.c_uint64()
+        err = C.create_string_buffer(1024)
+        rc = lib().mm3e_server_submit_text(self.h, b, len(b), w, h, _p(out), C.byref(rid), err, 1024)
+        if rc != 0:
+            raise Mm3eError(rc, err.value.decode() or "submit_text failed")
         self._outs[rid.value] = out
         return rid.value
 

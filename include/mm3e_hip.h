@@ -213,12 +213,18 @@ int mm3e_scene_emit_source(const void* scene_ir, size_t ir_len, char* src, size_
 /* ---- the `.mm3e` text scene format (mm3e-orchestrator/src/scene_io.rs) --------------------
  * parse (scene_io.rs:200-331): text -> scene IR + camera; errors carry the reference's
  * messages ("line N: ...").  With ir == NULL or *ir_len too small, *ir_len is set to the size
- * needed and MM3E_ERR_INVALID is returned.  The format holds no CSG, volumes, GI, shadow/AO
- * budgets or normal_h; those take Scene::new's values. */
+ * needed and MM3E_ERR_INVALID is returned.  The reference's format drops CSG, volumes, GI, the
+ * shadow/AO budgets, LOD, the march boosts and normal_h (scene_io.rs:47-62, 103-105); this
+ * library extends it losslessly with directives written only when a scene needs them
+ * (sun_raw, marcher_ext, gi bake|baked, volume sdfv|raw, csgnode, csgexpr, obj csg, obj volume;
+ * binary payloads in base64), so IR -> text -> IR returns the same bytes and a scene inside the
+ * reference's subset is written exactly in the reference's format. */
 int mm3e_scene_parse_text(const char* text, size_t text_len, uint8_t* ir, size_t* ir_len, mm3e_camera* cam,
                           char* err, size_t errlen);
 /* try_serialize (scene_io.rs:32-164): scene IR + camera -> text (numbers as the shortest
- * decimal that reads back to the same f32), NUL-terminated; same size protocol as above. */
+ * decimal that reads back to the same f32, plus the extension directives above), NUL-terminated;
+ * same size protocol as above.  Texture appearance and triangle surfaces are refused with the
+ * reference's messages. */
 int mm3e_scene_to_text(const void* scene_ir, size_t ir_len, const mm3e_camera* cam, char* text, size_t* text_len,
                        char* err, size_t errlen);
 
@@ -372,6 +378,14 @@ int mm3e_server_add_scene(mm3e_server* s, mm3e_scene* scene, uint32_t* scene_id)
  * with MM3E_ERR_BUSY. */
 int mm3e_server_submit(mm3e_server* s, uint32_t scene_id, const mm3e_camera* cam, uint32_t w, uint32_t h, uint8_t* out,
                        uint64_t* request_id);
+/* A request given as `.mm3e` scene text (scene_io.rs:200-333 with this library's lossless
+ * extension, see mm3e_scene_parse_text): the text is parsed (scene + `cam` line), compiled once
+ * and kept resident by the server under its IR bytes (later requests with the same scene skip
+ * the compile, whatever their camera), then queued as mm3e_server_submit at (w, h), or at the
+ * text's `size` when both are 0.  Parse and compile errors are copied into err; the compile
+ * (tables, device GI bake, generated kernel) runs on the calling thread. */
+int mm3e_server_submit_text(mm3e_server* s, const char* text, size_t text_len, uint32_t w, uint32_t h, uint8_t* out,
+                            uint64_t* request_id, char* err, size_t errlen);
 int mm3e_server_poll(mm3e_server* s, uint64_t request_id, int* done);
 /* Block until the request completes; returns its status and releases it.  Each request id may
  * be waited for once: a second wait on an id (while the first is blocked, or after it returned)

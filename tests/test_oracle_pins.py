@@ -403,3 +403,23 @@ def test_point_light_scene_device(renderer, height):
     finally:
         g.close()
     assert np.abs(hdr[y, x] - want).max() <= rel * want.max() + 1e-6, (hdr[y, x], want)
+
+
+# ---- lib.rs:1223-1243 shade_pixel: the aa x aa grid at ((s + 0.5) / n) --------------------
+
+def test_aa_grid_equals_a_box_filtered_supersampled_frame():
+    """Sample (sx, sy) of pixel (x, y) at aa = n is the ray through (x + (sx + 0.5)/n,
+    y + (sy + 0.5)/n): the centre of pixel (n x + sx, n y + sy) of the frame n times larger at
+    aa = 1.  So the aa = 2 frame is the 2x2 box filter of the aa = 1 frame at twice the size,
+    up to the rounding of the ray's screen coordinate (samples off the grid, e.g. at s/n, or a
+    mean over the wrong count, miss by far more)."""
+    sc, cam = configs.c1_scene(width=40, height=24)
+    sc.aa = 2
+    sc.post.bloom = False
+    small = po.OracleScene(sc.to_ir()).render_linear(cam).astype(np.float64)
+    sc.aa = 1
+    sc.width, sc.height = 80, 48
+    big = po.OracleScene(sc.to_ir()).render_linear(cam).astype(np.float64)
+    box = 0.25 * (big[0::2, 0::2] + big[0::2, 1::2] + big[1::2, 0::2] + big[1::2, 1::2])
+    err = np.abs(small - box).max(axis=-1)
+    assert np.percentile(err, 99) < 1e-5 and np.mean(err < 1e-5) > 0.99, (np.percentile(err, 99), err.max())

@@ -100,3 +100,119 @@ def test_parsed_scene_renders_like_the_original():
     ir2, _ = hip.parse_text(hip.to_text(p.to_ir(), c2))
     assert np.array_equal(po.OracleScene(p.to_ir(), threads=4).render_rgba(c2, threads=4),
                           po.OracleScene(ir2, threads=4).render_rgba(c2, threads=4))
+
+
+# ---- the lossless extension (CSG, volumes, GI, marcher budgets, normal_h) -----------------
+
+def _baked_gi_scene():
+    sc, _ = configs.c2_scene(width=40, height=24, gi_dims=(3, 3, 3), gi_rays=1)
+    g = po.OracleScene(sc.to_ir()).gi_raw()
+    sc.set_baked_gi(g[0], g[1], g[2], g[3])
+    return sc
+
+
+def _ext_cases():
+    c1, cam1 = configs.c1_scene()
+    c2, cam2 = configs.c2_scene()
+    c5, cam5 = configs.c5_scene()
+    csg, camc = configs.csg_scene()
+    knobs, _ = configs.c1_scene(width=64, height=36)
+    knobs.marcher.shadow_steps, knobs.marcher.ao_samples = 17, 3
+    knobs.marcher.lod_footprint, knobs.marcher.subitize, knobs.marcher.secant = 0.002, 0.6, True
+    knobs.marcher.normal_h = 0.0013
+    knobs.sun_dir = (0.3, 0.8, 0.2)  # not unit length: kept as stored
+    return {"c1": (c1, cam1), "c2": (c2, cam2), "c5": (c5, cam5), "csg": (csg, camc), "knobs": (knobs, cam1),
+            "baked_gi": (_baked_gi_scene(), cam2)}
+
+
+@pytest.mark.parametrize("name", ["c1", "c2", "c5", "csg", "knobs", "baked_gi"])
+def test_ir_text_ir_is_bit_identical(name):
+    """IR -> text -> IR returns the same bytes for the benchmark scenes (c2's GI bake request,
+    c5's 128^3 volume and GI request), object-local CSG, a baked GI volume and every marcher
+    knob; text -> IR -> text is stable too."""
+    sc, c = _ext_cases()[name]
+    ir0 = sc.to_ir()
+    text = hip.to_text(ir0, c)
+    ir1, _ = hip.parse_text(text)
+    assert ir1 == ir0
+    assert hip.to_text(ir1, c) == text
+
+
+def test_reference_subset_is_written_in_the_reference_format():
+    text = hip.to_text(demo_scene().to_ir(), cam())
+    for ext in ("sun_raw", "marcher_ext", "\ngi ", "\nvolume ", "csgnode", "csgexpr", "obj csg", "obj volume"):
+        assert ext not in text
+
+
+def test_extension_directives_in_text():
+    sc, c = _ext_cases()["c5"]
+    text = hip.to_text(sc.to_ir(), c)
+    assert "\ngi bake 32 32 32 5\n" in text and "\nvolume sdfv " in text and "\nobj volume 0 " in text
+    assert "\nmarcher_ext 64 5 0 0 0 " in text  # normal_h raised by the volume (lib.rs:631-638)
+    sc, c = _ext_cases()["csg"]
+    text = hip.to_text(sc.to_ir(), c)
+    assert text.count("\ncsgexpr ") == len(sc.csg_roots) and text.count("\nobj csg ") == 4
+
+
+@pytest.mark.parametrize("text,msg", [
+    ("gi sideways 1 1 1 1\n", "line 1: unknown gi kind 'sideways'"),
+    ("gi baked 1 1 1 0 0 0 1 1 1 AAAA\n", "line 1: gi cubes must be base64 of nx*ny*nz*18 f32"),
+    ("volume raw 2 2 2 0 0 0 1 1 1 !!!!\n", "line 1: volume samples must be base64 of nx*ny*nz f32"),
+    ("volume sdfv QUJD\n", "line 1: sdfv buffer is shorter than the header"),
+    ("volume cube 1\n", "line 1: unknown volume kind 'cube'"),
+    ("csgnode 0 0 0 0 1\n", "line 1: expected csg p"),
+])
+def test_extension_rejects_garbage(text, msg):
+    with pytest.raises(hip.Mm3eError) as e:
+        hip.parse_text(text)
+    assert str(e.value) == msg
+
+
+def test_extension_ids_are_checked_at_compile():
+    """A csg id without a registered expression parses (ids resolve against the whole file) and
+    is then refused by the compile-time validation, as for an IR built any other way."""
+    ir, _ = hip.parse_text("size 8 8\nobj csg 0 mat 0\n")
+    assert hip.scene_check(ir, 8, 8) == "unregistered CSG expression"
+
+
+def test_text_scene_with_csg_and_volume_renders_like_the_original():
+    sc, c = configs.csg_scene(48, 32)
+    ir0 = sc.to_ir()
+    ir1, _ = hip.parse_text(hip.to_text(ir0, c))
+    assert np.array_equal(po.OracleScene(ir0, threads=4).render_rgba(c, threads=4),
+                          po.OracleScene(ir1, threads=4).render_rgba(c, threads=4))
+
+
+@pytest.mark.gpu
+def test_server_text_requests_equal_compiled_ir_renders(renderer):
+    """Requests given as `.mm3e` text (GI bake request, object-local CSG, a sampled volume) are
+    served like requests on a compiled IR: the image equals mm3e_scene_render_rgba of the parsed
+    IR and camera, bit for bit; a second request with the same scene and another camera reuses
+    the resident compiled scene."""
+    c2, cam2 = configs.c2_scene(width=240, height=135, gi_dims=(4, 4, 4), gi_rays=5)
+    csg, camc = configs.csg_scene(160, 90)
+    c5, cam5 = configs.c5_scene(width=160, height=90, n_prims=60, gi_dims=(4, 4, 4), vol_n=32)
+    srv = hip.Server(renderer, hbm_budget_bytes=128 << 20)
+    try:
+        for sc, c in ((c2, cam2), (csg, camc), (c5, cam5)):
+            text = hip.to_text(sc.to_ir(), c)
+            ir, pc = hip.parse_text(text)
+            assert ir == sc.to_ir()
+            g = renderer.compile(ir, sc.width, sc.height)
+            try:
+                ref = g.render_rgba(pc)
+                img, _ = srv.wait(srv.submit_text(text))
+                assert (img == ref).all()
+                # the same scene from another viewpoint: only the cam line changes
+                other = configs.orbit_camera((0.0, 1.0, 0.0), 9.0, 0.7, 0.3, math.radians(50.0))
+                text2 = hip.to_text(ir, other)
+                ir2, pc2 = hip.parse_text(text2)
+                img2, _ = srv.wait(srv.submit_text(text2))
+                assert (img2 == g.render_rgba(pc2)).all()
+            finally:
+                g.close()
+        with pytest.raises(hip.Mm3eError) as e:
+            srv.submit_text("size 8 8\nobj banana 1\n")
+        assert "unknown primitive 'banana'" in str(e.value)
+    finally:
+        srv.close()

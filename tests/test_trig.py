@@ -1,0 +1,79 @@
+"""The device's sin/cos for twist, bend and the shadow-jitter hash (csrc/cr_trig.h), checked on
+the host: the header is plain C++ whose every step is one IEEE f64 operation in source order,
+so the host build computes exactly what the device computes.  Over a stride-5 sample of all f32
+bit patterns with |x| < 2^20 (about 5e8 inputs per function) it must equal the host libm's
+f64 sin / cos rounded once to f32 wherever that is decidable (mismatches only within 1 ulp,
+at most a handful: both sides round an f64 approximation; none were found), and it is compared
+with the f32 sinf / cosf the oracle calls (the reference's Rust f32 calls the same libm)."""
+import os
+import subprocess
+import tempfile
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HDR = os.path.join(ROOT, "atom-3d-engine_amd", "csrc")
+
+SRC = r"""
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+#include "cr_trig.h"
+static float bits(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+static int64_t ulps(float a, float b) {
+  int32_t x, y; std::memcpy(&x, &a, 4); std::memcpy(&y, &b, 4);
+  if ((x < 0) != (y < 0)) return (a == b) ? 0 : 1 << 30;
+  return x > y ? (int64_t)x - y : (int64_t)y - x;
+}
+int main(int argc, char** argv) {
+  const uint32_t stride = (uint32_t)atoi(argv[1]);
+  const unsigned nt = std::thread::hardware_concurrency();
+  const uint32_t lim = 0x49800000u;  // 2^20
+  std::atomic<uint64_t> n{0}, mis_cr{0}, far_cr{0}, mis_f{0}, far_f{0};
+  std::vector<std::thread> ts;
+  for (unsigned t = 0; t < nt; t++) ts.emplace_back([&, t]() {
+    uint64_t a = 0, b = 0, c = 0, d = 0, e = 0;
+    for (uint64_t u = (uint64_t)t * stride; u < lim; u += (uint64_t)nt * stride) {
+      for (int sign = 0; sign < 2; sign++) {
+        const float x = bits((uint32_t)u | (sign ? 0x80000000u : 0u));
+        float s, co;
+        mm3e::cr_sincos(x, s, co);
+        const float rs = (float)std::sin((double)x), rc = (float)std::cos((double)x);
+        const float fs = sinf(x), fc = cosf(x);
+        a += 2;
+        b += (s != rs) + (co != rc);
+        c += (ulps(s, rs) > 1) + (ulps(co, rc) > 1);
+        d += (s != fs) + (co != fc);
+        e += (ulps(s, fs) > 1) + (ulps(co, fc) > 1);
+      }
+    }
+    n += a; mis_cr += b; far_cr += c; mis_f += d; far_f += e;
+  });
+  for (auto& t : ts) t.join();
+  std::printf("%llu %llu %llu %llu %llu\n", (unsigned long long)n.load(), (unsigned long long)mis_cr.load(),
+              (unsigned long long)far_cr.load(), (unsigned long long)mis_f.load(), (unsigned long long)far_f.load());
+  return 0;
+}
+"""
+
+
+def test_sincos_matches_libm_rounded_once():
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "trig.cpp")
+        exe = os.path.join(d, "trig")
+        with open(src, "w") as f:
+            f.write(SRC)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-pthread",
+                        "-I" + HDR, src, "-o", exe], check=True)
+        out = subprocess.run([exe, "5"], check=True, capture_output=True, text=True).stdout.split()
+    n, mis_cr, far_cr, mis_f, far_f = (int(x) for x in out)
+    print(f"\n[cr_sincos] {n} values: vs libm f64 rounded once {mis_cr} differ ({far_cr} by > 1 ulp); "
+          f"vs libm f32 sinf/cosf {mis_f} differ ({far_f} by > 1 ulp)", flush=True)
+    assert n > 4e8
+    assert far_cr == 0 and mis_cr <= 16
+    # glibc's f32 sinf/cosf are faithful, not correctly rounded: 0.21 % of these inputs differ
+    # from the rounded f64 value, always by one ulp (the last few pixels of device/oracle
+    # differences in twisted, bent or shadow-jittered shading)
+    assert far_f == 0 and mis_f <= n * 5e-3
