@@ -386,7 +386,21 @@ static bool launch_queued(mm3e_scene* s, const DevScene* dsc, const FrameParams&
   return hip_ok(launch_wf_resolve(st, fp, w, s->aa, s->d_hdr), "queued resolve launch", &msg);
 }
 
-bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std::string& msg) {
+// Where a frame's kernels write (HDR radiance, bloom intermediate, RGBA8) when not into the
+// scene's own framebuffers.  Such frames take neither the tile-order nor the queued path, which
+// use per-scene buffers allocated on first use: one kernel per frame in tile index order.
+struct FrameTargets {
+  float* hdr;
+  float* tmp;
+  uchar4* rgba;
+};
+
+bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std::string& msg,
+                  const FrameTargets* tg = nullptr) {
+  float* const d_hdr = tg ? tg->hdr : s->d_hdr;
+  float* const d_tmp = tg ? tg->tmp : s->d_tmp;
+  uchar4* const d_rgba = tg ? tg->rgba : s->d_rgba;
+  const bool to_on = s->to_on && !tg;
   hipStream_t st = s->r->stream;
   const uint32_t stage = dsc == s->d_scene ? s->stage_bytes : s->dyn_stage_bytes;
   const bool general = dsc == s->d_scene ? s->general : s->dyn_general;
@@ -395,7 +409,7 @@ bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std
   hipEvent_t* ev = s->ev[s->frames % mm3e_scene::kRing];
   hipEventRecord(ev[0], st);
   if (s->host.hdr.mode != 0) {  // debug render mode: straight to RGBA8, no post (lib.rs:873-878)
-    if (!hip_ok(launch_aov_mode_table(st, dsc, fp, s->d_rgba, s->host.hdr.mode), "render-mode launch", &msg))
+    if (!hip_ok(launch_aov_mode_table(st, dsc, fp, d_rgba, s->host.hdr.mode), "render-mode launch", &msg))
       return false;
     hipEventRecord(ev[1], st);
     hipEventRecord(ev[2], st);
@@ -407,14 +421,15 @@ bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std
   bool ordered = false;
   uint32_t wf_b, wf_n, wf_c;
   wf_settings(s, &wf_b, &wf_n, &wf_c);
+  if (tg) wf_b = 0;
   if (wf_b > 0) {
     if (!launch_queued(s, dsc, fp, grid, block, stage, general, msg)) return false;
   } else if (dsc == s->d_scene && gj && gj->state.load(std::memory_order_acquire) == MM3E_CODEGEN_READY) {
     const DevScene* sc_arg = dsc;
     FrameParams fp_arg = fp;
-    float* hdr_arg = s->d_hdr;
+    float* hdr_arg = d_hdr;
     uint32_t tw = s->tw, th = s->th, gstage = s->gen_stage;
-    if (gj->fn_o && s->to_on && tile_order_ready(s, grid.x * grid.y, msg)) {
+    if (gj->fn_o && to_on && tile_order_ready(s, grid.x * grid.y, msg)) {
       TileOrder o = scene_tile_order(s, grid.x);
       void* args[] = {&sc_arg, &fp_arg, &hdr_arg, &tw, &th, &gstage, &o};
       if (!hip_ok(hipModuleLaunchKernel(gj->fn_o, s->to_tiles, 1, 1, block.x, 1, 1, gstage, st, args, nullptr),
@@ -427,19 +442,19 @@ bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std
                   "generated render launch", &msg))
         return false;
     }
-  } else if (dsc == s->d_scene && s->to_on && tile_order_ready(s, grid.x * grid.y, msg)) {
+  } else if (dsc == s->d_scene && to_on && tile_order_ready(s, grid.x * grid.y, msg)) {
     TileOrder o = scene_tile_order(s, grid.x);
-    if (!hip_ok(launch_render_ordered(s->bounces, s->to_tiles, block, st, dsc, stage, general, fp, s->d_hdr, s->tw,
+    if (!hip_ok(launch_render_ordered(s->bounces, s->to_tiles, block, st, dsc, stage, general, fp, d_hdr, s->tw,
                                       s->th, o),
                 "render launch", &msg))
       return false;
     ordered = true;
-  } else if (!hip_ok(launch_render_table(s->bounces, grid, block, st, dsc, stage, general, fp, s->d_hdr, s->tw, s->th),
+  } else if (!hip_ok(launch_render_table(s->bounces, grid, block, st, dsc, stage, general, fp, d_hdr, s->tw, s->th),
                      "render launch", &msg)) {
     return false;
   }
   hipEventRecord(ev[1], st);
-  if (!hip_ok(launch_post(st, s->pp, s->d_kern, s->d_hdr, s->d_tmp, s->d_rgba), "post launch", &msg)) return false;
+  if (!hip_ok(launch_post(st, s->pp, s->d_kern, d_hdr, d_tmp, d_rgba), "post launch", &msg)) return false;
   hipEventRecord(ev[2], st);
   if (ordered && !hip_ok(launch_tile_order(st, s->d_to + s->to_tiles, s->to_tiles, s->d_to), "tile order launch", &msg))
     return false;
@@ -451,6 +466,39 @@ bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std
 }  // namespace mm3e
 
 using namespace mm3e;
+
+// ---- frames that allocate nothing -------------------------------------------------------
+// A fixed device array holds the HDR radiance and the bloom intermediate of frames rendered
+// by mm3e_scene_render_rgba_noalloc (12 + 12 bytes per pixel).  Each GPU the library runs on
+// holds its own copy until the process ends; in host code `scratch` is only a handle for
+// hipGetSymbolAddress, which yields the address of the copy of the current device.
+static constexpr size_t kNoallocScratchBytes = (size_t)MM3E_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) uint8_t scratch[kNoallocScratchBytes];
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// Binds the calling thread's current device's copy of `scratch` (once per device).
+static bool bind_scratch(int dev, std::string& msg) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    msg = "device index out of range for the frame scratch";
+    return false;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return true;
+  void* ptr = nullptr;
+  if (!hip_ok(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)", &msg)) return false;
+  if (!ptr) {
+    msg = "hipGetSymbolAddress(scratch) returned no address";
+    return false;
+  }
+  g_scratch_ptr[dev] = ptr;
+  return true;
+}
+static void* scratch_of(int dev) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return dev >= 0 && dev < kMaxDevices ? g_scratch_ptr[dev] : nullptr;
+}
 
 extern "C" {
 
@@ -537,7 +585,8 @@ int mm3e_renderer_create(const char* adapter_filter, mm3e_renderer** out, char* 
   // the copy to 56 KiB so three 256-lane blocks still fit one CU.
   r->lds_budget = (uint32_t)std::min<size_t>(56 * 1024, p.sharedMemPerBlock > 4096 ? p.sharedMemPerBlock - 4096 : 0);
   std::string msg;
-  if (!hip_ok(hipSetDevice(dev), "hipSetDevice", &msg) || !hip_ok(hipStreamCreate(&r->stream), "hipStreamCreate", &msg)) {
+  if (!hip_ok(hipSetDevice(dev), "hipSetDevice", &msg) || !hip_ok(hipStreamCreate(&r->stream), "hipStreamCreate", &msg) ||
+      !bind_scratch(dev, msg)) {
     set_err(err, errlen, "request_device failed: " + msg);
     delete r;
     return MM3E_ERR_DEVICE;
@@ -748,6 +797,42 @@ static bool dyn_scene(mm3e_scene* s, const mm3e_dyn_sphere* dyn, uint32_t n_dyn,
   if (!upload_tables(s, xo, xm, &s->d_dyn, &s->dyn_cap, nullptr, msg)) return false;
   *out = reinterpret_cast<const DevScene*>(s->d_dyn);
   return true;
+}
+
+int mm3e_scene_render_rgba_noalloc(mm3e_scene* s, const mm3e_camera* cam, void* rgba_dev, size_t rgba_len,
+                                   char* err, size_t errlen) {
+  if (!s || validate_cam(cam) || !rgba_dev) {
+    set_err(err, errlen, "render_rgba_noalloc: null scene, camera or output, or a non-finite camera");
+    return MM3E_ERR_INVALID;
+  }
+  const uint64_t npx = (uint64_t)s->w * s->h;
+  if (rgba_len < npx * 4) {
+    set_err(err, errlen, "render_rgba_noalloc: output buffer smaller than width*height*4 bytes");
+    return MM3E_ERR_INVALID;
+  }
+  const uint64_t hdr_bytes = (npx * 12 + 255) & ~uint64_t(255);  // the bloom intermediate starts 256-aligned
+  if (hdr_bytes + npx * 12 > kNoallocScratchBytes) {
+    set_err(err, errlen, "render_rgba_noalloc: " + std::to_string(s->w) + "x" + std::to_string(s->h) +
+                             " needs " + std::to_string(npx * 24) + " bytes of frame scratch; the maximum is " +
+                             std::to_string(kNoallocScratchBytes) + " (" +
+                             std::to_string(kNoallocScratchBytes / 2 / 12) + " pixels)");
+    return MM3E_ERR_INVALID;
+  }
+  std::lock_guard<std::mutex> lk(s->r->mu);
+  hipSetDevice(s->r->device);
+  uint8_t* base = static_cast<uint8_t*>(scratch_of(s->r->device));
+  if (!base) {
+    set_err(err, errlen, "render_rgba_noalloc: the frame scratch is not bound on this device");
+    return MM3E_ERR_DEVICE;
+  }
+  FrameTargets tg{reinterpret_cast<float*>(base), reinterpret_cast<float*>(base + hdr_bytes),
+                  static_cast<uchar4*>(rgba_dev)};
+  std::string msg;
+  if (!launch_frame(s, s->d_scene, make_frame(*cam, s->w, s->h), msg, &tg)) {
+    set_err(err, errlen, "render_rgba_noalloc: " + msg);
+    return MM3E_ERR_DEVICE;
+  }
+  return MM3E_OK;
 }
 
 int mm3e_scene_render_only(mm3e_scene* s, const mm3e_camera* cam) {

@@ -35,7 +35,7 @@ EXPORTS = (
     "mm3e_mesh_baker_read", "mm3e_mesh_baker_destroy", "mm3e_sdfv_encode", "mm3e_sdfv_decode",
     "mm3e_scene_render_film", "mm3e_gframe_create", "mm3e_gframe_destroy", "mm3e_gframe_upload",
     "mm3e_gframe_read", "mm3e_scene_render_gbuffer", "mm3e_gframe_reproject", "mm3e_scene_reproject_hybrid",
-    "mm3e_server_submit_text",
+    "mm3e_scene_render_rgba_noalloc", "mm3e_server_submit_text",
 )
 
 CODEGEN_OFF, CODEGEN_BACKGROUND, CODEGEN_SYNC = 0, 1, 2
@@ -58,6 +58,7 @@ def lib():
         L = C.CDLL(LIB_PATH)
         P = C.c_void_p
         L.mm3e_renderer_create.argtypes = [C.c_char_p, C.POINTER(P), C.c_char_p, C.c_size_t]
+        L.mm3e_scene_render_rgba_noalloc.argtypes = [P, P, P, C.c_size_t, C.c_char_p, C.c_size_t]
         L.mm3e_server_submit_text.argtypes = [P, C.c_char_p, C.c_size_t, C.c_uint32, C.c_uint32, P,
                                               C.POINTER(C.c_uint64), C.c_char_p, C.c_size_t]
         L.mm3e_renderer_adapter_name.argtypes = [P]
@@ -552,6 +553,14 @@ class GpuScene:
         if rc != 0:
             raise Mm3eError(rc, "scene autotune failed: " + rep.value.decode(errors="replace"))
         return rep.value.decode()
+
+    def render_rgba_noalloc(self, cam, rgba_dev: int, nbytes: int):
+        """mm3e_scene_render_rgba_noalloc: enqueue a frame into the device buffer at address
+        rgba_dev (w*h*4 bytes); the library allocates nothing.  Does not wait."""
+        err = C.create_string_buffer(512)
+        rc = lib().mm3e_scene_render_rgba_noalloc(self.h, _p(_cam(cam)), C.c_void_p(rgba_dev), nbytes, err, 512)
+        if rc != 0:
+            raise Mm3eError(rc, err.value.decode())
 
     def render_only(self, cam):
         rc = lib().mm3e_scene_render_only(self.h, _p(_cam(cam)))

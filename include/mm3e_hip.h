@@ -99,6 +99,21 @@ int mm3e_scene_render_linear(mm3e_scene* s, const mm3e_camera* cam, float* out, 
  * miss), camera-forward depth (+inf on a miss), march steps.  Any pointer may be NULL. */
 int mm3e_scene_render_aov(mm3e_scene* s, const mm3e_camera* cam, uint8_t* hit, uint32_t* mat, float* depth,
                           uint32_t* steps);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): one frame of the
+ * compiled scene, as mm3e_scene_render_rgba renders it, written as RGBA8 (w*h*4 bytes,
+ * top-down, alpha 255) into the caller's DEVICE buffer rgba_dev.  The frame's HDR radiance and
+ * bloom intermediate (24 bytes per pixel) live in a fixed device array of
+ * MM3E_NOALLOC_SCRATCH_BYTES that the library holds on each GPU it runs on until the process
+ * ends; larger frames are rejected with a message (at most 2,796,202 pixels, e.g. 1920x1080).
+ * Calls no allocator.  Enqueued on the renderer's stream without waiting
+ * (mm3e_renderer_wait_idle waits).  All calls on one GPU share that array and it keeps the last
+ * frame's contents: calls that overlap on one GPU (renderers on other streams) race.  The frame
+ * is one kernel in tile index order (no tile order, no queued stages); dynamic spheres are not
+ * taken; debug render modes write their image directly. */
+#define MM3E_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int mm3e_scene_render_rgba_noalloc(mm3e_scene* s, const mm3e_camera* cam, void* rgba_dev, size_t rgba_len, char* err,
+                                   size_t errlen);
 /* Thin lens (Lens, mm3e-kit/src/camera.rs:14-35): the eye is the lens centre; the focus plane
  * is perpendicular to the camera's forward axis at focus_distance.  Radius 0 = pinhole. */
 typedef struct mm3e_lens {
