@@ -343,8 +343,60 @@ def star_volume(seed: int, n: int = 128, r0: float = 0.8, half: float = 1.3) -> 
     return Volume((n, n, n), (-half, -half, -half), (cell, cell, cell), d.astype(np.float32))
 
 
+def real_sh(l: int, m: int, theta: np.ndarray, phi: np.ndarray) -> np.ndarray:
+    """Real spherical harmonic Y_l^m (orthonormal; theta polar, phi azimuth) in float64."""
+    from scipy.special import lpmv
+    am = abs(m)
+    norm = math.sqrt((2 * l + 1) / (4 * math.pi) * math.factorial(l - am) / math.factorial(l + am))
+    base = norm * lpmv(am, l, np.cos(theta))
+    if m == 0:
+        return base
+    return math.sqrt(2.0) * base * (np.cos(am * phi) if m > 0 else np.sin(am * phi))
+
+
+def star_mesh(seed: int, r0: float = 0.8, n_lat: int = 64, n_lon: int = 128):
+    """A seeded star-shaped closed triangle mesh r(theta, phi) = r0 (1 + sum_i a_i Y_i(theta, phi))
+    over five real spherical harmonics with 2 <= l <= 4 (a_i in [0.05, 0.12], so r stays
+    positive): a latitude-longitude grid of n_lat bands and n_lon sectors with one vertex at each
+    pole.  Returns (positions[n][3] f32, triangles[t][3] u32)."""
+    rng = SplitMix64(seed)
+    terms = []
+    for _ in range(5):
+        l = rng.randint(2, 4)
+        m = rng.randint(-l, l)
+        terms.append((l, m, rng.uniform(0.05, 0.12)))
+    th = np.concatenate([[0.0], np.arange(1, n_lat) * (math.pi / n_lat), [math.pi]])
+    ph = np.arange(n_lon) * (2 * math.pi / n_lon)
+    T, P = np.meshgrid(th[1:-1], ph, indexing="ij")
+    pts_t = np.concatenate([[0.0], T.reshape(-1), [math.pi]])
+    pts_p = np.concatenate([[0.0], P.reshape(-1), [0.0]])
+    shape = np.ones_like(pts_t)
+    for l, m, a in terms:
+        shape += a * real_sh(l, m, pts_t, pts_p)
+    rad = r0 * shape
+    pos = np.stack([rad * np.sin(pts_t) * np.cos(pts_p), rad * np.cos(pts_t), rad * np.sin(pts_t) * np.sin(pts_p)],
+                   -1).astype(np.float32)
+    ring = lambda i, j: 1 + (i - 1) * n_lon + (j % n_lon)  # noqa: E731  vertex of band ring i (1..n_lat-1)
+    south = 1 + (n_lat - 1) * n_lon
+    tri = []
+    for j in range(n_lon):
+        tri.append((0, ring(1, j + 1), ring(1, j)))
+        for i in range(1, n_lat - 1):
+            a, b, c, d = ring(i, j), ring(i, j + 1), ring(i + 1, j), ring(i + 1, j + 1)
+            tri.append((a, b, d))
+            tri.append((a, d, c))
+        tri.append((ring(n_lat - 1, j), ring(n_lat - 1, j + 1), south))
+    return pos, np.asarray(tri, np.uint32)
+
+
 def c5_scene(seed: int = 12, width: int = 3840, height: int = 2160, n_prims: int = 1000, gi_dims=(32, 32, 32),
-             vol_n: int = 128) -> Tuple[Scene, np.ndarray]:
+             vol_n: int = 128, bake=None) -> Tuple[Scene, np.ndarray]:
+    """The stress workload.  Its 128^3 grid is the signed distance of a seeded star-shaped closed
+    surface: star_mesh baked to an SDF grid with vol_n samples along the longest axis (padding
+    0.15, mesh::bake_sdf semantics) by `bake(positions, triangles, resolution, padding)`, which
+    returns (dims, min, cell, samples[nz][ny][nx]) — the library's device bake, or any bake with
+    the same result.  Without `bake` the grid is star_volume's analytic stand-in of the same
+    shape family."""
     sc = procedural_scene(seed, n_prims, width, height, extent=14.0)
     rng = SplitMix64(seed ^ 0x5A5A)
     sc.bounces = 2
@@ -353,7 +405,14 @@ def c5_scene(seed: int = 12, width: int = 3840, height: int = 2160, n_prims: int
     sc.objects[1].twist(0.8)
     sc.objects[2].bend(0.5)
     sc.marcher.step_scale = 0.6
-    vid = sc.volume(star_volume(seed, vol_n))
+    if bake is not None:
+        mpos, mtri = star_mesh(seed)
+        dims, lo, cell, data = bake(mpos, mtri, vol_n, 0.15)
+        vol = Volume(tuple(int(d) for d in dims), tuple(float(x) for x in lo), tuple(float(x) for x in cell),
+                     np.ascontiguousarray(data, np.float32).reshape(-1))
+    else:
+        vol = star_volume(seed, vol_n)
+    vid = sc.volume(vol)
     vm = sc.material(Material.solid((0.8, 0.3, 0.2)).roughness(0.35))
     sc.add(Object(Prim.volume(vid), Transform.at((0.0, 1.4, 0.0)).scaled(1.5), vm))
     if gi_dims is not None:

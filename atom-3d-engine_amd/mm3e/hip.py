@@ -501,6 +501,15 @@ class Renderer:
         return lay, out
 
 
+def device_bake(renderer):
+    """A `bake(positions, triangles, resolution, padding) -> (dims, min, cell, samples)` callable
+    running mesh::bake_sdf on the renderer's GPU (what configs.c5_scene takes)."""
+    def bake(positions, triangles, resolution, padding):
+        lay, out = renderer.bake_sdf(positions, triangles, resolution, padding)
+        return tuple(int(d) for d in lay.dims), tuple(lay.min), tuple(lay.cell), out
+    return bake
+
+
 class GpuScene:
     """GpuScene (mm3e-gpu/src/lib.rs:289-416)."""
 

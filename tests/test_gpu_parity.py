@@ -270,10 +270,17 @@ def test_csg_objects_image_and_field_parity(renderer):
 
 
 def test_c5_stress_scene_reduced_parity(renderer):
-    """The stress scene (1000 primitives, twist + bend, 128^3 sampled volume, 2 bounces, fog) at
-    320x180 with 8^3 GI probes so the CPU oracle finishes in seconds; the full 3840x2160 / 32^3
-    configuration is the bench's stress leg."""
-    sc, cam = configs.c5_scene(width=320, height=180, gi_dims=(8, 8, 8))
+    """The stress scene (1000 primitives, twist + bend, the star mesh's SDF grid baked on the
+    device, 2 bounces, fog) at 320x180 with 8^3 GI probes and the grid at 64 along its longest
+    axis, so the CPU oracle finishes in seconds; the device bake equals the oracle's bit for bit
+    and the frame meets the image bars.  The full 3840x2160 / 128 / 32^3 configuration is the
+    bench's stress leg."""
+    def oracle_bake(pos, tri, res, pad):
+        return po.mesh_bake(pos, tri, res, pad, threads=8)[:4]
+    sc, cam = configs.c5_scene(width=320, height=180, gi_dims=(8, 8, 8), vol_n=64, bake=hip.device_bake(renderer))
+    ref, _ = configs.c5_scene(width=320, height=180, gi_dims=(8, 8, 8), vol_n=64, bake=oracle_bake)
+    assert sc.volumes[0].dims == ref.volumes[0].dims
+    assert np.array_equal(sc.volumes[0].data.view(np.uint32), ref.volumes[0].data.view(np.uint32))
     g, o, _, _ = image_parity(renderer, sc, cam, "c5 reduced 320x180")
     info = g.info()
     assert info["objects"] == 1002 and info["grid_cells"] > 0
