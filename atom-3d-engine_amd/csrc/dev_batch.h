@@ -62,7 +62,9 @@ MM3E_DEV void batch_loop(const BatchJob* __restrict__ jobs, uint32_t n_jobs, uin
       if (q.order) s_start = atomicAdd(q.ticket, 1u);
     }
     __syncthreads();
-    const uint32_t tile = s_tile;
+    // the same value in every lane: made provably uniform so the job record, the scene pointer
+    // and the scene's wave-uniform table records are read with scalar loads
+    const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);
     __syncthreads();  // every lane has read s_tile before lane 0 may overwrite it
     if (tile >= total_tiles) break;  // uniform: the whole block leaves together
     const uint32_t j = find_job(jobs, n_jobs, tile, [](const BatchJob& b) { return b.tile_base; });

@@ -399,6 +399,27 @@ MM3E_DEV void run_end(Field& acc, bool& seeded, float bd, uint32_t bm) {
   seeded = true;
 }
 
+// A wave-uniform record of the scene's HBM tables (the segment a fold step visits, the objects
+// of a One segment, of a linear run and a run's unbounded members are the same for every lane)
+// read with scalar loads: the fields land in SGPRs, which VALU instructions read directly, and
+// only the fields a path uses are loaded.  `i` must be the same in every active lane.
+template <class T> MM3E_DEV T load_uniform(const T* table, uint32_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef __attribute__((address_space(4))) const T ConstT;
+  // the base may reach here in VGPRs (a scene pointer read from a batch job record): take it
+  // through readfirstlane as well, so the load's address is scalar.  The builtin returns a
+  // signed int, so each half is widened through uint32_t: a sign-extended low word would set
+  // the high word of every address whose bit 31 is set.
+  const uint64_t a = (uint64_t)(uintptr_t)table;
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+  const uint64_t base = (uint64_t)lo | ((uint64_t)hi << 32);
+  return ((ConstT*)(uintptr_t)base)[(uint32_t)__builtin_amdgcn_readfirstlane(i)];
+#else
+  return table[i];  // the host pass only parses device code
+#endif
+}
+
 // One segment of WorldPlan::eval (accel.rs:209-252) folded into `acc`.  Runs of plain unions
 // walk the same bounding-sphere tree in the same near-first order with the same `lower > best`
 // cut and earliest-index tie break as Bvh::query (accel.rs:79-119); instead of a 64-entry
@@ -407,9 +428,10 @@ MM3E_DEV void run_end(Field& acc, bool& seeded, float bd, uint32_t bm) {
 template <bool CSG>
 MM3E_DEV void seg_step(const DevScene& sc, const Tables& tb, V3 p, uint32_t si, Field& acc, bool& seeded) {
   {
-    const DSeg& seg = tb.seg[si];
+    const DSeg seg = load_uniform(sc.seg, si);
     if (seg.kind == 0) {
-      seg_one<CSG>(sc, tb.obj[seg.index], seg.combine, seg.k, p, acc, seeded);
+      const DObj o = load_uniform(sc.obj, seg.index);
+      seg_one<CSG>(sc, o, seg.combine, seg.k, p, acc, seeded);
       return;
     }
     if (seg.kind == 2) {
@@ -417,7 +439,10 @@ MM3E_DEV void seg_step(const DevScene& sc, const Tables& tb, V3 p, uint32_t si, 
       uint32_t bm;
       bool any;
       run_begin(acc, seeded, bd, bm, any);
-      for (uint32_t u = 0; u < seg.unb_count; u++) run_member<CSG>(sc, tb.obj[tb.unb[seg.unb_base + u]], p, bd, bm, any);
+      for (uint32_t u = 0; u < seg.unb_count; u++) {
+        const DObj o = load_uniform(sc.obj, load_uniform(sc.unb, seg.unb_base + u));
+        run_member<CSG>(sc, o, p, bd, bm, any);
+      }
       run_end(acc, seeded, bd, bm);
       return;
     }
@@ -430,8 +455,8 @@ MM3E_DEV void seg_step(const DevScene& sc, const Tables& tb, V3 p, uint32_t si, 
       bd = __builtin_inff(); bi = INT64_MAX; bm = 0u;
     }
     for (uint32_t u = 0; u < seg.unb_count; u++) {
-      uint32_t i = tb.unb[seg.unb_base + u];
-      const DObj& o = tb.obj[i];
+      const uint32_t i = load_uniform(sc.unb, seg.unb_base + u);
+      const DObj o = load_uniform(sc.obj, i);
       // a skip sphere proves d > bd, so neither the < nor the tie test can fire
       if ((o.flags & OBJ_SKIP) && far_noop(o, p, rmax(bd, sc.slack))) continue;
       float d = object_distance<CSG>(sc, o, p);
@@ -522,10 +547,10 @@ MM3E_DEV GridSel grid_select(const DevScene& sc, V3 p, const Field& acc) {
   GridSel g;
   g.on = false;
   const float thr = rmax(acc.dist + sc.grid_kmax, sc.slack) * 1.0001f + sc.grid_eps;
-  const bool near0 = thr < sc.grid_m[0];
-  const bool near1 = thr < sc.grid_m[1];
-  if (__builtin_amdgcn_ballot_w64(!near1) != 0) return g;
-  const uint32_t level = __builtin_amdgcn_ballot_w64(!near0) != 0 ? 1u : 0u;
+  // the smallest level whose reach every active lane's threshold is below
+  if (__builtin_amdgcn_ballot_w64(!(thr < sc.grid_m[kGridLevels - 1])) != 0) return g;
+  uint32_t level = kGridLevels - 1;
+  while (level > 0 && __builtin_amdgcn_ballot_w64(!(thr < sc.grid_m[level - 1])) == 0) level--;
   const uint32_t cell = grid_cell(sc, p);
   uint64_t rem = __builtin_amdgcn_ballot_w64(true);
   const uint32_t c0 = __builtin_amdgcn_readlane(cell, (uint32_t)__builtin_ctzll(rem));
@@ -547,10 +572,10 @@ MM3E_DEV GridSel grid_select(const DevScene& sc, V3 p, const Field& acc) {
   const uint32_t* __restrict__ grid = sc.grid;
   g.on = true;
   g.W = sc.grid_words;
-  g.m0 = grid + ((uint64_t)c0 * 2 + level) * g.W;
-  g.m1 = grid + ((uint64_t)c1 * 2 + level) * g.W;
-  g.m2 = grid + ((uint64_t)c2 * 2 + level) * g.W;
-  g.m3 = grid + ((uint64_t)c3 * 2 + level) * g.W;
+  g.m0 = grid + ((uint64_t)c0 * kGridLevels + level) * g.W;
+  g.m1 = grid + ((uint64_t)c1 * kGridLevels + level) * g.W;
+  g.m2 = grid + ((uint64_t)c2 * kGridLevels + level) * g.W;
+  g.m3 = grid + ((uint64_t)c3 * kGridLevels + level) * g.W;
   return g;
 }
 MM3E_DEV uint32_t grid_word(const GridSel& g, uint32_t w) { return g.m0[w] | g.m1[w] | g.m2[w] | g.m3[w]; }

@@ -172,17 +172,20 @@ struct alignas(16) DevScene {
   const float* vol_data;
   // Segment candidate grid (grid == nullptr when absent).  Cubic cells of side 1/grid_inv
   // from grid_lo; cell (x, y, z) has index (z * ny + y) * nx + x and index nx*ny*nz stands for
-  // every point outside the grid.  Each cell holds two bit masks over the segments (level 0 and
-  // level 1, grid_words words each, [cell][level][word]): bit s is clear only if segment s's
-  // members all lie farther than grid_m[level] from every point of the cell.  Built only for
-  // scenes without subtract segments, where the accumulator never increases along the fold.
+  // every point outside the grid.  Each cell holds kGridLevels bit masks over the segments
+  // (grid_words words each, [cell][level][word]): bit s of level l is clear only if segment s's
+  // members all lie farther than grid_m[l] from every point of the cell.  Built only for scenes
+  // without subtract segments, where the accumulator never increases along the fold.
   float grid_lo[3], grid_inv;
   uint32_t grid_dims[3], grid_words;
-  float grid_m[2], grid_kmax, grid_eps;
+  float grid_m[6];
+  float grid_kmax, grid_eps;
   const uint32_t* grid;
   // Postfix CSG programs; a CSG object's p[0] / p[1] hold its first op and op count (bits).
   const DCsgOp* csg;
 };
+
+constexpr uint32_t kGridLevels = 6;
 
 // Per-frame camera record (mm3e-kit/src/camera.rs:39-47) plus the image size.
 // Table view used by the field walk: device addresses in HBM, or the same tables staged in LDS.

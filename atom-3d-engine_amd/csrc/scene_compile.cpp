@@ -21,13 +21,15 @@ namespace mm3e {
 // Longest run folded linearly; longer runs walk their tree (O(log n) per evaluation).
 static const size_t kLinearRunMax = MM3E_LINEAR_RUN_MAX;
 // Segment candidate grid: built for scenes with at least this many segments; about
-// kGridTargetCells cells; level-0 / level-1 masks keep segments within slack + kGridNear /
-// slack + kGridFar of a cell.
+// kGridTargetCells cells; the level-l mask keeps the segments within slack + kGridReach[l] of a
+// cell.  The grid spans the bounded segments' spheres plus slack + kGridPadReach; a point
+// outside it is farther than that from every bounded segment, so the outside cell holds every
+// segment only at the levels that reach farther.
 static const size_t kGridMinSegments = 8;
 static const double kGridTargetCells = 32768.0;
 static const double kGridMaxDim = 256.0;
-static const double kGridNear = 0.25;
-static const double kGridFar = 2.0;
+static const double kGridReach[kGridLevels] = {0.25, 0.75, 2.0, 4.0, 8.0, 16.0};
+static const double kGridPadReach = 2.0;
 
 namespace {
 
@@ -617,7 +619,9 @@ void build_seg_grid(const std::vector<DSeg>& segs, const std::vector<std::vector
     }
   }
   if (!any_bounded) return;
-  const double m0 = (double)slack + kGridNear, m1 = (double)slack + kGridFar;
+  double mreach[kGridLevels];
+  for (uint32_t l = 0; l < kGridLevels; l++) mreach[l] = (double)slack + kGridReach[l];
+  const double m1 = (double)slack + kGridPadReach;  // padding of the grid around the spheres
   double ext = 0.0;
   for (int a = 0; a < 3; a++) ext = std::max(ext, hi[a] - lo[a] + 2.0 * m1);
   const double pad = 0.01 + 1e-5 * ext;
@@ -640,24 +644,27 @@ void build_seg_grid(const std::vector<DSeg>& segs, const std::vector<std::vector
   const double csd = 1.0 / (double)inv;  // the cell side the device's index actually uses
   const uint64_t n_cells = (uint64_t)dims[0] * dims[1] * dims[2];
   const uint32_t W = (uint32_t)((n + 31) / 32);
-  g.words.assign((n_cells + 1) * 2 * W, 0u);
+  const uint32_t L = kGridLevels;
+  g.words.assign((n_cells + 1) * L * W, 0u);
   const double e = 1e-3 * csd + 1e-5 * ext;  // index rounding margin
   float lof[3];
   for (int a = 0; a < 3; a++) lof[a] = (float)lo[a];
   for (size_t s = 0; s < n; s++) {
     const uint32_t word = (uint32_t)(s / 32), bit = 1u << (s % 32);
     if (sph[s].always) {
-      for (uint64_t c = 0; c <= n_cells; c++) {
-        g.words[(c * 2 + 0) * W + word] |= bit;
-        g.words[(c * 2 + 1) * W + word] |= bit;
-      }
+      for (uint64_t c = 0; c <= n_cells; c++)
+        for (uint32_t l = 0; l < L; l++) g.words[(c * L + l) * W + word] |= bit;
       continue;
     }
+    // outside the grid: farther than the padding reach from this segment, nearer is unknown
+    for (uint32_t l = 0; l < L; l++)
+      if (mreach[l] > m1) g.words[(n_cells * L + l) * W + word] |= bit;
     const S& q = sph[s];
+    const double mmax = mreach[L - 1];
     int64_t r0[3], r1[3];
     for (int a = 0; a < 3; a++) {
-      r0[a] = (int64_t)std::floor((q.c[a] - q.r - m1 - e - (double)lof[a]) / csd) - 1;
-      r1[a] = (int64_t)std::floor((q.c[a] + q.r + m1 + e - (double)lof[a]) / csd) + 1;
+      r0[a] = (int64_t)std::floor((q.c[a] - q.r - mmax - e - (double)lof[a]) / csd) - 1;
+      r1[a] = (int64_t)std::floor((q.c[a] + q.r + mmax + e - (double)lof[a]) / csd) + 1;
       r0[a] = std::max<int64_t>(r0[a], 0);
       r1[a] = std::min<int64_t>(r1[a], (int64_t)dims[a] - 1);
     }
@@ -673,8 +680,8 @@ void build_seg_grid(const std::vector<DSeg>& segs, const std::vector<std::vector
           }
           const double tag = std::sqrt(d2) - q.r;
           const uint64_t c = ((uint64_t)z * dims[1] + (uint64_t)y) * dims[0] + (uint64_t)x;
-          if (tag <= m0) g.words[(c * 2 + 0) * W + word] |= bit;
-          if (tag <= m1) g.words[(c * 2 + 1) * W + word] |= bit;
+          for (uint32_t l = 0; l < L; l++)
+            if (tag <= mreach[l]) g.words[(c * L + l) * W + word] |= bit;
         }
   }
   for (int a = 0; a < 3; a++) {
@@ -683,8 +690,7 @@ void build_seg_grid(const std::vector<DSeg>& segs, const std::vector<std::vector
   }
   g.inv = inv;
   g.n_words = W;
-  g.m[0] = (float)m0;
-  g.m[1] = (float)m1;
+  for (uint32_t l = 0; l < L; l++) g.m[l] = (float)mreach[l];
   g.kmax = (float)kmax;
   // absolute margin for the device's float bounding-sphere lower bounds inside the grid
   g.eps = (float)(1e-4 + 4e-6 * (ext + std::fabs(lo[0]) + std::fabs(lo[1]) + std::fabs(lo[2])));
@@ -1012,8 +1018,7 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
     }
     d.grid_inv = grid.inv;
     d.grid_words = grid.n_words;
-    d.grid_m[0] = grid.m[0];
-    d.grid_m[1] = grid.m[1];
+    for (uint32_t l = 0; l < kGridLevels; l++) d.grid_m[l] = grid.m[l];
     d.grid_kmax = grid.kmax;
     d.grid_eps = grid.eps;
     d.grid = (const uint32_t*)(dev_base + o_grid);

@@ -74,7 +74,7 @@ uint64_t blob_size(const HostScene& sc, const std::vector<mm3e_ir_object>& extra
 struct SegGrid {
   float lo[3] = {0, 0, 0}, inv = 0;
   uint32_t dims[3] = {0, 0, 0}, n_words = 0;
-  float m[2] = {0, 0}, kmax = 0, eps = 0;
+  float m[kGridLevels] = {}, kmax = 0, eps = 0;
   std::vector<uint32_t> words;  // [cell (+1 outside)][level][word]
 };
 void build_seg_grid(const std::vector<DSeg>& segs, const std::vector<std::vector<uint32_t>>& members,
