@@ -218,14 +218,7 @@ bool enqueue(mm3e_batch* b, std::string& msg) {
       const mm3e_scene* s = b->in[b->order[b->gen[i]]].scene;
       const int lane = (int)(i % mm3e_batch::kStreams);
       hipStream_t gs = lane == 0 ? st : b->aux[lane - 1];
-      const DevScene* sc_arg = J.sc;
-      FrameParams fp_arg = J.fp;
-      float* hdr_arg = J.hdr;
-      uint32_t tw = J.tw, th = J.th, gstage = J.stage_bytes;
-      void* args[] = {&sc_arg, &fp_arg, &hdr_arg, &tw, &th, &gstage};
-      const uint32_t gx = J.tiles_x, gy = (J.fp.height + J.th - 1) / J.th;
-      if (!hip_ok(hipModuleLaunchKernel(s->gen->fn, gx, gy, 1, tw * th * s->aa * s->aa, 1, 1, gstage, gs, args,
-                                        nullptr),
+      if (!hip_ok(launch_generated_index(s->gen->fn, J.sc, J.fp, J.hdr, J.tw, J.th, s->aa, J.stage_bytes, gs),
                   "generated batch job", &msg))
         return false;
     }

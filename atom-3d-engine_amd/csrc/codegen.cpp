@@ -158,15 +158,10 @@ std::string emit_scene_source(const Blob& b, uint32_t maxb, bool sm, bool queued
   emit_map(o, b, "GenMap");
   o += "}  // namespace mm3e\n";
   o += "using namespace mm3e;\n";
-  o += "extern \"C\" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY\n"
-       "    mm3e_gen_render(const DevScene* __restrict__ sc, FrameParams fp, float* __restrict__ hdr, uint32_t tw,\n"
-       "                    uint32_t th, uint32_t stage_bytes) {\n"
-       "  __shared__ float smem[256 * 3];\n"
-       "  extern __shared__ uint4 tables_lds[];\n"
-       "  const Tables tb = stage_bytes ? stage_tables(sc, tables_lds, stage_bytes) : global_tables(sc);\n";
-  o += "  render_tile<GenMap, " + std::to_string(maxb) + ", " + (sm ? "true" : "false") +
-       ">(sc, tb, fp, hdr, tw, th, blockIdx.x, blockIdx.y, smem);\n}\n";
-  // the same tiles in a given order (block b renders tile o.order[b])
+  // The scene's one render kernel: block b renders tile o.order[b] (most expensive first), or
+  // tile b when o.order is null.  One kernel for both dispatch orders halves the run-time
+  // compile, which grows with every inlined copy of the fold (and hipRTC compiles one program
+  // at a time per process).
   o += "extern \"C\" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY\n"
        "    mm3e_gen_render_o(const DevScene* __restrict__ sc, FrameParams fp, float* __restrict__ hdr, uint32_t tw,\n"
        "                      uint32_t th, uint32_t stage_bytes, TileOrder o) {\n"

@@ -535,57 +535,52 @@ MM3E_DEV uint32_t grid_cell(const DevScene& sc, V3 p) {
   return (iz * ny + iy) * nx + ix;
 }
 
+// OR of v over the active lanes of the wave, uniform: each pass takes the bits of the first
+// lane that still adds any, so the loop runs once per distinct contribution (lanes in the same
+// cell and level share theirs) and ends after at most 32 passes.
+MM3E_DEV uint32_t wave_or(uint32_t v) {
+  uint32_t acc = 0u;
+  uint64_t rem = __builtin_amdgcn_ballot_w64(true);
+  while (rem != 0) {
+    acc |= (uint32_t)__builtin_amdgcn_readlane(v, (uint32_t)__builtin_ctzll(rem));
+    rem = __builtin_amdgcn_ballot_w64((v & ~acc) != 0u);
+  }
+  return acc;
+}
+
 // The wave's segment masks once the first segment has seeded `acc` (see world_eval): `on` is
-// false (fold every segment) unless every active lane's threshold is inside a grid level and
-// the lanes' points fall in at most four distinct cells.  All fields are wave-uniform.
+// false (fold every segment) when some active lane's threshold lies beyond the grid's widest
+// level.  Otherwise each lane takes its own cell at the smallest level its own threshold is
+// below, and word w of the wave's mask is the OR of those lanes' words (`grid_word`): the
+// segments any lane may need, whatever the number of distinct cells the wave's points touch.
 struct GridSel {
   bool on;
   uint32_t W;
-  const uint32_t *m0, *m1, *m2, *m3;
+  uint32_t base;  // per lane: first word of the lane's (cell, level) mask
+  const uint32_t* m;
 };
 MM3E_DEV GridSel grid_select(const DevScene& sc, V3 p, const Field& acc) {
   GridSel g;
   g.on = false;
   const float thr = rmax(acc.dist + sc.grid_kmax, sc.slack) * 1.0001f + sc.grid_eps;
-  // the smallest level whose reach every active lane's threshold is below
   if (__builtin_amdgcn_ballot_w64(!(thr < sc.grid_m[kGridLevels - 1])) != 0) return g;
   uint32_t level = kGridLevels - 1;
-  while (level > 0 && __builtin_amdgcn_ballot_w64(!(thr < sc.grid_m[level - 1])) == 0) level--;
-  const uint32_t cell = grid_cell(sc, p);
-  uint64_t rem = __builtin_amdgcn_ballot_w64(true);
-  const uint32_t c0 = __builtin_amdgcn_readlane(cell, (uint32_t)__builtin_ctzll(rem));
-  rem &= ~__builtin_amdgcn_ballot_w64(cell == c0);
-  uint32_t c1 = c0, c2 = c0, c3 = c0;
-  if (rem) {
-    c1 = __builtin_amdgcn_readlane(cell, (uint32_t)__builtin_ctzll(rem));
-    rem &= ~__builtin_amdgcn_ballot_w64(cell == c1);
-  }
-  if (rem) {
-    c2 = __builtin_amdgcn_readlane(cell, (uint32_t)__builtin_ctzll(rem));
-    rem &= ~__builtin_amdgcn_ballot_w64(cell == c2);
-  }
-  if (rem) {
-    c3 = __builtin_amdgcn_readlane(cell, (uint32_t)__builtin_ctzll(rem));
-    rem &= ~__builtin_amdgcn_ballot_w64(cell == c3);
-  }
-  if (rem != 0) return g;
-  const uint32_t* __restrict__ grid = sc.grid;
+  for (int l = (int)kGridLevels - 2; l >= 0; l--)
+    if (thr < sc.grid_m[l]) level = (uint32_t)l;
   g.on = true;
   g.W = sc.grid_words;
-  g.m0 = grid + ((uint64_t)c0 * kGridLevels + level) * g.W;
-  g.m1 = grid + ((uint64_t)c1 * kGridLevels + level) * g.W;
-  g.m2 = grid + ((uint64_t)c2 * kGridLevels + level) * g.W;
-  g.m3 = grid + ((uint64_t)c3 * kGridLevels + level) * g.W;
+  g.base = (grid_cell(sc, p) * kGridLevels + level) * g.W;
+  g.m = sc.grid;
   return g;
 }
-MM3E_DEV uint32_t grid_word(const GridSel& g, uint32_t w) { return g.m0[w] | g.m1[w] | g.m2[w] | g.m3[w]; }
+MM3E_DEV uint32_t grid_word(const GridSel& g, uint32_t w) { return wave_or(g.m[g.base + w]); }
 
 // accel.rs:209-252 WorldPlan::eval — the reference `world()` field: the ordered fold of
 // seg_step over every segment.  With a segment grid, after the first segment seeds the
 // accumulator each lane checks that max(acc + kmax, slack) (plus rounding margins) is below a
-// grid level's reach; if every active lane passes, the wave gathers the distinct cells of its
-// lanes (at most four; scalar readlane/ballot), ORs their masks with scalar loads and folds only
-// the segments whose bits are set, in index order.  A clear bit is a segment whose every member
+// grid level's reach; if every active lane passes, each lane picks its cell's mask at its own
+// level, the wave ORs those masks word by word and folds only the segments whose bits are set,
+// in index order.  A clear bit is a segment whose every member
 // is a provable no-op for every point of that cell at that threshold, so the fold is the same.
 // GRID: whether the scene may carry a segment grid (kernels for scenes without one keep the
 // plain fold).

@@ -435,7 +435,7 @@ MM3E_DEV void render_tile(const DevScene* __restrict__ scp, const Tables& tb, co
 // as render_tile renders it, whatever the order.
 // ---------------------------------------------------------------------------------------
 struct TileOrder {
-  const uint32_t* order;  // tile indices (ty * tiles_x + tx), most expensive first
+  const uint32_t* order;  // tile indices (ty * tiles_x + tx), most expensive first; null: index order
   uint32_t* cost;         // per tile: tickets drawn by other tiles while it ran
   uint32_t* ticket;       // running ticket counter (wraps; only differences are used)
   uint32_t* start;        // per tile: its start ticket (plain launches only)
@@ -447,6 +447,10 @@ template <class Map, int MAXB, bool SM>
 MM3E_DEV void render_ordered(const DevScene* __restrict__ scp, const Tables& tb, const FrameParams& fp,
                              float* __restrict__ hdr, uint32_t tw, uint32_t th, const TileOrder& o, float* smem,
                              uint32_t* /*unused*/) {
+  if (o.order == nullptr) {  // tile index order, no cost bookkeeping
+    render_tile<Map, MAXB, SM>(scp, tb, fp, hdr, tw, th, blockIdx.x % o.tiles_x, blockIdx.x / o.tiles_x, smem);
+    return;
+  }
   const uint32_t tile = o.order[blockIdx.x];
   // Device-scope atomics only: a __threadfence here writes back the XCD's L2 and costs more
   // than the frame (measured: c2 1.4 -> 2.2 ms).  The cost is a heuristic for the next order,

@@ -222,11 +222,14 @@ static void gen_worker(mm3e_renderer* r) {
 }
 
 // Emit the scene's generated kernel and compile it (SYNC) or queue it (BACKGROUND).  Scenes
-// with more than kCodegenMaxObjects objects keep the table kernels: measured on procedural
-// scenes at 1280x720 (ordered tiles, the generated kernel's occupancy picked by codegen), the
-// generated fold is 1.24x faster at 40 objects, 1.26x at 60, 1.09x at 80, 0.95x at 100 and
-// 0.93x at 130 (its code no longer fits the instruction cache, while the table walk's does).
-static const size_t kCodegenMaxObjects = 80;
+// with more than kCodegenMaxObjects objects keep the table kernels.  Measured on MI355X at
+// 1280x720 (ordered tiles, the generated kernel's occupancy picked by codegen), the generated
+// fold beats the table walk at every size of the batched load's eight scenes (39 / 72 / 76 / 82
+// / 117 / 127 / 174 / 184 objects: 7.4 / 11.8 / 1.9 / 12.1 / 1.8 / 36.1 / 53.4 / 4.2 ms against
+// 9.1 / 14.5 / 2.1 / 15.1 / 2.2 / 39.3 / 66.1 / 5.9 ms) and on the 1000-object stress frame
+// (777 against 844 ms at 3840x2160), once its grid masks are per lane and its sin/cos is
+// inline; the cap only bounds the compile (75 s for 1000 objects, on a background thread).
+static const size_t kCodegenMaxObjects = 2000;
 size_t codegen_object_limit() {  // MM3E_CODEGEN_MAX_OBJECTS overrides (tuning)
   if (const char* v = std::getenv("MM3E_CODEGEN_MAX_OBJECTS")) {
     const long n = std::atol(v);
@@ -234,6 +237,18 @@ size_t codegen_object_limit() {  // MM3E_CODEGEN_MAX_OBJECTS overrides (tuning)
   }
   return kCodegenMaxObjects;
 }
+hipError_t launch_generated_index(hipFunction_t fn, const DevScene* sc, const FrameParams& fp, float* hdr, uint32_t tw,
+                                  uint32_t th, uint32_t aa, uint32_t stage_bytes, hipStream_t st) {
+  const uint32_t tiles_x = (fp.width + tw - 1) / tw, tiles_y = (fp.height + th - 1) / th;
+  TileOrder o{};
+  o.tiles_x = tiles_x;
+  const DevScene* sc_arg = sc;
+  FrameParams fp_arg = fp;
+  float* hdr_arg = hdr;
+  void* args[] = {&sc_arg, &fp_arg, &hdr_arg, &tw, &th, &stage_bytes, &o};
+  return hipModuleLaunchKernel(fn, tiles_x * tiles_y, 1, 1, tw * th * aa * aa, 1, 1, stage_bytes, st, args, nullptr);
+}
+
 static void gen_schedule(mm3e_scene* s) {
   mm3e_renderer* r = s->r;
   if (s->host.objects.size() > codegen_object_limit()) return;
@@ -436,11 +451,9 @@ bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std
                   "generated render launch", &msg))
         return false;
       ordered = true;
-    } else {
-      void* args[] = {&sc_arg, &fp_arg, &hdr_arg, &tw, &th, &gstage};
-      if (!hip_ok(hipModuleLaunchKernel(gj->fn, grid.x, grid.y, 1, block.x, 1, 1, gstage, st, args, nullptr),
-                  "generated render launch", &msg))
-        return false;
+    } else if (!hip_ok(launch_generated_index(gj->fn, sc_arg, fp_arg, hdr_arg, tw, th, s->aa, gstage, st),
+                       "generated render launch", &msg)) {
+      return false;
     }
   } else if (dsc == s->d_scene && to_on && tile_order_ready(s, grid.x * grid.y, msg)) {
     TileOrder o = scene_tile_order(s, grid.x);
@@ -899,8 +912,8 @@ int mm3e_scene_kernel_name(const mm3e_scene* s, char* buf, size_t buflen) {
     else std::snprintf(buf, buflen, "%s%u", s->general ? "mm3e_wf_shade_x_b" : "mm3e_wf_shade_b", s->bounces);
     return MM3E_OK;
   }
-  if (gen_ready) {
-    std::snprintf(buf, buflen, "%s", s->gen->fn_o && s->to_on ? "mm3e_gen_render_o" : "mm3e_gen_render");
+  if (gen_ready) {  // one kernel for both dispatch orders (tile order or index order)
+    std::snprintf(buf, buflen, "mm3e_gen_render_o");
     return MM3E_OK;
   }
   const char* base = s->to_on ? (s->general ? "mm3e_render_ord_x_b" : "mm3e_render_ord_b")

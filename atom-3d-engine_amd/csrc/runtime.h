@@ -21,8 +21,10 @@ struct GenJob {
   std::string src;
   int device = 0;
   std::atomic<int> state{MM3E_CODEGEN_QUEUED};
+  // the render kernel mm3e_gen_render_o (both fields name it): tiles in a given order, or in
+  // index order with a null TileOrder::order (launch_generated_index)
   hipFunction_t fn = nullptr;
-  hipFunction_t fn_o = nullptr;  // ordered-tiles variant
+  hipFunction_t fn_o = nullptr;
   // optional kernels (GenOpt order: queued-frame stages, film, G-buffer); null unless the
   // source was generated with them
   hipFunction_t fn_opt[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -105,6 +107,9 @@ FrameParams make_frame(const mm3e_camera& c, uint32_t w, uint32_t h);
 void tile_shape(uint32_t aa, uint32_t* tw, uint32_t* th);
 // Largest scene (objects) that gets a generated kernel.
 size_t codegen_object_limit();
+// The generated render kernel over a frame's tiles in index order (one block per tile).
+hipError_t launch_generated_index(hipFunction_t fn, const DevScene* sc, const FrameParams& fp, float* hdr, uint32_t tw,
+                                  uint32_t th, uint32_t aa, uint32_t stage_bytes, hipStream_t st);
 // The scene's generated optional kernel `k` (GenOpt order) when its module is ready, else null.
 hipFunction_t gen_opt(const mm3e_scene* s, int k);
 // G-buffer pass of a scene (its generated kernel when ready, else the table kernels): the

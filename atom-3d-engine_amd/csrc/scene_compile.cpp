@@ -645,6 +645,8 @@ void build_seg_grid(const std::vector<DSeg>& segs, const std::vector<std::vector
   const uint64_t n_cells = (uint64_t)dims[0] * dims[1] * dims[2];
   const uint32_t W = (uint32_t)((n + 31) / 32);
   const uint32_t L = kGridLevels;
+  // the device indexes the masks with 32-bit word offsets
+  if ((n_cells + 1) * L * W > 0xFFFFFFFFull) return;
   g.words.assign((n_cells + 1) * L * W, 0u);
   const double e = 1e-3 * csd + 1e-5 * ext;  // index rounding margin
   float lof[3];

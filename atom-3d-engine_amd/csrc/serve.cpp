@@ -3,8 +3,9 @@
 // Requests (resident compiled scene, camera, size) arrive from any host thread.  A worker
 // thread admits them FIFO into batches while the HBM budget allows (framebuffers for a
 // request: w*h*(12 + 12 + 4) bytes, carved from one slab allocated at start-up), launches each
-// batch as the persistent tile-queue render + batched post (the kernels of batch_kernel.hip)
-// followed by the RGBA8 copies into pinned staging, and keeps up to `max_inflight` batches in
+// batch as one render launch per request whose scene has its generated kernel and one
+// persistent tile-queue render for the others, then the batched post (the kernels of
+// batch_kernel.hip) and the RGBA8 copies into pinned staging, and keeps up to `max_inflight` batches in
 // flight on separate default-priority streams.  Requests that arrive while a batch runs are
 // admitted into the next batch as soon as a stream slot and memory are free, so their work
 // overlaps the running batch's tail on the device.  Completion is detected with events;
@@ -138,6 +139,12 @@ void pinned_put(mm3e_server* S, uint8_t* p, size_t n) {
   if (p) S->pinned_free.emplace(n, p);
 }
 
+// A request whose scene has its generated kernel ready renders with that kernel, in its own
+// launch on the slot's stream (the kernel in tile index order: it keeps no per-scene state, so
+// requests of one scene may run on several slots at once); the others share one persistent
+// tile-queue launch of the table-walk batch kernel.  One batched post pass serves all of them.
+// Both kernels compute the same pixels (the generated fold performs the table walk's
+// operations), so a request's image does not depend on which one rendered it.
 bool launch_slot(mm3e_server* S, Slot& sl, std::string& msg) {
   using namespace mm3e;
   sl.jobs.clear();
@@ -147,55 +154,68 @@ bool launch_slot(mm3e_server* S, Slot& sl, std::string& msg) {
                    [&](Request* a, Request* b) { return a->scene_id < b->scene_id; });
   uint32_t tile = 0, row = 0, max_w = 0, max_r = 1, maxb = 0, stage_max = 0;
   bool all_fit = true;
+  hipStream_t st = sl.stream;
   for (Request* q : sl.reqs) {
     mm3e_scene* s = q->scene;
-    BatchJob J{};
-    J.sc = s->d_scene;
-    J.fp = make_frame(q->cam, q->w, q->h);
-    if (s->aa <= 1) {
-      J.tw = 16;
-      J.th = 16;
-    } else {
-      tile_shape(s->aa, &J.tw, &J.th);
-    }
-    J.tiles_x = (q->w + J.tw - 1) / J.tw;
-    J.tile_base = tile;
-    tile += J.tiles_x * ((q->h + J.th - 1) / J.th);
-    J.stage_bytes = s->hdr.stage_bytes;
-    stage_max = std::max(stage_max, J.stage_bytes);
-    all_fit = all_fit && J.stage_bytes <= S->r->lds_budget;
+    const GenJob* gj = s->gen.get();
+    const bool generated = gj && gj->state.load(std::memory_order_acquire) == MM3E_CODEGEN_READY && gj->fn;
     const size_t px = (size_t)q->w * q->h;
     uint8_t* base = S->d_slab + q->off;
-    J.hdr = (float*)base;
+    const FrameParams fp = make_frame(q->cam, q->w, q->h);
+    if (generated) {
+      if (!hip_ok(launch_generated_index(gj->fn, s->d_scene, fp, (float*)base, s->tw, s->th, s->aa, s->gen_stage, st),
+                  "serve: generated render", &msg))
+        return false;
+    } else {
+      BatchJob J{};
+      J.sc = s->d_scene;
+      J.fp = fp;
+      if (s->aa <= 1) {
+        J.tw = 16;
+        J.th = 16;
+      } else {
+        tile_shape(s->aa, &J.tw, &J.th);
+      }
+      J.tiles_x = (q->w + J.tw - 1) / J.tw;
+      J.tile_base = tile;
+      tile += J.tiles_x * ((q->h + J.th - 1) / J.th);
+      J.stage_bytes = s->hdr.stage_bytes;
+      J.gen = kGenTable;
+      stage_max = std::max(stage_max, J.stage_bytes);
+      all_fit = all_fit && J.stage_bytes <= S->r->lds_budget;
+      J.hdr = (float*)base;
+      maxb = std::max(maxb, s->bounces);
+      sl.jobs.push_back(J);
+    }
     BatchPost P{};
     P.pp = s->pp;
     P.pp.width = q->w;
     P.pp.height = q->h;
     P.kern = s->d_kern;
-    P.hdr = J.hdr;
+    P.hdr = (float*)base;
     P.tmp = (float*)(base + px * 12);
     P.rgba = (uchar4*)(base + px * 24);
     P.row_base = row;
     row += q->h;
     max_w = std::max(max_w, q->w);
     max_r = std::max(max_r, P.pp.radius);
-    maxb = std::max(maxb, s->bounces);
-    sl.jobs.push_back(J);
     sl.posts.push_back(P);
   }
   const uint32_t lds = all_fit ? stage_max : 0;
-  hipStream_t st = sl.stream;
-  if (!hip_ok(hipMemcpyAsync(sl.d_jobs, sl.jobs.data(), sl.jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice, st),
-              "serve: upload jobs", &msg))
-    return false;
+  if (!sl.jobs.empty()) {
+    if (!hip_ok(hipMemcpyAsync(sl.d_jobs, sl.jobs.data(), sl.jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice,
+                               st),
+                "serve: upload jobs", &msg))
+      return false;
+    if (!hip_ok(launch_batch_reset(st, sl.d_counter), "serve: reset", &msg)) return false;
+    if (!hip_ok(launch_batch_render(maxb, S->cus * S->cfg.blocks_per_cu, st, sl.d_jobs, (uint32_t)sl.jobs.size(), tile,
+                                    sl.d_counter, lds),
+                "serve: render", &msg))
+      return false;
+  }
   if (!hip_ok(hipMemcpyAsync(sl.d_posts, sl.posts.data(), sl.posts.size() * sizeof(BatchPost), hipMemcpyHostToDevice,
                              st),
               "serve: upload posts", &msg))
-    return false;
-  if (!hip_ok(launch_batch_reset(st, sl.d_counter), "serve: reset", &msg)) return false;
-  if (!hip_ok(launch_batch_render(maxb, S->cus * S->cfg.blocks_per_cu, st, sl.d_jobs, (uint32_t)sl.jobs.size(), tile,
-                                  sl.d_counter, lds),
-              "serve: render", &msg))
     return false;
   if (!hip_ok(launch_batch_post(st, sl.d_posts, (uint32_t)sl.posts.size(), row, max_w, max_r), "serve: post", &msg))
     return false;

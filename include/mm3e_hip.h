@@ -172,8 +172,9 @@ int mm3e_write_bmp(const uint8_t* rgba, uint32_t w, uint32_t h, uint8_t* out, si
  * scene's world fold is emitted as HIP source with the scene's constants folded in and compiled
  * at run time (hipRTC) into a render kernel for that scene.  The generated kernel performs the
  * same floating-point operations as the table-driven kernels, so images are identical; only
- * speed differs; scenes of more than 80 objects keep the table kernels (their generated fold
- * outgrows the instruction cache and runs slower; MM3E_CODEGEN_MAX_OBJECTS overrides).  Modes: OFF (table kernels only), BACKGROUND (default: compile on worker
+ * speed differs.  Scenes of more than 2000 objects keep the table kernels (the cap bounds the
+ * run-time compile: about 75 s for 1000 objects; MM3E_CODEGEN_MAX_OBJECTS overrides).  Modes:
+ * OFF (table kernels only), BACKGROUND (default: compile on worker
  * threads, the scene renders with the table kernels until its kernel is ready), SYNC (compile
  * inside mm3e_scene_compile).  The environment variable MM3E_CODEGEN=off|background|sync sets
  * the default for new renderers. */

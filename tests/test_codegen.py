@@ -20,7 +20,9 @@ def _scenes():
     refl = configs.procedural_scene(43, 40, 240, 135)
     refl.bounces, refl.fog_density, refl.aa = 2, 0.03, 2
     p20 = configs.procedural_scene(40, 20, 320, 180)  # segment grid, nested generated kernel
+    p180 = configs.procedural_scene(42, 180, 160, 90)  # long fold, state machine at 5 waves
     return [("c1", sc1, cam1), ("c2", sc2, cam2),
+            ("procedural180", p180, configs.orbit_pose(configs.SplitMix64(42))),
             ("procedural20", p20, configs.orbit_pose(configs.SplitMix64(40))),
             ("procedural60", p60, configs.orbit_pose(configs.SplitMix64(41))),
             ("procedural40", p40, configs.orbit_pose(configs.SplitMix64(42))),
@@ -31,10 +33,10 @@ def test_generated_kernel_family():
     """Up to 24 objects the generated kernel takes the nested path, segment grid or not; above,
     the state machine, register-allocated for 4 waves per SIMD (5 above 48 objects)."""
     small = hip.emit_source(configs.procedural_scene(40, 20, 64, 36).to_ir())
-    assert "grid_select" in small and "render_tile<GenMap, 1, false>" in small
+    assert "grid_select" in small and "render_ordered<GenMap, 1, false>" in small
     assert "#define MM3E_RENDER_WAVES 6" in small
     mid = hip.emit_source(configs.procedural_scene(42, 40, 64, 36).to_ir())
-    assert "render_tile<GenMap, 1, true>" in mid and "#define MM3E_RENDER_WAVES 4" in mid
+    assert "render_ordered<GenMap, 1, true>" in mid and "#define MM3E_RENDER_WAVES 4" in mid
     big = hip.emit_source(configs.procedural_scene(41, 60, 64, 36).to_ir())
     assert "#define MM3E_RENDER_WAVES 5" in big
 
@@ -68,7 +70,8 @@ def test_generated_kernels_equal_table_kernels():
         for g in (gg, go):
             state, msg = g.codegen_status(wait=True)
             assert state == hip.CODEGEN_READY, msg
-        assert gg.kernel_name() == "mm3e_gen_render"
+        # one generated kernel serves both dispatch orders
+        assert gg.kernel_name() == "mm3e_gen_render_o"
         assert go.kernel_name() == "mm3e_gen_render_o"
         a, b = gt.render_rgba(cam), gg.render_rgba(cam)
         frames = [go.render_rgba(cam) for _ in range(3)]
@@ -86,7 +89,10 @@ def test_generated_kernels_equal_table_kernels():
 
 
 @pytest.mark.gpu
-def test_long_folds_keep_the_table_kernels():
+def test_scenes_over_the_object_cap_keep_the_table_kernels(monkeypatch):
+    """MM3E_CODEGEN_MAX_OBJECTS lowers the object cap of code generation (default 2000): a
+    180-object scene over a cap of 100 gets no generated kernel and renders with the table walk."""
+    monkeypatch.setenv("MM3E_CODEGEN_MAX_OBJECTS", "100")
     r = hip.Renderer()
     r.set_codegen(hip.CODEGEN_SYNC)
     g = r.compile(configs.procedural_scene(42, 180, 64, 36).to_ir(), 64, 36)

@@ -498,9 +498,11 @@ def test_mixed_batch_with_generated_kernels_equals_per_frame_renders():
 
 def test_generated_batch_module_equals_per_frame_renders(monkeypatch):
     """The batch module holding the generated maps of the batch's scenes (40-object and c2
-    scenes; the 90-object one keeps the table walk inside the same loop), forced on through
-    MM3E_BATCH_GEN=1: every job equals the per-frame render bit for bit, across camera changes."""
+    scenes; the 90-object one, over an object cap of 80, keeps the table walk inside the same
+    loop), forced on through MM3E_BATCH_GEN=1: every job equals the per-frame render bit for
+    bit, across camera changes."""
     monkeypatch.setenv("MM3E_BATCH_GEN", "1")
+    monkeypatch.setenv("MM3E_CODEGEN_MAX_OBJECTS", "80")
     r = hip.Renderer()
     r.set_codegen(hip.CODEGEN_SYNC)
     scenes = [configs.procedural_scene(61, 40, 320, 180), configs.procedural_scene(62, 90, 320, 180),

@@ -29,13 +29,15 @@ static int64_t ulps(float a, float b) {
 }
 int main(int argc, char** argv) {
   const uint32_t stride = (uint32_t)atoi(argv[1]);
+  const bool large = argc > 2;  // |x| in [2^20, 2^128): the integer (Payne-Hanek) reduction
   const unsigned nt = std::thread::hardware_concurrency();
-  const uint32_t lim = 0x49800000u;  // 2^20
+  const uint32_t base = large ? 0x49800000u : 0u;  // 2^20
+  const uint32_t lim = large ? 0x7f800000u : 0x49800000u;
   std::atomic<uint64_t> n{0}, mis_cr{0}, far_cr{0}, mis_f{0}, far_f{0};
   std::vector<std::thread> ts;
   for (unsigned t = 0; t < nt; t++) ts.emplace_back([&, t]() {
     uint64_t a = 0, b = 0, c = 0, d = 0, e = 0;
-    for (uint64_t u = (uint64_t)t * stride; u < lim; u += (uint64_t)nt * stride) {
+    for (uint64_t u = base + (uint64_t)t * stride; u < lim; u += (uint64_t)nt * stride) {
       for (int sign = 0; sign < 2; sign++) {
         const float x = bits((uint32_t)u | (sign ? 0x80000000u : 0u));
         float s, co;
@@ -52,6 +54,13 @@ int main(int argc, char** argv) {
     n += a; mis_cr += b; far_cr += c; mis_f += d; far_f += e;
   });
   for (auto& t : ts) t.join();
+  if (large) {  // non-finite arguments give NaN, as libm
+    float s, co;
+    mm3e::cr_sincos(bits(0x7f800000u), s, co);
+    if (!(s != s) || !(co != co)) return 3;
+    mm3e::cr_sincos(bits(0xffc00000u), s, co);
+    if (!(s != s) || !(co != co)) return 3;
+  }
   std::printf("%llu %llu %llu %llu %llu\n", (unsigned long long)n.load(), (unsigned long long)mis_cr.load(),
               (unsigned long long)far_cr.load(), (unsigned long long)mis_f.load(), (unsigned long long)far_f.load());
   return 0;
@@ -59,7 +68,7 @@ int main(int argc, char** argv) {
 """
 
 
-def test_sincos_matches_libm_rounded_once():
+def _run(*args):
     with tempfile.TemporaryDirectory() as d:
         src = os.path.join(d, "trig.cpp")
         exe = os.path.join(d, "trig")
@@ -67,8 +76,12 @@ def test_sincos_matches_libm_rounded_once():
             f.write(SRC)
         subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-pthread",
                         "-I" + HDR, src, "-o", exe], check=True)
-        out = subprocess.run([exe, "5"], check=True, capture_output=True, text=True).stdout.split()
-    n, mis_cr, far_cr, mis_f, far_f = (int(x) for x in out)
+        out = subprocess.run([exe, *args], check=True, capture_output=True, text=True).stdout.split()
+    return (int(x) for x in out)
+
+
+def test_sincos_matches_libm_rounded_once():
+    n, mis_cr, far_cr, mis_f, far_f = _run("5")
     print(f"\n[cr_sincos] {n} values: vs libm f64 rounded once {mis_cr} differ ({far_cr} by > 1 ulp); "
           f"vs libm f32 sinf/cosf {mis_f} differ ({far_f} by > 1 ulp)", flush=True)
     assert n > 4e8
@@ -77,3 +90,15 @@ def test_sincos_matches_libm_rounded_once():
     # from the rounded f64 value, always by one ulp (the last few pixels of device/oracle
     # differences in twisted, bent or shadow-jittered shading)
     assert far_f == 0 and mis_f <= n * 5e-3
+
+
+def test_sincos_of_large_arguments_matches_libm_rounded_once():
+    """|x| >= 2^20 goes through the exact integer reduction by 96 bits of 2/pi: over a stride-7
+    sample of every finite f32 from 2^20 up (about 2.6e8 values with both signs) it must equal
+    the host libm's f64 sin / cos (full-precision reduction) rounded once to f32, and map
+    infinities and NaN to NaN."""
+    n, mis_cr, far_cr, mis_f, far_f = _run("7", "large")
+    print(f"\n[cr_sincos, |x| >= 2^20] {n} values: vs libm f64 rounded once {mis_cr} differ ({far_cr} by > 1 ulp); "
+          f"vs libm f32 sinf/cosf {mis_f} differ", flush=True)
+    assert n > 2e8
+    assert far_cr == 0 and mis_cr <= 16
