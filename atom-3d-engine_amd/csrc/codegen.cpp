@@ -134,6 +134,9 @@ bool generated_prefers_sm(const Blob& b) {
 }
 
 std::string emit_scene_source(const Blob& b, uint32_t maxb, bool sm, bool queued, bool film) {
+  // more than 4 reflection levels: the state machine (pending levels in a per-lane stack), not
+  // one nested inlined copy of march and shading per level
+  if (maxb > 4) sm = true;
   std::string o;
   o.reserve(4096 + b.objs.size() * 900);
   o += "// generated: world fold of one compiled scene (objects: " + std::to_string(b.objs.size()) +

@@ -792,6 +792,26 @@ MM3E_DEV void bloom_h(const PostParams& pp, const float* __restrict__ kern, cons
   o[0] = acc.x; o[1] = acc.y; o[2] = acc.z;
 }
 
+// bloom_h for radii whose LDS row segment + halo would not fit a block's LDS: every lane reads
+// its 2r+1 taps from the HDR image through the cache and takes their bright-pass itself (the
+// same operations in the same order, so the same sums).
+MM3E_DEV void bloom_h_wide(const PostParams& pp, const float* __restrict__ kern, const float* __restrict__ hdr,
+                           float* __restrict__ tmp, int x0, int y) {
+  const int W = (int)pp.width, r = (int)pp.radius;
+  const int x = x0 + threadIdx.x;
+  if (x >= W) return;
+  V3 acc = V3{0.0f, 0.0f, 0.0f};
+  const int n = 2 * r + 1;
+  for (int k = 0; k < n; k++) {
+    int sx = x + k - r;
+    sx = sx < 0 ? 0 : (sx > W - 1 ? W - 1 : sx);
+    const float* s = hdr + 3 * ((size_t)y * W + sx);
+    acc = acc + scale(bright_pass(V3{s[0], s[1], s[2]}, pp.threshold), kern[k]);
+  }
+  float* o = tmp + 3 * ((size_t)y * W + x);
+  o[0] = acc.x; o[1] = acc.y; o[2] = acc.z;
+}
+
 MM3E_DEV uint8_t to_u8(float c) {  // framebuffer.rs:102-113: (clamp(c,0,1)*255 + 0.5) as u8
   float v = rclamp(c, 0.0f, 1.0f) * 255.0f + 0.5f;
   return (uint8_t)(v >= 255.0f ? 255.0f : v);

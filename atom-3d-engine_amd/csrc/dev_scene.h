@@ -92,6 +92,9 @@ enum : uint32_t {
   CSG_OFFSET = 7,     // top -= p[0]
   CSG_SHELL = 8,      // top = |top| - p[0]
 };
+// Deepest reflection recursion (Scene::bounces) the kernels take: 0..4 as nested inlined levels
+// or the state machine, 5..16 as the state machine with a 16-level per-lane stack.
+constexpr uint32_t kMaxBounces = 16;
 constexpr uint32_t kCsgStack = 10;  // postfix depth of any expression of <= 1024 nodes (csg.rs:13-14)
 
 struct alignas(16) DCsgOp {

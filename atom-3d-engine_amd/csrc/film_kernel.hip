@@ -1,5 +1,5 @@
 // film_kernel.hip — render_film's sample pass (film.rs:63-198) for one reflection depth,
-// compiled once per MM3E_MAXB (0..4) like the render kernels: beauty, premultiplied foreground
+// compiled once per MM3E_MAXB (0..4, 16) like the render kernels: beauty, premultiplied foreground
 // and coverage alpha per pixel through a thin lens.  Two families as the render kernels:
 //   mm3e_film_bN     scenes without a segment grid or CSG objects (nested march/shade);
 //   mm3e_film_x_bN   every other scene (general fold, single-evaluation-site state machine).
@@ -24,6 +24,7 @@
 
 using namespace mm3e;
 
+#if MM3E_MAXB <= 4
 extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
     KNAME(const DevScene* __restrict__ sc, FrameParams fp, FilmParams lp, float* __restrict__ beauty,
           float* __restrict__ fg, float* __restrict__ alpha, uint32_t tw, uint32_t th) {
@@ -31,6 +32,8 @@ extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
   film_tile<SimpleMap, MM3E_MAXB, false>(sc, global_tables(sc), fp, lp, beauty, fg, alpha, tw, th, blockIdx.x,
                                          blockIdx.y, smem);
 }
+
+#endif
 
 extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
     KNAME_X(const DevScene* __restrict__ sc, FrameParams fp, FilmParams lp, float* __restrict__ beauty,
@@ -41,6 +44,7 @@ extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
 }
 
 // G-buffer pixels (16x16 tiles) or the pixels of a list (hybrid reprojection's holes).
+#if MM3E_MAXB <= 4
 extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
     GNAME(const DevScene* __restrict__ sc, FrameParams fp, float exposure, const GMover* __restrict__ movers,
           uint32_t n_movers, const uint32_t* __restrict__ list, const uint32_t* __restrict__ n_list,
@@ -48,6 +52,8 @@ extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
           float* __restrict__ depth, int32_t* __restrict__ obj) {
   gbuffer_tile<SimpleMap, MM3E_MAXB, false>(sc, fp, exposure, movers, n_movers, list, n_list, color, depth, obj);
 }
+
+#endif
 
 extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
     GNAME_X(const DevScene* __restrict__ sc, FrameParams fp, float exposure, const GMover* __restrict__ movers,
@@ -63,19 +69,27 @@ hipError_t GLNAME(hipStream_t st, const DevScene* sc, bool general, const FrameP
                   uint32_t max_list, uchar4* color, float* depth, int32_t* obj) {
   const dim3 grid = list ? dim3((max_list + 255) / 256) : dim3((fp.width + 15) / 16, (fp.height + 15) / 16);
   if (list && max_list == 0) return hipSuccess;
-  if (general)
-    hipLaunchKernelGGL(GNAME_X, grid, dim3(256), 0, st, sc, fp, exposure, movers, n_movers, list, n_list, color, depth,
-                       obj);
-  else
+#if MM3E_MAXB <= 4
+  if (!general) {
     hipLaunchKernelGGL(GNAME, grid, dim3(256), 0, st, sc, fp, exposure, movers, n_movers, list, n_list, color, depth,
                        obj);
+    return hipGetLastError();
+  }
+#endif
+  hipLaunchKernelGGL(GNAME_X, grid, dim3(256), 0, st, sc, fp, exposure, movers, n_movers, list, n_list, color, depth,
+                     obj);
   return hipGetLastError();
 }
 
 hipError_t LNAME(dim3 grid, dim3 block, hipStream_t st, const DevScene* sc, bool general, const FrameParams& fp,
                  const FilmParams& lp, float* beauty, float* fg, float* alpha, uint32_t tw, uint32_t th) {
-  if (general) hipLaunchKernelGGL(KNAME_X, grid, block, 0, st, sc, fp, lp, beauty, fg, alpha, tw, th);
-  else hipLaunchKernelGGL(KNAME, grid, block, 0, st, sc, fp, lp, beauty, fg, alpha, tw, th);
+#if MM3E_MAXB <= 4
+  if (!general) {
+    hipLaunchKernelGGL(KNAME, grid, block, 0, st, sc, fp, lp, beauty, fg, alpha, tw, th);
+    return hipGetLastError();
+  }
+#endif
+  hipLaunchKernelGGL(KNAME_X, grid, block, 0, st, sc, fp, lp, beauty, fg, alpha, tw, th);
   return hipGetLastError();
 }
 }  // namespace mm3e

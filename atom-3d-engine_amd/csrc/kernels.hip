@@ -37,6 +37,12 @@ extern "C" __global__ void __launch_bounds__(256)
 }
 
 extern "C" __global__ void __launch_bounds__(256)
+    mm3e_bloom_h_wide(PostParams pp, const float* __restrict__ kern, const float* __restrict__ hdr,
+                      float* __restrict__ tmp) {
+  bloom_h_wide(pp, kern, hdr, tmp, blockIdx.x * blockDim.x, blockIdx.y);
+}
+
+extern "C" __global__ void __launch_bounds__(256)
     mm3e_bloom_v_resolve(PostParams pp, const float* __restrict__ kern, const float* __restrict__ hdr,
                          const float* __restrict__ tmp, uchar4* __restrict__ rgba) {
   bloom_v_resolve(pp, kern, hdr, tmp, rgba, blockIdx.x * blockDim.x, blockIdx.y);
@@ -57,6 +63,15 @@ extern "C" __global__ void __launch_bounds__(256)
   const int x0 = blockIdx.x * blockDim.x;
   if (!P.pp.bloom || x0 >= (int)P.pp.width) return;  // uniform per block
   bloom_h(P.pp, P.kern, P.hdr, P.tmp, lds, x0, (int)(row - P.row_base));
+}
+
+extern "C" __global__ void __launch_bounds__(256)
+    mm3e_batch_bloom_h_wide(const BatchPost* __restrict__ posts, uint32_t n) {
+  const uint32_t row = blockIdx.y;
+  const BatchPost& P = posts[find_job(posts, n, row, [](const BatchPost& b) { return b.row_base; })];
+  const int x0 = blockIdx.x * blockDim.x;
+  if (!P.pp.bloom || x0 >= (int)P.pp.width) return;  // uniform per block
+  bloom_h_wide(P.pp, P.kern, P.hdr, P.tmp, x0, (int)(row - P.row_base));
 }
 
 extern "C" __global__ void __launch_bounds__(256)
@@ -114,7 +129,9 @@ hipError_t launch_render_table(uint32_t maxb, dim3 grid, dim3 block, hipStream_t
     case 2: return launch_render_table_b2(grid, block, st, sc, stage_bytes, general, fp, hdr, tw, th);
     case 3: return launch_render_table_b3(grid, block, st, sc, stage_bytes, general, fp, hdr, tw, th);
     case 4: return launch_render_table_b4(grid, block, st, sc, stage_bytes, general, fp, hdr, tw, th);
-    default: return hipErrorInvalidValue;
+    default:
+      return maxb <= kMaxBounces ? launch_render_table_b16(grid, block, st, sc, stage_bytes, general, fp, hdr, tw, th)
+                                 : hipErrorInvalidValue;
   }
 }
 
@@ -128,7 +145,7 @@ hipError_t launch_render_ordered(uint32_t maxb, uint32_t n_tiles, dim3 block, hi
     case 2: return MM3E_ORD(2);
     case 3: return MM3E_ORD(3);
     case 4: return MM3E_ORD(4);
-    default: return hipErrorInvalidValue;
+    default: return maxb <= kMaxBounces ? MM3E_ORD(16) : hipErrorInvalidValue;
   }
 #undef MM3E_ORD
 }
@@ -142,7 +159,9 @@ hipError_t launch_film(uint32_t maxb, dim3 grid, dim3 block, hipStream_t st, con
     case 2: return launch_film_b2(grid, block, st, sc, general, fp, lp, beauty, fg, alpha, tw, th);
     case 3: return launch_film_b3(grid, block, st, sc, general, fp, lp, beauty, fg, alpha, tw, th);
     case 4: return launch_film_b4(grid, block, st, sc, general, fp, lp, beauty, fg, alpha, tw, th);
-    default: return hipErrorInvalidValue;
+    default:
+      return maxb <= kMaxBounces ? launch_film_b16(grid, block, st, sc, general, fp, lp, beauty, fg, alpha, tw, th)
+                                 : hipErrorInvalidValue;
   }
 }
 
@@ -160,13 +179,18 @@ hipError_t launch_gi_reduce(dim3 grid, dim3 block, hipStream_t st, const GiBakeP
   return hipGetLastError();
 }
 
+// LDS of the horizontal bloom pass: a 256-pixel row segment and its halo; wider radii
+// (more than 2,602 pixels) read their taps from the cache instead (mm3e_bloom_h_wide).
+static const size_t kBloomLdsMax = 64 * 1024;
+
 hipError_t launch_post(hipStream_t st, const PostParams& pp, const float* kern, const float* hdr, float* tmp,
                        uchar4* rgba) {
   const uint32_t bx = 256;
   dim3 grid((pp.width + bx - 1) / bx, pp.height);
   if (pp.bloom) {
-    size_t lds = (size_t)(bx + 2 * pp.radius) * 3 * sizeof(float);
-    hipLaunchKernelGGL(mm3e_bloom_h, grid, dim3(bx), lds, st, pp, kern, hdr, tmp);
+    const size_t lds = (size_t)(bx + 2 * pp.radius) * 3 * sizeof(float);
+    if (lds <= kBloomLdsMax) hipLaunchKernelGGL(mm3e_bloom_h, grid, dim3(bx), lds, st, pp, kern, hdr, tmp);
+    else hipLaunchKernelGGL(mm3e_bloom_h_wide, grid, dim3(bx), 0, st, pp, kern, hdr, tmp);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(mm3e_bloom_v_resolve, grid, dim3(bx), 0, st, pp, kern, hdr, tmp, rgba);
@@ -180,8 +204,9 @@ hipError_t launch_batch_post(hipStream_t st, const BatchPost* posts, uint32_t n,
                              uint32_t max_radius) {
   const uint32_t bx = 256;
   dim3 grid((max_w + bx - 1) / bx, total_rows);
-  size_t lds = (size_t)(bx + 2 * max_radius) * 3 * sizeof(float);
-  hipLaunchKernelGGL(mm3e_batch_bloom_h, grid, dim3(bx), lds, st, posts, n);
+  const size_t lds = (size_t)(bx + 2 * max_radius) * 3 * sizeof(float);
+  if (lds <= kBloomLdsMax) hipLaunchKernelGGL(mm3e_batch_bloom_h, grid, dim3(bx), lds, st, posts, n);
+  else hipLaunchKernelGGL(mm3e_batch_bloom_h_wide, grid, dim3(bx), 0, st, posts, n);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(mm3e_batch_resolve, grid, dim3(bx), 0, st, posts, n);
@@ -201,7 +226,9 @@ hipError_t launch_batch_render(uint32_t maxb, uint32_t blocks, hipStream_t st, c
     case 2: return launch_batch_render_b2(blocks, st, jobs, n_jobs, total_tiles, counter, lds_bytes, q);
     case 3:
     case 4: return launch_batch_render_b4(blocks, st, jobs, n_jobs, total_tiles, counter, lds_bytes, q);
-    default: return hipErrorInvalidValue;
+    default:
+      return maxb <= kMaxBounces ? launch_batch_render_b16(blocks, st, jobs, n_jobs, total_tiles, counter, lds_bytes, q)
+                                 : hipErrorInvalidValue;
   }
 }
 
@@ -374,7 +401,7 @@ hipError_t launch_gbuffer(uint32_t maxb, hipStream_t st, const DevScene* sc, boo
     case 2: return MM3E_GB(2);
     case 3: return MM3E_GB(3);
     case 4: return MM3E_GB(4);
-    default: return hipErrorInvalidValue;
+    default: return maxb <= kMaxBounces ? MM3E_GB(16) : hipErrorInvalidValue;
   }
 #undef MM3E_GB
 }

@@ -19,8 +19,10 @@ struct BatchPost;
 MM3E_DECL_BATCH(1)
 MM3E_DECL_BATCH(2)
 MM3E_DECL_BATCH(4)
+MM3E_DECL_BATCH(16)
 #undef MM3E_DECL_BATCH
-// Persistent batched render; maxb is the batch's largest `bounces` (variants 1, 2, 4 cover 0..4).
+// Persistent batched render; maxb is the batch's largest `bounces` (variants 1, 2, 4, 16 cover
+// 0..16).
 // q.order == nullptr: tiles in index order, no cost record.
 hipError_t launch_batch_render(uint32_t maxb, uint32_t blocks, hipStream_t st, const BatchJob* jobs, uint32_t n_jobs,
                                uint32_t total_tiles, uint32_t* counter, uint32_t lds_bytes,
@@ -38,6 +40,7 @@ MM3E_DECL_RENDER(1)
 MM3E_DECL_RENDER(2)
 MM3E_DECL_RENDER(3)
 MM3E_DECL_RENDER(4)
+MM3E_DECL_RENDER(16)  // 5..16 bounces: the state machine only
 #undef MM3E_DECL_RENDER
 
 #define MM3E_DECL_ORD(B)                                                                                    \
@@ -49,6 +52,7 @@ MM3E_DECL_ORD(1)
 MM3E_DECL_ORD(2)
 MM3E_DECL_ORD(3)
 MM3E_DECL_ORD(4)
+MM3E_DECL_ORD(16)  // 5..16 bounces: the state machine only
 #undef MM3E_DECL_ORD
 // The table kernels with ordered tiles (one block per tile, block b renders o.order[b]).
 hipError_t launch_render_ordered(uint32_t maxb, uint32_t n_tiles, dim3 block, hipStream_t st, const DevScene* sc,
@@ -62,6 +66,7 @@ MM3E_DECL_WF(1)
 MM3E_DECL_WF(2)
 MM3E_DECL_WF(3)
 MM3E_DECL_WF(4)
+MM3E_DECL_WF(16)  // 5..16 bounces: the state machine only
 #undef MM3E_DECL_WF
 // The queued frame (dev_wavefront.h): counter reset; march pass k (0: one block per tile of
 // tw*th*aa*aa lanes; k >= 1: grid-stride over queue k-1); shading of the hit queue
@@ -97,6 +102,7 @@ MM3E_DECL_FILM(1)
 MM3E_DECL_FILM(2)
 MM3E_DECL_FILM(3)
 MM3E_DECL_FILM(4)
+MM3E_DECL_FILM(16)
 #undef MM3E_DECL_FILM
 struct GMover;
 #define MM3E_DECL_GBUF(B)                                                                                          \
@@ -109,6 +115,7 @@ MM3E_DECL_GBUF(1)
 MM3E_DECL_GBUF(2)
 MM3E_DECL_GBUF(3)
 MM3E_DECL_GBUF(4)
+MM3E_DECL_GBUF(16)
 #undef MM3E_DECL_GBUF
 // G-buffer of a frame (list == nullptr) or the display colour of the listed pixels (*n_list of
 // them in device memory, at most max_list).

@@ -1,6 +1,8 @@
 // render_kernel.hip — the per-pixel render kernels for one reflection depth.  Compiled once per
-// MM3E_MAXB (0..4) into separate objects so the variants build in parallel; the runtime picks
-// the variant equal to the scene's `bounces`, and per scene one of two families:
+// MM3E_MAXB (0..4, and 16 for 5..16 bounces) into separate objects so the variants build in
+// parallel; the runtime picks the variant equal to the scene's `bounces` (16 above 4), and per
+// scene one of two families (the deep variant has only the state machine, whose pending
+// reflection levels wait in a per-lane stack instead of nested inlined copies):
 //   mm3e_render_{table,lds}_bN     scenes without a segment grid or CSG objects: the world fold
 //                                  without those features, march/shade as nested loops;
 //   mm3e_render_{table,lds}_x_bN   every other scene: the general fold (segment grid, CSG) and
@@ -31,6 +33,7 @@
 
 using namespace mm3e;
 
+#if MM3E_MAXB <= 4
 // Scene tables read from HBM (scenes whose tables exceed the LDS budget).
 extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
     KNAME(const DevScene* __restrict__ sc, FrameParams fp, float* __restrict__ hdr, uint32_t tw, uint32_t th) {
@@ -48,6 +51,8 @@ extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
   render_tile<SimpleMap, MM3E_MAXB, false>(sc, tb, fp, hdr, tw, th, blockIdx.x, blockIdx.y, smem);
 }
 
+#endif
+
 extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
     KNAME_X(const DevScene* __restrict__ sc, FrameParams fp, float* __restrict__ hdr, uint32_t tw, uint32_t th) {
   __shared__ float smem[256 * 3];
@@ -63,6 +68,7 @@ extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
   render_tile<TableMap, MM3E_MAXB, true>(sc, tb, fp, hdr, tw, th, blockIdx.x, blockIdx.y, smem);
 }
 
+#if MM3E_MAXB <= 4
 // Ordered tiles (block b renders tile o.order[b]; see render_ordered), tables staged in LDS
 // when stage_bytes > 0, else read from HBM.
 extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
@@ -75,6 +81,8 @@ extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
   render_ordered<SimpleMap, MM3E_MAXB, false>(sc, tb, fp, hdr, tw, th, o, smem, &start);
 }
 
+#endif
+
 extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
     ONAME_X(const DevScene* __restrict__ sc, FrameParams fp, float* __restrict__ hdr, uint32_t tw, uint32_t th,
             uint32_t stage_bytes, TileOrder o) {
@@ -85,6 +93,7 @@ extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
   render_ordered<TableMap, MM3E_MAXB, true>(sc, tb, fp, hdr, tw, th, o, smem, &start);
 }
 
+#if MM3E_MAXB <= 4
 // Shading stage of the queued frame (dev_wavefront.h): the hit queue, nested loops or the
 // per-lane state machine.
 extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
@@ -93,6 +102,8 @@ extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
   const Tables tb = stage_bytes ? stage_tables(sc, tables_lds, stage_bytes) : global_tables(sc);
   wf_shade<SimpleMap, MM3E_MAXB, false>(sc, tb, fp, w);
 }
+
+#endif
 
 extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
     WNAME_X(const DevScene* __restrict__ sc, FrameParams fp, WfFrame w, uint32_t stage_bytes) {
@@ -104,27 +115,39 @@ extern "C" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY
 namespace mm3e {
 hipError_t WLNAME(dim3 grid, hipStream_t st, const DevScene* sc, uint32_t stage_bytes, bool general,
                   const FrameParams& fp, const WfFrame& w) {
-  if (general) hipLaunchKernelGGL(WNAME_X, grid, dim3(256), stage_bytes, st, sc, fp, w, stage_bytes);
-  else hipLaunchKernelGGL(WNAME, grid, dim3(256), stage_bytes, st, sc, fp, w, stage_bytes);
+#if MM3E_MAXB <= 4
+  if (!general) {
+    hipLaunchKernelGGL(WNAME, grid, dim3(256), stage_bytes, st, sc, fp, w, stage_bytes);
+    return hipGetLastError();
+  }
+#endif
+  hipLaunchKernelGGL(WNAME_X, grid, dim3(256), stage_bytes, st, sc, fp, w, stage_bytes);
   return hipGetLastError();
 }
 
 hipError_t OLNAME(uint32_t n_tiles, dim3 block, hipStream_t st, const DevScene* sc, uint32_t stage_bytes, bool general,
                   const FrameParams& fp, float* hdr, uint32_t tw, uint32_t th, const TileOrder& o) {
-  if (general) hipLaunchKernelGGL(ONAME_X, dim3(n_tiles), block, stage_bytes, st, sc, fp, hdr, tw, th, stage_bytes, o);
-  else hipLaunchKernelGGL(ONAME, dim3(n_tiles), block, stage_bytes, st, sc, fp, hdr, tw, th, stage_bytes, o);
+#if MM3E_MAXB <= 4
+  if (!general) {
+    hipLaunchKernelGGL(ONAME, dim3(n_tiles), block, stage_bytes, st, sc, fp, hdr, tw, th, stage_bytes, o);
+    return hipGetLastError();
+  }
+#endif
+  hipLaunchKernelGGL(ONAME_X, dim3(n_tiles), block, stage_bytes, st, sc, fp, hdr, tw, th, stage_bytes, o);
   return hipGetLastError();
 }
 
 hipError_t LNAME(dim3 grid, dim3 block, hipStream_t st, const DevScene* sc, uint32_t stage_bytes, bool general,
                  const FrameParams& fp, float* hdr, uint32_t tw, uint32_t th) {
-  if (general) {
-    if (stage_bytes) hipLaunchKernelGGL(KNAME_LDS_X, grid, block, stage_bytes, st, sc, fp, hdr, tw, th, stage_bytes);
-    else hipLaunchKernelGGL(KNAME_X, grid, block, 0, st, sc, fp, hdr, tw, th);
-  } else {
+#if MM3E_MAXB <= 4
+  if (!general) {
     if (stage_bytes) hipLaunchKernelGGL(KNAME_LDS, grid, block, stage_bytes, st, sc, fp, hdr, tw, th, stage_bytes);
     else hipLaunchKernelGGL(KNAME, grid, block, 0, st, sc, fp, hdr, tw, th);
+    return hipGetLastError();
   }
+#endif
+  if (stage_bytes) hipLaunchKernelGGL(KNAME_LDS_X, grid, block, stage_bytes, st, sc, fp, hdr, tw, th, stage_bytes);
+  else hipLaunchKernelGGL(KNAME_X, grid, block, 0, st, sc, fp, hdr, tw, th);
   return hipGetLastError();
 }
 }  // namespace mm3e

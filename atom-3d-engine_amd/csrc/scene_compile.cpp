@@ -20,6 +20,8 @@ namespace mm3e {
 #endif
 // Longest run folded linearly; longer runs walk their tree (O(log n) per evaluation).
 static const size_t kLinearRunMax = MM3E_LINEAR_RUN_MAX;
+// Largest bloom radius: the post pass keeps its 2r+1 Gaussian weights in HBM (8 MiB at most).
+static const uint32_t kMaxBloomRadius = 1u << 20;
 // Segment candidate grid: built for scenes with at least this many segments; about
 // kGridTargetCells cells; the level-l mask keeps the segments within slack + kGridReach[l] of a
 // cell.  The grid spans the bounded segments' spheres plus slack + kGridPadReach; a point
@@ -424,12 +426,12 @@ bool parse_scene_ir(const void* ir, size_t len_, HostScene& sc, std::string& err
     err = "aa > 16 is not supported by the MI355X render path";
     return false;
   }
-  if (h.bounces > 4) {
-    err = "bounces > 4 is not supported by the MI355X render path";
+  if (h.bounces > kMaxBounces) {
+    err = "bounces > 16 is not supported by the MI355X render path";
     return false;
   }
-  if (h.bloom && h.bloom_radius > 64) {
-    err = "bloom_radius > 64 is not supported by the MI355X render path";
+  if (h.bloom && h.bloom_radius > kMaxBloomRadius) {  // its 2r+1 Gaussian weights live in HBM
+    err = "bloom_radius > 1048576 is not supported by the MI355X render path";
     return false;
   }
   return true;

@@ -79,7 +79,7 @@ hipError_t launch_wf_shade(uint32_t maxb, dim3 grid, hipStream_t st, const DevSc
     case 2: return MM3E_WS(2);
     case 3: return MM3E_WS(3);
     case 4: return MM3E_WS(4);
-    default: return hipErrorInvalidValue;
+    default: return maxb <= kMaxBounces ? MM3E_WS(16) : hipErrorInvalidValue;
   }
 #undef MM3E_WS
 }

@@ -50,6 +50,16 @@ def test_scene_check_messages_match_reference():
     s4 = Scene(8, 8)
     s4.aa = 17
     assert "aa" in hip.scene_check(s4.to_ir(), 8, 8)
+    s6 = Scene(8, 8)
+    s6.bounces = 16
+    assert hip.scene_check(s6.to_ir(), 8, 8) is None
+    s6.bounces = 17
+    assert "bounces" in hip.scene_check(s6.to_ir(), 8, 8)
+    s5 = Scene(8, 8)
+    s5.post.bloom_radius = 1 << 20  # wide radii take the cache-read pass; only the weights' size is capped
+    assert hip.scene_check(s5.to_ir(), 8, 8) is None
+    s5.post.bloom_radius = (1 << 20) + 1
+    assert "bloom_radius" in hip.scene_check(s5.to_ir(), 8, 8)
 
 
 def test_renderer_create_fails_loudly_without_a_gpu():

@@ -312,6 +312,52 @@ def test_reflections_fog_two_bounces(renderer):
     image_parity(renderer, sc, cam, "bounces=2 fog aa=2")
 
 
+def _mirror_corridor(bounces):
+    """Two facing mirror walls with a sphere between and a reflective checkered floor: rays bounce
+    between the walls for as many levels as `bounces` allows."""
+    sc = Scene(128, 72)
+    sc.aa = 1
+    sc.bounces = bounces
+    sc.fog_density = 0.02
+    floor = sc.material(Material.solid((0.8, 0.8, 0.8)).checkered().roughness(0.4).reflective(0.3))
+    mirror = sc.material(Material.solid((0.9, 0.9, 0.95)).roughness(0.05).metallic(1.0).reflective(0.92))
+    red = sc.material(Material.solid((0.9, 0.15, 0.1)).roughness(0.3))
+    sc.add(Object(Prim.plane((0, 1, 0), 0.0), Transform(), floor))
+    sc.add(Object(Prim.box((0.1, 1.5, 4.0)), Transform.at((-1.6, 1.5, 0.0)), mirror))
+    sc.add(Object(Prim.box((0.1, 1.5, 4.0)), Transform.at((1.6, 1.5, 0.0)), mirror))
+    sc.add(Object(Prim.sphere(0.45), Transform.at((0.3, 0.9, -0.5)), red))
+    sc.light(Light.directional_light(sc.sun_dir, (1.3, 1.3, 1.3)).soft(0.05))
+    cam = configs.look_at((-0.9, 1.3, 3.6), (0.7, 1.0, -1.0), (0.0, 1.0, 0.0), np.radians(55.0))
+    return sc, cam
+
+
+@pytest.mark.parametrize("bounces", [5, 16])
+def test_deep_reflections_vs_oracle(renderer, bounces):
+    """Scene::bounces above 4 (the reference has no limit; this path takes up to 16): the
+    state-machine kernels with a 16-level per-lane stack, against the oracle; the scene's
+    generated kernel (forced onto the state machine) equals the table kernel bit for bit."""
+    sc, cam = _mirror_corridor(bounces)
+    image_parity(renderer, sc, cam, f"mirror corridor bounces={bounces}")
+    ir = sc.to_ir()
+    gt = renderer.compile(ir, sc.width, sc.height)
+    gen = hip.Renderer()
+    gen.set_codegen(hip.CODEGEN_SYNC)
+    gg = gen.compile(ir, sc.width, sc.height)
+    state, msg = gg.codegen_status(wait=True)
+    assert state == hip.CODEGEN_READY, msg
+    assert (gt.render_rgba(cam) == gg.render_rgba(cam)).all()
+    gg.close()
+    gen.close()
+    gt.close()
+    # a deeper recursion changes the image: the walls reflect each other past level 4
+    sc4, _ = _mirror_corridor(4)
+    g4 = renderer.compile(sc4.to_ir(), sc4.width, sc4.height)
+    g = renderer.compile(ir, sc.width, sc.height)
+    assert (g4.render_rgba(cam) != g.render_rgba(cam)).any()
+    g4.close()
+    g.close()
+
+
 def test_volume_prim_parity(renderer):
     sc = Scene(200, 120)
     sc.aa = 1
@@ -365,6 +411,26 @@ def test_compile_errors_on_device(renderer):
     with pytest.raises(hip.Mm3eError) as e:
         hip.Renderer("no-such-adapter-xyz")
     assert "no GPU adapter matching 'no-such-adapter-xyz'; available:" in str(e.value)
+
+
+@pytest.mark.parametrize("radius", [12, 2602, 2603])
+def test_bloom_radius_through_lds_and_wide_pass(renderer, radius):
+    """post.rs has no limit on bloom_radius: up to 2602 pixels the horizontal pass stages a row
+    segment and its halo in 64 KiB of LDS, above that it reads its taps through the cache.  The
+    device's RGBA equals the oracle's resolve (bloom + exposure + ACES + gamma) of the device's own
+    HDR frame on both sides of that switch (within one 8-bit step where device and host powf / expf
+    round differently, on at most 0.1 % of the channels)."""
+    sc, cam = configs.c1_scene(width=96, height=54)
+    sc.post.bloom, sc.post.bloom_radius, sc.post.bloom_threshold = True, radius, 0.5
+    ir = sc.to_ir()
+    g = renderer.compile(ir, sc.width, sc.height)
+    hdr = g.render_linear(cam)
+    img = g.render_rgba(cam).astype(int)
+    ref = po.OracleScene(ir).resolve(hdr).astype(int)
+    g.close()
+    d = np.abs(img - ref)[..., :3]
+    print(f"\n[bloom radius {radius}] max |diff| {d.max()}, channels differing {(d > 0).mean():.5f}", flush=True)
+    assert d.max() <= 1 and (d > 0).mean() <= 1e-3
 
 
 def test_batch_equals_per_frame_renders_across_replays(renderer):
