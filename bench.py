@@ -219,6 +219,13 @@ def served_leg(hip, configs, r, n_req, n_scenes=32, inflight=8, batch=2):
         return lat, span
 
     served_run([0.0] * len(reqs))  # warm the server's pinned staging and slab
+    # each request alone (plain launch, readback included): the latency floor no scheduling can beat
+    alone = []
+    for si, cam, (w, h) in reqs:
+        t1 = time.perf_counter()
+        plain_sc[(si, w, h)].render_rgba(cam)
+        alone.append((time.perf_counter() - t1) * 1e3)
+    log(f"served: each request alone: p50/p99/max {pct(alone, 50):.1f}/{pct(alone, 99):.1f}/{max(alone):.1f} ms")
     burst = [0.0] * len(reqs)
     plain_lat, plain_span = plain_run(burst)
     served_lat, served_span = served_run(burst)
@@ -256,6 +263,8 @@ def served_leg(hip, configs, r, n_req, n_scenes=32, inflight=8, batch=2):
                      "served": {"mpix_s": round(px / served_span / 1e6, 3),
                                 "p50_ms": round(pct(served_lat, 50), 2), "p99_ms": round(pct(served_lat, 99), 2)},
                      "served_vs_plain": round(plain_span / served_span, 3)},
+           "alone_ms": {"p50": round(pct(alone, 50), 2), "p99": round(pct(alone, 99), 2), "max": round(max(alone), 2),
+                        "over_50ms": sum(1 for a in alone if a > 50.0)},
            "poisson": points, "server_stats": st}
     log(f"served: burst plain {out['burst']['plain']['mpix_s']} Mpix/s, served {out['burst']['served']['mpix_s']} "
         f"Mpix/s ({out['burst']['served_vs_plain']}x)")
