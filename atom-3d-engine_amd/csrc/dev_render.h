@@ -447,17 +447,16 @@ template <class Map, int MAXB, bool SM>
 MM3E_DEV void render_ordered(const DevScene* __restrict__ scp, const Tables& tb, const FrameParams& fp,
                              float* __restrict__ hdr, uint32_t tw, uint32_t th, const TileOrder& o, float* smem,
                              uint32_t* /*unused*/) {
-  if (o.order == nullptr) {  // tile index order, no cost bookkeeping
-    render_tile<Map, MAXB, SM>(scp, tb, fp, hdr, tw, th, blockIdx.x % o.tiles_x, blockIdx.x / o.tiles_x, smem);
-    return;
-  }
-  const uint32_t tile = o.order[blockIdx.x];
+  // o.order null: tile index order, no cost bookkeeping.  One call site of render_tile either
+  // way, so the kernel holds one inlined copy of the march and the shading.
+  const bool ordered = o.order != nullptr;
+  const uint32_t tile = ordered ? o.order[blockIdx.x] : blockIdx.x;
   // Device-scope atomics only: a __threadfence here writes back the XCD's L2 and costs more
   // than the frame (measured: c2 1.4 -> 2.2 ms).  The cost is a heuristic for the next order,
   // so the start ticket is published by an atomic exchange and read back by an atomic add of 0.
-  if (threadIdx.x == 0) atomicExch(o.start + tile, atomicAdd(o.ticket, 1u));
+  if (ordered && threadIdx.x == 0) atomicExch(o.start + tile, atomicAdd(o.ticket, 1u));
   render_tile<Map, MAXB, SM>(scp, tb, fp, hdr, tw, th, tile % o.tiles_x, tile / o.tiles_x, smem);
-  if (__lane_id() == 0) {  // every lane of the wave is here: render_tile returns convergent
+  if (ordered && __lane_id() == 0) {  // every lane of the wave is here: render_tile returns convergent
     const uint32_t waves = (blockDim.x + 63u) / 64u;
     if (atomicAdd(o.done + tile, 1u) == waves - 1u) {  // the tile's last wave
       const uint32_t t0 = atomicAdd(o.start + tile, 0u);
