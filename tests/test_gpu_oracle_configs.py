@@ -7,7 +7,8 @@
   tile-queue launch, two cameras each, at 320x180.
 * the c4 server: four of the 32 seeded served scenes with 16^3 GI baked on the device and
   resident in the server, a burst of requests at sizes the oracle finishes in seconds; every
-  image equals the same request served alone and meets the oracle bars.
+  image equals the same request served alone and meets the oracle bars, through the table walk
+  and through the scenes' generated kernels.
 
 Bars: hit mask and material ids agree on >= 99.9 % of pixels, >= 99.9 % of pixels within
 2/255 on every channel (MAE reported).
@@ -104,10 +105,15 @@ def test_c3_batch_vs_oracle(renderer):
             g.close()
 
 
-def test_c4_server_vs_oracle_alone_and_in_mix(renderer):
+@pytest.mark.parametrize("kernels", ["table", "generated"])
+def test_c4_server_vs_oracle_alone_and_in_mix(renderer, gen_renderer, kernels):
     """Four served scenes (16^3 GI baked on the device at compile, resident in the server),
     twelve requests in one burst at 160x90 / 240x135 / 320x180: each result equals the same
-    request served alone, bit for bit, and meets the oracle bars."""
+    request served alone, bit for bit, and meets the oracle bars.  With code generation on
+    (the bench's server) each request renders through its scene's generated kernel on its
+    batch's stream; with it off, through the batch's persistent table-walk launch."""
+    if kernels == "generated":
+        renderer = gen_renderer
     all_scenes = configs.c4_scenes(n_scenes=32)
     pick = [0, 1, 2, 3]
     sizes = [(160, 90), (240, 135), (320, 180), (240, 135)]
@@ -119,6 +125,9 @@ def test_c4_server_vs_oracle_alone_and_in_mix(renderer):
         oracles.append(po.OracleScene(sc.to_ir()))
     for g in resident:
         assert g.gi_raw() is not None  # probes baked on the device and resident
+        if kernels == "generated":
+            state, msg = g.codegen_status(wait=True)
+            assert state == hip.CODEGEN_READY, msg
     srv = hip.Server(renderer, hbm_budget_bytes=512 << 20)
     try:
         ids = [srv.add_scene(g) for g in resident]
