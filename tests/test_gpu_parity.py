@@ -241,7 +241,8 @@ def test_c2_image_parity_and_gi(renderer, oracle_c2):
     rel = err / np.maximum(np.abs(go[3]), 1e-2)
     report("c2 gi cubes", {"exact": float((gg[3] == go[3]).mean()), "max_abs": float(err.max()),
                            "max_rel": float(rel.max()), "p999_rel": float(np.percentile(rel, 99.9))})
-    assert rel.max() < 1e-3
+    # the device bake performs the oracle's f32 operations in its order: identical cubes
+    assert np.array_equal(gg[3].view(np.uint32), np.asarray(go[3], np.float32).view(np.uint32))
 
 
 @pytest.mark.parametrize("seed,n", [(41, 60), (42, 180)])
