@@ -358,7 +358,8 @@ MM3E_DEV V3 trace_sm(const DevScene& sc, const Map& field, V3 ro, V3 rd, Hit* hi
   return ps.radiance;
 }
 
-// Render one tile: blockDim.x = tw*th*aa*aa lanes; lane -> (pixel in tile, sample).
+// Render one tile: blockDim.x = tw*th*aa*aa lanes (at most 256; above that the tile is one pixel
+// and its samples run in rounds); lane -> (pixel in tile, sample).
 // SM selects the state-machine path (long world folds) over the nested one.
 template <class Map, int MAXB, bool SM>
 MM3E_DEV void render_tile(const DevScene* __restrict__ scp, const Tables& tb, const FrameParams& fp,
@@ -369,6 +370,43 @@ MM3E_DEV void render_tile(const DevScene* __restrict__ scp, const Tables& tb, co
   const uint32_t aa = sc.aa;
   const uint32_t ns = aa * aa;
   const uint32_t t = threadIdx.x;
+  if (ns > blockDim.x) {
+    // More samples than lanes (aa > 16): the tile is one pixel and the block runs its samples
+    // in rounds of blockDim.x; the first lane adds each round's results in sample order, so the
+    // sum is the reference's sy-major sequence of additions.
+    const uint32_t px = tile_x * tw, py = tile_y * th;
+    const bool in = px < fp.width && py < fp.height;  // the same for the whole block
+    V3 acc = V3{0.0f, 0.0f, 0.0f};
+    for (uint32_t base = 0; base < ns; base += blockDim.x) {
+      const uint32_t s = base + t;
+      V3 rad = V3{0.0f, 0.0f, 0.0f};
+      if (in && s < ns) {
+        const uint32_t sy = s / aa, sx = s - sy * aa;
+        const float ox = ((float)sx + 0.5f) / (float)aa;
+        const float oy = ((float)sy + 0.5f) / (float)aa;
+        V3 ro, rd;
+        camera_ray(fp, (float)px + ox, (float)py + oy, ro, rd);
+        if constexpr (SM) rad = trace_sm<Map, MAXB>(sc, field, ro, rd);
+        else rad = trace_level<Map, 0, MAXB>(sc, field, ro, rd);
+      }
+      smem[3 * t + 0] = rad.x;
+      smem[3 * t + 1] = rad.y;
+      smem[3 * t + 2] = rad.z;
+      __syncthreads();
+      if (t == 0) {
+        const uint32_t n = ns - base < blockDim.x ? ns - base : blockDim.x;
+        for (uint32_t k = 0; k < n; k++) acc = acc + V3{smem[3 * k], smem[3 * k + 1], smem[3 * k + 2]};
+      }
+      __syncthreads();
+    }
+    if (in && t == 0) {
+      const float inv = 1.0f / (float)ns;
+      acc = scale(acc, inv);
+      float* o = hdr + 3 * ((size_t)py * fp.width + px);
+      o[0] = acc.x; o[1] = acc.y; o[2] = acc.z;
+    }
+    return;
+  }
   const uint32_t pl = t / ns, s = t - pl * ns;
   const uint32_t px = tile_x * tw + pl % tw, py = tile_y * th + pl / tw;
   const bool valid = pl < tw * th && px < fp.width && py < fp.height;

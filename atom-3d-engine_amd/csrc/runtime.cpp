@@ -246,7 +246,8 @@ hipError_t launch_generated_index(hipFunction_t fn, const DevScene* sc, const Fr
   FrameParams fp_arg = fp;
   float* hdr_arg = hdr;
   void* args[] = {&sc_arg, &fp_arg, &hdr_arg, &tw, &th, &stage_bytes, &o};
-  return hipModuleLaunchKernel(fn, tiles_x * tiles_y, 1, 1, tw * th * aa * aa, 1, 1, stage_bytes, st, args, nullptr);
+  return hipModuleLaunchKernel(fn, tiles_x * tiles_y, 1, 1, std::min(256u, tw * th * aa * aa), 1, 1, stage_bytes, st,
+                               args, nullptr);
 }
 
 static void gen_schedule(mm3e_scene* s) {
@@ -301,8 +302,10 @@ static TileOrder scene_tile_order(const mm3e_scene* s, uint32_t tiles_x) {
 }
 
 // The queued-frame setting in force for a scene: its own, else the renderer's.
+// A tile of more than 256 samples (aa > 16) renders in rounds inside one block, which the
+// queued frame's one-lane-per-sample layout does not have: such scenes take the one-kernel frame.
 static void wf_settings(const mm3e_scene* s, uint32_t* budget, uint32_t* passes, uint32_t* bpc) {
-  *budget = s->wf_own ? s->wf_set[0] : s->r->wf_budget;
+  *budget = s->aa * s->aa > 256u ? 0u : s->wf_own ? s->wf_set[0] : s->r->wf_budget;
   *passes = s->wf_own ? s->wf_set[1] : s->r->wf_passes;
   *bpc = s->wf_own ? s->wf_set[2] : s->r->wf_bpc;
 }
@@ -420,7 +423,7 @@ bool launch_frame(mm3e_scene* s, const DevScene* dsc, const FrameParams& fp, std
   const uint32_t stage = dsc == s->d_scene ? s->stage_bytes : s->dyn_stage_bytes;
   const bool general = dsc == s->d_scene ? s->general : s->dyn_general;
   dim3 grid((fp.width + s->tw - 1) / s->tw, (fp.height + s->th - 1) / s->th);
-  dim3 block(s->tw * s->th * s->aa * s->aa);
+  dim3 block(std::min(256u, s->tw * s->th * s->aa * s->aa));  // aa > 16: one pixel, samples in rounds
   hipEvent_t* ev = s->ev[s->frames % mm3e_scene::kRing];
   hipEventRecord(ev[0], st);
   if (s->host.hdr.mode != 0) {  // debug render mode: straight to RGBA8, no post (lib.rs:873-878)
@@ -998,6 +1001,10 @@ int mm3e_scene_render_film(mm3e_scene* s, const mm3e_camera* cam, const mm3e_len
   }
   if (!std::isfinite(lens->focus_distance) || lens->focus_distance <= 0.0f) {
     set_err(err, errlen, "focus_distance must be finite and positive");
+    return MM3E_ERR_INVALID;
+  }
+  if (s->aa > 16) {  // film_tile keeps one lane per sample: at most 256 samples per pixel
+    set_err(err, errlen, "render_film supports aa <= 16 on the MI355X render path");
     return MM3E_ERR_INVALID;
   }
   std::lock_guard<std::mutex> lk(s->r->mu);

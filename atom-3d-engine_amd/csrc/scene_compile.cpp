@@ -20,6 +20,8 @@ namespace mm3e {
 #endif
 // Longest run folded linearly; longer runs walk their tree (O(log n) per evaluation).
 static const size_t kLinearRunMax = MM3E_LINEAR_RUN_MAX;
+// Largest AA factor: aa^2 = 65,536 samples per pixel, rendered 256 at a time by one block.
+static const uint32_t kMaxAA = 256;
 // Largest bloom radius: the post pass keeps its 2r+1 Gaussian weights in HBM (8 MiB at most).
 static const uint32_t kMaxBloomRadius = 1u << 20;
 // Segment candidate grid: built for scenes with at least this many segments; about
@@ -422,8 +424,8 @@ bool parse_scene_ir(const void* ir, size_t len_, HostScene& sc, std::string& err
     return false;
   }
   uint32_t aa = h.aa == 0 ? 1 : h.aa;
-  if (aa > 16) {
-    err = "aa > 16 is not supported by the MI355X render path";
+  if (aa > kMaxAA) {
+    err = "aa > 256 is not supported by the MI355X render path";
     return false;
   }
   if (h.bounces > kMaxBounces) {

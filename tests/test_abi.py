@@ -48,7 +48,9 @@ def test_scene_check_messages_match_reference():
     assert "magic" in hip.scene_check(bytes(bad), 32, 18)
     assert "length" in hip.scene_check(ir[:-4], 32, 18)
     s4 = Scene(8, 8)
-    s4.aa = 17
+    s4.aa = 256  # 65,536 samples per pixel, 256 at a time
+    assert hip.scene_check(s4.to_ir(), 8, 8) is None
+    s4.aa = 257
     assert "aa" in hip.scene_check(s4.to_ir(), 8, 8)
     s6 = Scene(8, 8)
     s6.bounces = 16

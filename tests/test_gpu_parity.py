@@ -332,6 +332,29 @@ def _mirror_corridor(bounces):
     return sc, cam
 
 
+def test_more_samples_than_lanes_vs_oracle(renderer, gen_renderer):
+    """aa = 17 (289 samples per pixel, more than a block's 256 lanes): each block renders one
+    pixel in rounds and sums the samples in the reference's order.  Table and generated kernels
+    against the oracle, equal to each other and to the same frame as a batched job; the film
+    (one lane per sample) rejects it."""
+    sc, cam = configs.c1_scene(width=40, height=24)
+    sc.aa = 17
+    g, o, img, _ = image_parity(renderer, sc, cam, "c1 40x24 aa=17")
+    gg = gen_renderer.compile(sc.to_ir(), sc.width, sc.height)
+    state, msg = gg.codegen_status(wait=True)
+    assert state == hip.CODEGEN_READY, msg
+    assert (gg.render_rgba(cam) == img).all()
+    b = hip.Batch(renderer, [(g, cam, (sc.width, sc.height))])
+    b.launch()
+    b.wait()
+    assert (b.read(0) == img).all()
+    b.close()
+    with pytest.raises(hip.Mm3eError):
+        g.render_film(cam, aperture_radius=0.0, focus_distance=1.0)
+    gg.close()
+    g.close()
+
+
 @pytest.mark.parametrize("bounces", [5, 16])
 def test_deep_reflections_vs_oracle(renderer, bounces):
     """Scene::bounces above 4 (the reference has no limit; this path takes up to 16): the
