@@ -563,13 +563,27 @@ MM3E_DEV GridSel grid_select(const DevScene& sc, V3 p, const Field& acc) {
   GridSel g;
   g.on = false;
   const float thr = rmax(acc.dist + sc.grid_kmax, sc.slack) * 1.0001f + sc.grid_eps;
-  if (__builtin_amdgcn_ballot_w64(!(thr < sc.grid_m[kGridLevels - 1])) != 0) return g;
+  const uint32_t cell = grid_cell(sc, p);
+  // A point outside the grid whose distance to the box of the bounded segments' spheres is above
+  // its threshold (with margins for the f32 arithmetic) has every bounded member farther than the
+  // threshold: it folds only the unbounded segments (the outside cell's level 0), whatever its
+  // threshold — rays far beyond the scene, toward the horizon or the sun.
+  bool far = false;
+  if (cell == sc.grid_dims[0] * sc.grid_dims[1] * sc.grid_dims[2]) {
+    const float dx = rmax(rmax(sc.grid_in_lo[0] - p.x, p.x - sc.grid_in_hi[0]), 0.0f);
+    const float dy = rmax(rmax(sc.grid_in_lo[1] - p.y, p.y - sc.grid_in_hi[1]), 0.0f);
+    const float dz = rmax(rmax(sc.grid_in_lo[2] - p.z, p.z - sc.grid_in_hi[2]), 0.0f);
+    const float tf = thr * 1.0002f + sc.grid_eps;
+    far = dx * dx + dy * dy + dz * dz > tf * tf;
+  }
+  if (__builtin_amdgcn_ballot_w64(!far && !(thr < sc.grid_m[kGridLevels - 1])) != 0) return g;
   uint32_t level = kGridLevels - 1;
   for (int l = (int)kGridLevels - 2; l >= 0; l--)
     if (thr < sc.grid_m[l]) level = (uint32_t)l;
+  if (far) level = 0;
   g.on = true;
   g.W = sc.grid_words;
-  g.base = (grid_cell(sc, p) * kGridLevels + level) * g.W;
+  g.base = (cell * kGridLevels + level) * g.W;
   g.m = sc.grid;
   return g;
 }

@@ -177,12 +177,16 @@ struct alignas(16) DevScene {
   // from grid_lo; cell (x, y, z) has index (z * ny + y) * nx + x and index nx*ny*nz stands for
   // every point outside the grid.  Each cell holds kGridLevels bit masks over the segments
   // (grid_words words each, [cell][level][word]): bit s of level l is clear only if segment s's
-  // members all lie farther than grid_m[l] from every point of the cell.  Built only for scenes
+  // members all lie farther than grid_m[l] from every point of the cell; the outside cell's level
+  // 0 holds only the unbounded segments.  Built only for scenes
   // without subtract segments, where the accumulator never increases along the fold.
   float grid_lo[3], grid_inv;
   uint32_t grid_dims[3], grid_words;
   float grid_m[6];
   float grid_kmax, grid_eps;
+  // Box around the bounded segments' spheres (rounded outwards): a point outside the grid whose
+  // distance to this box exceeds its threshold is farther than that from every bounded member.
+  float grid_in_lo[3], grid_in_hi[3];
   const uint32_t* grid;
   // Postfix CSG programs; a CSG object's p[0] / p[1] hold its first op and op count (bits).
   const DCsgOp* csg;

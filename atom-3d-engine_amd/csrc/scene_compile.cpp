@@ -623,6 +623,11 @@ void build_seg_grid(const std::vector<DSeg>& segs, const std::vector<std::vector
     }
   }
   if (!any_bounded) return;
+  float in_lo[3], in_hi[3];  // the spheres' box, rounded outwards to float
+  for (int a = 0; a < 3; a++) {
+    in_lo[a] = std::nextafter((float)lo[a], -INFINITY);
+    in_hi[a] = std::nextafter((float)hi[a], INFINITY);
+  }
   double mreach[kGridLevels];
   for (uint32_t l = 0; l < kGridLevels; l++) mreach[l] = (double)slack + kGridReach[l];
   const double m1 = (double)slack + kGridPadReach;  // padding of the grid around the spheres
@@ -693,6 +698,8 @@ void build_seg_grid(const std::vector<DSeg>& segs, const std::vector<std::vector
   for (int a = 0; a < 3; a++) {
     g.lo[a] = lof[a];
     g.dims[a] = dims[a];
+    g.in_lo[a] = in_lo[a];
+    g.in_hi[a] = in_hi[a];
   }
   g.inv = inv;
   g.n_words = W;
@@ -1021,6 +1028,8 @@ void build_blob(const HostScene& sc, const std::vector<mm3e_ir_object>& extra_ob
     for (int a = 0; a < 3; a++) {
       d.grid_lo[a] = grid.lo[a];
       d.grid_dims[a] = grid.dims[a];
+      d.grid_in_lo[a] = grid.in_lo[a];
+      d.grid_in_hi[a] = grid.in_hi[a];
     }
     d.grid_inv = grid.inv;
     d.grid_words = grid.n_words;

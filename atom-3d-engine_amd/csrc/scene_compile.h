@@ -75,6 +75,7 @@ struct SegGrid {
   float lo[3] = {0, 0, 0}, inv = 0;
   uint32_t dims[3] = {0, 0, 0}, n_words = 0;
   float m[kGridLevels] = {}, kmax = 0, eps = 0;
+  float in_lo[3] = {0, 0, 0}, in_hi[3] = {0, 0, 0};  // box of the bounded segments' spheres
   std::vector<uint32_t> words;  // [cell (+1 outside)][level][word]
 };
 void build_seg_grid(const std::vector<DSeg>& segs, const std::vector<std::vector<uint32_t>>& members,

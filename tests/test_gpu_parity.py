@@ -57,15 +57,21 @@ def test_field_parity(renderer, which):
 @pytest.mark.parametrize("seed,n", [(31, 150), (44, 200)])
 def test_field_parity_segment_grid(renderer, seed, n):
     """Spatially coherent probes (64 points per wave within a 0.3-unit box, centres from the
-    floor to above the scene and beyond the grid) so waves take the segment-grid path at both
-    mask levels; every value must match the oracle's full world-plan fold."""
+    floor to above the scene, beyond the grid and far beyond it, near the floor's horizon) so waves
+    take the segment-grid path at every mask level and the far-point test; every value must match
+    the oracle's full world-plan fold."""
     sc = configs.procedural_scene(seed, n, 64, 36)
     g, o = both(renderer, sc)
     info = g.info()
     assert info["grid_cells"] > 0 and info["segments"] >= 8
     rng = np.random.default_rng(seed)
+    # far beyond the grid: toward the horizon just above / below the floor, and anywhere
+    ang = rng.uniform(0, 2 * np.pi, 90)
+    rad = np.exp(rng.uniform(np.log(18.0), np.log(5000.0), 90))
+    far = np.stack([rad * np.cos(ang), np.where(np.arange(90) < 45, rng.uniform(-0.5, 1.5, 90),
+                                                rng.uniform(-0.5, 1.0, 90) * rad), rad * np.sin(ang)], 1)
     centres = np.concatenate([rng.uniform([-8, -0.3, -8], [8, 3.5, 8], (300, 3)),
-                              rng.uniform([-14, -1, -14], [14, 9, 14], (60, 3))])
+                              rng.uniform([-14, -1, -14], [14, 9, 14], (60, 3)), far])
     pts = (centres[:, None, :] + rng.uniform(-0.15, 0.15, (len(centres), 64, 3))).reshape(-1, 3).astype(np.float32)
     dg, mg = g.field(pts)
     do, mo = o.field(pts)
