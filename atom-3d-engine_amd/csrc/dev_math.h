@@ -39,8 +39,12 @@ MM3E_DEV float length_sq(V3 a) { return dot(a, a); }
 // x - n*s says so — the compiler's IEEE sequence without its tiny-input rescaling (taken only
 // below 2^-96, where the residuals would lose bits; zero and negative inputs go there too) and
 // without its +-0 / +inf class fix-up, which the two comparisons already leave right (a NaN
-// residual compares false).  Six fewer VALU instructions per root on the common path.
+// residual compares false).  Six fewer VALU instructions per root on the common path.  An
+// argument known at compile time (a generated kernel's object constants) takes __builtin_sqrtf,
+// which the compiler folds to the correctly rounded constant; the hardware estimate would
+// instead be computed at kernel start and held in a register, or spilled, for the whole kernel.
 MM3E_DEV float crsqrt(float x) {
+  if (__builtin_constant_p(x)) return __builtin_sqrtf(x);
   if (__builtin_expect(!(x >= 0x1p-96f), 0)) return __builtin_sqrtf(x);  // tiny, zero, negative, NaN
   const float s = __builtin_amdgcn_sqrtf(x);
   const float sdn = __uint_as_float(__float_as_uint(s) - 1u);

@@ -370,73 +370,74 @@ MM3E_DEV void render_tile(const DevScene* __restrict__ scp, const Tables& tb, co
   const uint32_t aa = sc.aa;
   const uint32_t ns = aa * aa;
   const uint32_t t = threadIdx.x;
-  if (ns > blockDim.x) {
-    // More samples than lanes (aa > 16): the tile is one pixel and the block runs its samples
-    // in rounds of blockDim.x; the first lane adds each round's results in sample order, so the
-    // sum is the reference's sy-major sequence of additions.
-    const uint32_t px = tile_x * tw, py = tile_y * th;
-    const bool in = px < fp.width && py < fp.height;  // the same for the whole block
-    V3 acc = V3{0.0f, 0.0f, 0.0f};
-    for (uint32_t base = 0; base < ns; base += blockDim.x) {
-      const uint32_t s = base + t;
-      V3 rad = V3{0.0f, 0.0f, 0.0f};
-      if (in && s < ns) {
-        const uint32_t sy = s / aa, sx = s - sy * aa;
-        const float ox = ((float)sx + 0.5f) / (float)aa;
-        const float oy = ((float)sy + 0.5f) / (float)aa;
-        V3 ro, rd;
-        camera_ray(fp, (float)px + ox, (float)py + oy, ro, rd);
-        if constexpr (SM) rad = trace_sm<Map, MAXB>(sc, field, ro, rd);
-        else rad = trace_level<Map, 0, MAXB>(sc, field, ro, rd);
-      }
+  // More samples than lanes (aa > 16): the tile is one pixel and the block runs its samples in
+  // rounds of blockDim.x; the first lane adds each round's results in sample order, so the sum
+  // is the reference's sy-major sequence of additions.  Both forms share one trace call site,
+  // so the kernel holds a single inlined copy of the march and the shading.
+  const bool rounds = ns > blockDim.x;
+  const uint32_t n_rounds = rounds ? (ns + blockDim.x - 1u) / blockDim.x : 1u;
+  V3 sum = V3{0.0f, 0.0f, 0.0f};
+  for (uint32_t r = 0; r < n_rounds; r++) {
+    const uint32_t pl = rounds ? 0u : t / ns;
+    const uint32_t s = rounds ? r * blockDim.x + t : t - pl * ns;
+    const uint32_t px = tile_x * tw + pl % tw, py = tile_y * th + pl / tw;
+    const bool valid = pl < tw * th && px < fp.width && py < fp.height && s < ns;
+    V3 rad = V3{0.0f, 0.0f, 0.0f};
+    if (valid) {
+      const uint32_t sy = s / aa, sx = s - sy * aa;
+      const float ox = ((float)sx + 0.5f) / (float)aa;
+      const float oy = ((float)sy + 0.5f) / (float)aa;
+      V3 ro, rd;
+      camera_ray(fp, (float)px + ox, (float)py + oy, ro, rd);
+      if constexpr (SM) rad = trace_sm<Map, MAXB>(sc, field, ro, rd);
+      else rad = trace_level<Map, 0, MAXB>(sc, field, ro, rd);
+    }
+    if (rounds) {
       smem[3 * t + 0] = rad.x;
       smem[3 * t + 1] = rad.y;
       smem[3 * t + 2] = rad.z;
       __syncthreads();
       if (t == 0) {
-        const uint32_t n = ns - base < blockDim.x ? ns - base : blockDim.x;
-        for (uint32_t k = 0; k < n; k++) acc = acc + V3{smem[3 * k], smem[3 * k + 1], smem[3 * k + 2]};
+        const uint32_t n = ns - r * blockDim.x < blockDim.x ? ns - r * blockDim.x : blockDim.x;
+        for (uint32_t k = 0; k < n; k++) sum = sum + V3{smem[3 * k], smem[3 * k + 1], smem[3 * k + 2]};
       }
       __syncthreads();
+      continue;
     }
-    if (in && t == 0) {
-      const float inv = 1.0f / (float)ns;
-      acc = scale(acc, inv);
-      float* o = hdr + 3 * ((size_t)py * fp.width + px);
-      o[0] = acc.x; o[1] = acc.y; o[2] = acc.z;
+    if (ns == 1) {
+      if (valid) {
+        float* o = hdr + 3 * ((size_t)py * fp.width + px);
+        // acc = ZERO + s0; acc.scale(1/1)
+        V3 acc = V3{0.0f, 0.0f, 0.0f} + rad;
+        o[0] = acc.x * 1.0f; o[1] = acc.y * 1.0f; o[2] = acc.z * 1.0f;
+      }
+      return;
     }
-    return;
-  }
-  const uint32_t pl = t / ns, s = t - pl * ns;
-  const uint32_t px = tile_x * tw + pl % tw, py = tile_y * th + pl / tw;
-  const bool valid = pl < tw * th && px < fp.width && py < fp.height;
-  V3 rad = V3{0.0f, 0.0f, 0.0f};
-  if (valid) {
-    const uint32_t sy = s / aa, sx = s - sy * aa;
-    const float ox = ((float)sx + 0.5f) / (float)aa;
-    const float oy = ((float)sy + 0.5f) / (float)aa;
-    V3 ro, rd;
-    camera_ray(fp, (float)px + ox, (float)py + oy, ro, rd);
-    if constexpr (SM) rad = trace_sm<Map, MAXB>(sc, field, ro, rd);
-    else rad = trace_level<Map, 0, MAXB>(sc, field, ro, rd);
-  }
-  if (ns == 1) {
-    if (valid) {
-      float* o = hdr + 3 * ((size_t)py * fp.width + px);
-      // acc = ZERO + s0; acc.scale(1/1)
+    if ((64u % ns) == 0u) {
+      // A pixel's samples sit in adjacent lanes of one wave (lane t = pixel * ns + sample with
+      // ns | 64): its first lane sums them by cross-lane reads in sample order, the same additions
+      // as below, and no wave waits for the rest of the block.
       V3 acc = V3{0.0f, 0.0f, 0.0f} + rad;
-      o[0] = acc.x * 1.0f; o[1] = acc.y * 1.0f; o[2] = acc.z * 1.0f;
+      for (uint32_t k = 1; k < ns; k++)
+        acc = acc + V3{__shfl_down(rad.x, k), __shfl_down(rad.y, k), __shfl_down(rad.z, k)};
+      if (valid && s == 0) {
+        const float inv = 1.0f / (float)ns;
+        acc = scale(acc, inv);
+        float* o = hdr + 3 * ((size_t)py * fp.width + px);
+        o[0] = acc.x; o[1] = acc.y; o[2] = acc.z;
+      }
+      return;
     }
-    return;
-  }
-  if ((64u % ns) == 0u) {
-    // A pixel's samples sit in adjacent lanes of one wave (lane t = pixel * ns + sample with
-    // ns | 64): its first lane sums them by cross-lane reads in sample order, the same additions
-    // as below, and no wave waits for the rest of the block.
-    V3 acc = V3{0.0f, 0.0f, 0.0f} + rad;
-    for (uint32_t k = 1; k < ns; k++)
-      acc = acc + V3{__shfl_down(rad.x, k), __shfl_down(rad.y, k), __shfl_down(rad.z, k)};
+    smem[3 * t + 0] = rad.x;
+    smem[3 * t + 1] = rad.y;
+    smem[3 * t + 2] = rad.z;
+    __syncthreads();
     if (valid && s == 0) {
+      V3 acc = V3{0.0f, 0.0f, 0.0f};
+      for (uint32_t k = 0; k < ns; k++) {
+        const float* q = smem + 3 * (t + k);
+        acc = acc + V3{q[0], q[1], q[2]};
+      }
       const float inv = 1.0f / (float)ns;
       acc = scale(acc, inv);
       float* o = hdr + 3 * ((size_t)py * fp.width + px);
@@ -444,20 +445,13 @@ MM3E_DEV void render_tile(const DevScene* __restrict__ scp, const Tables& tb, co
     }
     return;
   }
-  smem[3 * t + 0] = rad.x;
-  smem[3 * t + 1] = rad.y;
-  smem[3 * t + 2] = rad.z;
-  __syncthreads();
-  if (valid && s == 0) {
-    V3 acc = V3{0.0f, 0.0f, 0.0f};
-    for (uint32_t k = 0; k < ns; k++) {
-      const float* q = smem + 3 * (t + k);
-      acc = acc + V3{q[0], q[1], q[2]};
-    }
+  // rounds: the block's one pixel
+  const uint32_t px = tile_x * tw, py = tile_y * th;
+  if (t == 0 && px < fp.width && py < fp.height) {
     const float inv = 1.0f / (float)ns;
-    acc = scale(acc, inv);
+    sum = scale(sum, inv);
     float* o = hdr + 3 * ((size_t)py * fp.width + px);
-    o[0] = acc.x; o[1] = acc.y; o[2] = acc.z;
+    o[0] = sum.x; o[1] = sum.y; o[2] = sum.z;
   }
 }
 
