@@ -13,6 +13,7 @@
 // Constants are written as bit patterns, so the text round-trips every float exactly.
 #include "codegen.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -208,20 +209,31 @@ std::string emit_scene_source(const Blob& b, uint32_t maxb, bool sm, bool queued
 std::string emit_batch_source(const std::vector<const Blob*>& maps, uint32_t maxb, bool staged) {
   std::string o;
   o += "// generated: batch module with " + std::to_string(maps.size()) + " scene maps\n";
-  // the table kernels' occupancy: every map runs the state machine, as the table walk does
-  o += "#define MM3E_RENDER_WAVES 4\n";
+  // every map runs the state machine, as the table walk does; the occupancy of a single scene's
+  // state-machine kernel for the longest map (MM3E_GEN_WAVES overrides)
+  size_t most = 0;
+  for (const Blob* m : maps) most = std::max(most, m->objs.size());
+  int waves = most > 48 ? 5 : 4;
+  if (const char* wv = std::getenv("MM3E_GEN_WAVES")) {
+    const int n = std::atoi(wv);
+    if (n >= 1 && n <= 8) waves = n;
+  }
+  o += "#define MM3E_RENDER_WAVES " + std::to_string(waves) + "\n";
   o += "#ifndef __HIPCC_RTC__\n#include <hip/hip_runtime.h>\n#endif\n#include \"dev_batch.h\"\nnamespace mm3e {\n";
   for (size_t k = 0; k < maps.size(); k++) emit_map(o, *maps[k], "GenMap" + std::to_string(k));
+  // One map for the whole module, switching on the job's scene map (uniform per tile) inside the
+  // fold: the kernel holds one inlined copy of the march and shading and one fold per scene.
+  o += "struct MultiMap {\n  const DevScene* sc;\n  Tables tb;\n  uint32_t k;  // the job's scene map\n"
+       "  MM3E_DEV Field operator()(V3 p) const {\n    switch (k) {\n";
+  for (size_t k = 0; k < maps.size(); k++)
+    o += "      case " + std::to_string(k) + "u: return GenMap" + std::to_string(k) + "{sc, tb}(p);\n";
+  o += "      default: return TableMap{sc, tb}(p);\n    }\n  }\n};\n";
   o += "}  // namespace mm3e\nusing namespace mm3e;\n";
   const std::string mb = std::to_string(maxb);
   o += "struct GenTiles {\n"
        "  MM3E_DEV void operator()(const BatchJob& J, const Tables& tb, uint32_t tx, uint32_t ty, float* smem) const {\n"
-       "    switch (J.gen) {\n";
-  for (size_t k = 0; k < maps.size(); k++)
-    o += "      case " + std::to_string(k) + "u: render_tile<GenMap" + std::to_string(k) + ", " + mb +
-         ", true>(J.sc, tb, J.fp, J.hdr, J.tw, J.th, tx, ty, smem); return;\n";
-  o += "      default: render_tile<TableMap, " + mb + ", true>(J.sc, tb, J.fp, J.hdr, J.tw, J.th, tx, ty, smem);\n"
-       "    }\n  }\n};\n";
+       "    render_tile_map<MultiMap, " + mb + ", true>(MultiMap{J.sc, tb, J.gen}, J.sc, J.fp, J.hdr, J.tw, J.th, tx, ty, smem);\n"
+       "  }\n};\n";
   o += "extern \"C\" __global__ void __launch_bounds__(256) MM3E_RENDER_OCCUPANCY\n"
        "    mm3e_gen_batch(const BatchJob* __restrict__ jobs, uint32_t n_jobs, uint32_t total_tiles,\n"
        "                   uint32_t* __restrict__ counter, TileOrder q) {\n"

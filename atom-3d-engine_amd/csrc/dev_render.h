@@ -362,11 +362,10 @@ MM3E_DEV V3 trace_sm(const DevScene& sc, const Map& field, V3 ro, V3 rd, Hit* hi
 // and its samples run in rounds); lane -> (pixel in tile, sample).
 // SM selects the state-machine path (long world folds) over the nested one.
 template <class Map, int MAXB, bool SM>
-MM3E_DEV void render_tile(const DevScene* __restrict__ scp, const Tables& tb, const FrameParams& fp,
-                          float* __restrict__ hdr, uint32_t tw, uint32_t th, uint32_t tile_x, uint32_t tile_y,
-                          float* smem) {
+MM3E_DEV void render_tile_map(const Map& field, const DevScene* __restrict__ scp, const FrameParams& fp,
+                              float* __restrict__ hdr, uint32_t tw, uint32_t th, uint32_t tile_x, uint32_t tile_y,
+                              float* smem) {
   const DevScene& sc = *scp;
-  Map field{scp, tb};
   const uint32_t aa = sc.aa;
   const uint32_t ns = aa * aa;
   const uint32_t t = threadIdx.x;
@@ -453,6 +452,14 @@ MM3E_DEV void render_tile(const DevScene* __restrict__ scp, const Tables& tb, co
     float* o = hdr + 3 * ((size_t)py * fp.width + px);
     o[0] = sum.x; o[1] = sum.y; o[2] = sum.z;
   }
+}
+
+// render_tile_map with the map built from the scene and its tables.
+template <class Map, int MAXB, bool SM>
+MM3E_DEV void render_tile(const DevScene* __restrict__ scp, const Tables& tb, const FrameParams& fp,
+                          float* __restrict__ hdr, uint32_t tw, uint32_t th, uint32_t tile_x, uint32_t tile_y,
+                          float* smem) {
+  render_tile_map<Map, MAXB, SM>(Map{scp, tb}, scp, fp, hdr, tw, th, tile_x, tile_y, smem);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -131,7 +131,11 @@ def batched_leg(hip, configs, r, reps):
     report = b.autotune(reps=1)
     tune_s = time.perf_counter() - t0
     info = b.info()
-    log(f"batched: autotune {tune_s:.1f} s -> tile {info['tile']}, {info['blocks_per_cu']} blocks/CU")
+    chosen = next((ln[len("chosen: "):] for ln in report.splitlines() if ln.startswith("chosen: ")), "")
+    mixed = chosen.startswith("mixed")
+    for ln in report.splitlines():
+        log("batched autotune: " + ln)
+    log(f"batched: autotune {tune_s:.1f} s -> {chosen}")
     b.launch()
     b.wait()
     wall, gpu = [], []
@@ -148,7 +152,9 @@ def batched_leg(hip, configs, r, reps):
     roof = None
     if fl and fl.get("pixels") == px:
         ach = fl["flops_per_frame"] / (min(gpu) * 1e-3) / 1e12
-        roof = {"bound": "valu", "kernel": "persistent batch render (queue reset + render + post)",
+        kernel = ("each job's generated scene kernel, four streams, then the batch post" if mixed else
+                  "persistent batch render (queue reset + render + post)")
+        roof = {"bound": "valu", "kernel": kernel,
                 "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(ach / PEAK_FP32_TFLOPS, 4), "frac_nonpacked_peak": round(ach / PEAK_FP32_NONPACKED_TFLOPS, 4),
                 "flops_per_launch": fl["flops_per_frame"], "flops_sample": fl["sample"]}
@@ -159,7 +165,9 @@ def batched_leg(hip, configs, r, reps):
            "plain_mpix_s": round(px / plain_s / 1e6, 3), "plain_s": round(plain_s, 4),
            "batched_mpix_s": round(px / batched_s / 1e6, 3), "batched_s": round(batched_s, 4),
            "batched_gpu_ms": round(min(gpu), 3), "speedup_vs_plain": round(plain_s / batched_s, 3),
-           "tile": list(info["tile"]), "blocks_per_cu": info["blocks_per_cu"], "autotune_s": round(tune_s, 2)}
+           "mode": chosen, "autotune_s": round(tune_s, 2)}
+    if not mixed:
+        out["tile"], out["blocks_per_cu"] = list(info["tile"]), info["blocks_per_cu"]
     log(f"batched: plain {out['plain_mpix_s']} Mpix/s, batched {out['batched_mpix_s']} Mpix/s "
         f"({out['speedup_vs_plain']}x)")
     return out
