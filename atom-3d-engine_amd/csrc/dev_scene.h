@@ -146,6 +146,9 @@ enum : uint32_t {
   SC_SS1 = 1u << 5,  // step_scale >= 1: enhanced sphere tracing with omega = 1.4
 };
 
+// Mask levels of the segment candidate grid (DevScene::grid_m).
+constexpr uint32_t kGridLevels = 8;
+
 struct alignas(16) DevScene {
   uint32_t n_obj, n_seg, n_mat, n_light;
   uint32_t max_steps, shadow_steps, ao_samples, bounces;
@@ -182,7 +185,7 @@ struct alignas(16) DevScene {
   // without subtract segments, where the accumulator never increases along the fold.
   float grid_lo[3], grid_inv;
   uint32_t grid_dims[3], grid_words;
-  float grid_m[6];
+  float grid_m[kGridLevels];
   float grid_kmax, grid_eps;
   // Box around the bounded segments' spheres (rounded outwards): a point outside the grid whose
   // distance to this box exceeds its threshold is farther than that from every bounded member.
@@ -191,8 +194,6 @@ struct alignas(16) DevScene {
   // Postfix CSG programs; a CSG object's p[0] / p[1] hold its first op and op count (bits).
   const DCsgOp* csg;
 };
-
-constexpr uint32_t kGridLevels = 6;
 
 // Per-frame camera record (mm3e-kit/src/camera.rs:39-47) plus the image size.
 // Table view used by the field walk: device addresses in HBM, or the same tables staged in LDS.

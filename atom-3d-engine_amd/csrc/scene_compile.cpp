@@ -32,7 +32,7 @@ static const uint32_t kMaxBloomRadius = 1u << 20;
 static const size_t kGridMinSegments = 8;
 static const double kGridTargetCells = 32768.0;
 static const double kGridMaxDim = 256.0;
-static const double kGridReach[kGridLevels] = {0.25, 0.75, 2.0, 4.0, 8.0, 16.0};
+static const double kGridReach[kGridLevels] = {0.25, 0.75, 1.5, 2.5, 4.0, 6.0, 10.0, 16.0};
 static const double kGridPadReach = 2.0;
 
 namespace {
