@@ -145,12 +145,16 @@ std::string emit_scene_source(const Blob& b, uint32_t maxb, bool sm, bool queued
   // Waves per SIMD the generated kernel is register-allocated for.  Nested path: 6 (at most 80
   // VGPRs); it is bound by dependency latency more than by issue (c2 960x540 at 4 / 5 / 6 / 7
   // waves: 1.59 / 1.38 / 1.32 / 1.34 ms; c1 and a CSG scene at 1280x720 also fastest at 6).
+  // Since the kernel holds one trace call site, 7 (at most 72) wins for short folds and loses for
+  // longer ones (alternating on one box, 6 against 7: c2's 9 objects 1.306 / 1.294 ms, a
+  // 13-object scene at 1280x720 2.111 / 2.084 ms, c1 level; 20 / 21 objects 2.18 / 2.27 and
+  // 4.98 / 5.11 ms): 7 up to 16 objects.
   // State-machine path: 4 (at most 128 VGPRs) -- its per-lane path state spills in the hot loop
   // below that (1280x720 procedural scenes of 13 / 21 / 41 objects at 4 / 5 / 6 waves:
   // 2.08 / 4.06 / 6.83, 5.40 / 6.38 / 7.61, 6.37 / 8.15 / 10.05 ms) -- except for long folds,
   // where latency hiding wins again (61 objects: 22.2 / 20.6 / 20.4 ms): 5 above 48 objects.
   // MM3E_GEN_WAVES overrides.
-  int waves = sm ? (b.objs.size() > 48 ? 5 : 4) : 6;
+  int waves = sm ? (b.objs.size() > 48 ? 5 : 4) : (b.objs.size() <= 16 ? 7 : 6);
   if (const char* wv = std::getenv("MM3E_GEN_WAVES")) {
     const int n = std::atoi(wv);
     if (n >= 1 && n <= 8) waves = n;

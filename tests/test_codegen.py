@@ -30,8 +30,11 @@ def _scenes():
 
 
 def test_generated_kernel_family():
-    """Up to 24 objects the generated kernel takes the nested path, segment grid or not; above,
-    the state machine, register-allocated for 4 waves per SIMD (5 above 48 objects)."""
+    """Up to 24 objects the generated kernel takes the nested path, segment grid or not,
+    register-allocated for 7 waves per SIMD up to 16 objects and 6 above; above 24, the state
+    machine, for 4 waves per SIMD (5 above 48 objects)."""
+    c2 = hip.emit_source(configs.c2_scene(width=64, height=36, gi_dims=None)[0].to_ir())
+    assert "render_ordered<GenMap, 1, false>" in c2 and "#define MM3E_RENDER_WAVES 7" in c2
     small = hip.emit_source(configs.procedural_scene(40, 20, 64, 36).to_ir())
     assert "grid_select" in small and "render_ordered<GenMap, 1, false>" in small
     assert "#define MM3E_RENDER_WAVES 6" in small
