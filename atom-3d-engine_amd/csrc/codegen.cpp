@@ -152,9 +152,12 @@ std::string emit_scene_source(const Blob& b, uint32_t maxb, bool sm, bool queued
   // State-machine path: 4 (at most 128 VGPRs) -- its per-lane path state spills in the hot loop
   // below that (1280x720 procedural scenes of 13 / 21 / 41 objects at 4 / 5 / 6 waves:
   // 2.08 / 4.06 / 6.83, 5.40 / 6.38 / 7.61, 6.37 / 8.15 / 10.05 ms) -- except for long folds,
-  // where latency hiding wins again (61 objects: 22.2 / 20.6 / 20.4 ms): 5 above 48 objects.
+  // where latency hiding wins again (61 objects: 22.2 / 20.6 / 20.4 ms): 5 above 48 objects,
+  // and 7 above 200: the 1000-object stress frame, alternating builds on one box each, at 5 / 6
+  // waves 708.5 / 649.9 and 709.1 / 653.6 ms, at 6 / 7 668.0 / 634.0 and 668.9 / 626.1 ms, at 7 /
+  // 8 631.7 / 666.1 and 638.2 / 667.3 ms (at 6 the 127-object scene is 4 % slower, 174 level).
   // MM3E_GEN_WAVES overrides.
-  int waves = sm ? (b.objs.size() > 48 ? 5 : 4) : (b.objs.size() <= 16 ? 7 : 6);
+  int waves = sm ? (b.objs.size() > 200 ? 7 : b.objs.size() > 48 ? 5 : 4) : (b.objs.size() <= 16 ? 7 : 6);
   if (const char* wv = std::getenv("MM3E_GEN_WAVES")) {
     const int n = std::atoi(wv);
     if (n >= 1 && n <= 8) waves = n;

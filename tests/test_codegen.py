@@ -32,7 +32,7 @@ def _scenes():
 def test_generated_kernel_family():
     """Up to 24 objects the generated kernel takes the nested path, segment grid or not,
     register-allocated for 7 waves per SIMD up to 16 objects and 6 above; above 24, the state
-    machine, for 4 waves per SIMD (5 above 48 objects)."""
+    machine, for 4 waves per SIMD (5 above 48 objects, 7 above 200)."""
     c2 = hip.emit_source(configs.c2_scene(width=64, height=36, gi_dims=None)[0].to_ir())
     assert "render_ordered<GenMap, 1, false>" in c2 and "#define MM3E_RENDER_WAVES 7" in c2
     small = hip.emit_source(configs.procedural_scene(40, 20, 64, 36).to_ir())
@@ -42,6 +42,8 @@ def test_generated_kernel_family():
     assert "render_ordered<GenMap, 1, true>" in mid and "#define MM3E_RENDER_WAVES 4" in mid
     big = hip.emit_source(configs.procedural_scene(41, 60, 64, 36).to_ir())
     assert "#define MM3E_RENDER_WAVES 5" in big
+    huge = hip.emit_source(configs.procedural_scene(43, 240, 64, 36).to_ir())
+    assert "render_ordered<GenMap, 1, true>" in huge and "#define MM3E_RENDER_WAVES 7" in huge
 
 
 def test_emitted_source_names_every_segment():
