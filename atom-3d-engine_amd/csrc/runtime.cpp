@@ -228,7 +228,7 @@ static void gen_worker(mm3e_renderer* r) {
 // / 117 / 127 / 174 / 184 objects: 7.4 / 11.8 / 1.9 / 12.1 / 1.8 / 36.1 / 53.4 / 4.2 ms against
 // 9.1 / 14.5 / 2.1 / 15.1 / 2.2 / 39.3 / 66.1 / 5.9 ms) and on the 1000-object stress frame
 // (777 against 844 ms at 3840x2160), once its grid masks are per lane and its sin/cos is
-// inline; the cap only bounds the compile (75 s for 1000 objects, on a background thread).
+// inline; the cap only bounds the compile (~45 s for 1000 objects, in a worker process).
 static const size_t kCodegenMaxObjects = 2000;
 size_t codegen_object_limit() {  // MM3E_CODEGEN_MAX_OBJECTS overrides (tuning)
   if (const char* v = std::getenv("MM3E_CODEGEN_MAX_OBJECTS")) {

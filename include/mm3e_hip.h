@@ -173,7 +173,7 @@ int mm3e_write_bmp(const uint8_t* rgba, uint32_t w, uint32_t h, uint8_t* out, si
  * at run time (hipRTC) into a render kernel for that scene.  The generated kernel performs the
  * same floating-point operations as the table-driven kernels, so images are identical; only
  * speed differs.  Scenes of more than 2000 objects keep the table kernels (the cap bounds the
- * run-time compile: about 75 s for 1000 objects; MM3E_CODEGEN_MAX_OBJECTS overrides).  Modes:
+ * run-time compile: about 45 s for 1000 objects; MM3E_CODEGEN_MAX_OBJECTS overrides).  Modes:
  * OFF (table kernels only), BACKGROUND (default: compile on worker
  * threads, the scene renders with the table kernels until its kernel is ready), SYNC (compile
  * inside mm3e_scene_compile).  The environment variable MM3E_CODEGEN=off|background|sync sets
