@@ -220,7 +220,7 @@ std::string emit_batch_source(const std::vector<const Blob*>& maps, uint32_t max
   // state-machine kernel for the longest map (MM3E_GEN_WAVES overrides)
   size_t most = 0;
   for (const Blob* m : maps) most = std::max(most, m->objs.size());
-  int waves = most > 48 ? 5 : 4;
+  int waves = most > 200 ? 7 : most > 48 ? 5 : 4;
   if (const char* wv = std::getenv("MM3E_GEN_WAVES")) {
     const int n = std::atoi(wv);
     if (n >= 1 && n <= 8) waves = n;
